@@ -1,0 +1,101 @@
+// Internal helpers shared by the HIP translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <hip/hip_bf16.h>
+#include <cstdint>
+#include <cstddef>
+#include "../../include/sptol_hip.h"
+#include "host_internal.h"
+
+// Map a hipError_t to the reference's device-error slot and record it.
+#define SPTOL_HIP(call, module)                                                        \
+    do {                                                                               \
+        hipError_t sptol_he_ = (call);                                                 \
+        if (sptol_he_ != hipSuccess) {                                                 \
+            sptol_complain((module), SPTERR_CUDA_ERROR + (int)sptol_he_, __FILE__,     \
+                           __LINE__, hipGetErrorString(sptol_he_));                    \
+            return SPTERR_CUDA_ERROR + (int)sptol_he_;                                 \
+        }                                                                              \
+    } while (0)
+
+// Propagate an SptError returned by an internal call.
+#define SPTOL_TRY(expr)                 \
+    do {                                \
+        int sptol_r_ = (expr);          \
+        if (sptol_r_) return sptol_r_;  \
+    } while (0)
+
+namespace sptol {
+
+// Budget-tracked device allocation (zero-size requests return nullptr).
+int dmalloc(void **ptr, size_t bytes, const char *module);
+void dfree(void *ptr);
+hipStream_t stream_of(void *s);
+int verbose();
+
+constexpr int kMaxModes = 8;
+
+// Device COO tensor: structure-of-arrays copy of the host tensor, plus
+// optional copies sorted by one mode.
+struct SortedCopy {
+    uint32_t *inds[kMaxModes] = {};
+    float *vals = nullptr;
+    bool ready = false;
+};
+
+}  // namespace sptol
+
+struct sptolDevCoo_s {
+    uint32_t nmodes = 0;
+    uint32_t ndims[sptol::kMaxModes] = {};
+    uint64_t nnz = 0;
+    uint32_t *inds[sptol::kMaxModes] = {};
+    float *vals = nullptr;
+    sptol::SortedCopy sorted[sptol::kMaxModes];
+};
+
+struct sptolDevHicoo_s {
+    uint32_t nmodes = 0;
+    uint32_t ndims[sptol::kMaxModes] = {};
+    uint64_t nnz = 0, nb = 0, nk = 0;
+    uint32_t sb_bits = 0, sk_bits = 0;
+    uint64_t *bptr = nullptr;                       // nb+1
+    uint32_t *binds[sptol::kMaxModes] = {};         // nb each
+    uint8_t *einds[sptol::kMaxModes] = {};          // nnz each
+    float *vals = nullptr;                          // nnz
+};
+
+namespace sptol {
+
+// Launch parameters for the MTTKRP kernels (passed by value).
+struct MttkrpArgs {
+    uint64_t nnz;
+    uint32_t nmodes;
+    uint32_t mode;
+    uint32_t R;            // columns computed
+    uint32_t stride;       // row stride of factors and output
+    uint32_t order[kMaxModes];      // order[1..nmodes-1]: Khatri-Rao order
+    const uint32_t *inds[kMaxModes];
+    const float *vals;
+    const void *mats[kMaxModes];    // device factor pointers (dtype per launch)
+    float *out;
+    uint64_t per_group;    // nonzeros per lane group
+    // HiCOO
+    const uint64_t *bptr;
+    const uint32_t *binds[kMaxModes];
+    const uint8_t *einds[kMaxModes];
+    uint64_t nb;
+    uint32_t sb_bits;
+};
+
+// Optional per-kernel timing with hipEvents recorded on the launch stream.
+// prof_start returns a token (or -1 when profiling is off); prof_stop closes it.
+int prof_start(const char *name, hipStream_t s);
+void prof_stop(int token, hipStream_t s);
+
+int launch_mttkrp_coo(const MttkrpArgs &a, int dtype, hipStream_t s);
+int launch_mttkrp_hicoo(const MttkrpArgs &a, int dtype, hipStream_t s);
+int launch_fill_zero(float *p, uint64_t n, hipStream_t s);
+
+}  // namespace sptol

@@ -1,0 +1,62 @@
+/* Internal declarations shared by the host C files and the HIP layer. */
+#ifndef SPTOL_HOST_INTERNAL_H
+#define SPTOL_HOST_INTERNAL_H
+
+#include "../../include/ParTI.h"
+#include <errno.h>
+#include <string.h>
+#include <stdlib.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Record an error in the thread-local last-error slot and print it (the
+ * reference's spt_ComplainError, src/error/error.c:46-71). */
+void sptol_complain(const char *module, int errcode, const char *file, unsigned line,
+                    const char *reason);
+
+#define SPTOL_CHECK(errcode, module, reason)                                   \
+    do {                                                                       \
+        int sptol_e_ = (errcode);                                              \
+        if (sptol_e_ != 0) {                                                   \
+            sptol_complain((module), sptol_e_, __FILE__, __LINE__, (reason));  \
+            return sptol_e_;                                                   \
+        }                                                                      \
+    } while (0)
+
+#define SPTOL_CHECK_OS(cond, module)                                           \
+    do {                                                                       \
+        if (cond) {                                                            \
+            int sptol_en_ = errno ? errno : ENOMEM;                            \
+            sptol_complain((module), SPTERR_OS_ERROR + sptol_en_, __FILE__,    \
+                           __LINE__, strerror(sptol_en_));                     \
+            return SPTERR_OS_ERROR + sptol_en_;                                \
+        }                                                                      \
+    } while (0)
+
+/* Seeded generator used by sptRandomizeMatrix (splitmix64 stream). */
+uint64_t sptol_rng_next(void);
+float    sptol_rng_uniform01(void);
+
+/* Stable LSD radix sort of (key, idx) pairs on the low `bits` bits of key.
+ * keys/idx are permuted in place; tmp buffers are allocated internally.
+ * Returns 0 or an OS error code. */
+int sptol_radix_sort_u64(uint64_t *keys, uint64_t *idx, uint64_t n, unsigned bits, int nthreads);
+
+/* Apply perm (new position i takes old element perm[i]) to tensor rows
+ * [begin, begin+n) of a COO tensor's index and value arrays. */
+int sptol_permute_coo(sptSparseTensor *tsr, uint64_t begin, const uint64_t *perm, uint64_t n);
+
+/* Number of bits needed to hold values in [0, n). */
+static inline unsigned sptol_bits_for(uint64_t n)
+{
+    unsigned b = 0;
+    while (b < 64 && (n > (1ull << b))) ++b;
+    return b;
+}
+
+#ifdef __cplusplus
+}
+#endif
+#endif

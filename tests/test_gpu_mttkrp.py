@@ -1,0 +1,194 @@
+"""GPU parity of COO and HiCOO MTTKRP through the C ABI.
+
+Bar: FP32 results within 1e-5 relative Frobenius error of the reference's
+sequential FP32 output (golden fixtures / bit-exact oracle restatement); the
+error against the FP64 oracle is asserted alongside.  Padding columns must
+be exactly zero.  BF16-stored factors are compared with the FP64 oracle run
+on the same BF16-rounded factors (FP32 accumulation bar 1e-5)."""
+import ctypes as C
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+from conftest import rel_fro
+from gpu_util import bf16_round, dev_coo, dev_mttkrp_coo, dev_mttkrp_hicoo
+from sptol_amd import gen
+from sptol_amd.parti import (CooTensor, Matrix, SPTOL_BF16, check, mats_array, ptr, sptIndex, to_hicoo)
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-5
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _dropin(lib, d, inds, vals, A, order, mode):
+    coo = CooTensor(d, inds, vals)
+    mats = [Matrix(a) for a in A] + [Matrix(nrows=int(max(d)), ncols=A[0].shape[1])]
+    o = np.ascontiguousarray(order, dtype=np.uint32)
+    check(lib, lib.sptMTTKRP(coo.ref(), mats_array(mats), ptr(o, sptIndex), mode), "sptMTTKRP")
+    return mats[-1].buf[: int(d[mode])].copy()
+
+
+def _check(got, ref32, ref64, R):
+    assert np.all(got[:, R:] == 0), "padding columns must be zero"
+    e32 = rel_fro(got[:, :R], ref32[:, :R])
+    e64 = rel_fro(got[:, :R], ref64[:, :R])
+    assert e32 <= TOL and e64 <= TOL, (e32, e64)
+    return e32, e64
+
+
+def test_dropin_coo_vs_reference_golden(gpu_lib, golden_coo):
+    g = golden_coo
+    for case in g.cases():
+        d, inds, vals = g.tensor(case)
+        R = int(g.get(case, "R"))
+        A = [g.get(case, f"A{m}") for m in range(len(d))]
+        for mode in range(len(d)):
+            for k in range(2):
+                _say("dropin coo", case, mode, k)
+                order = g.get(case, f"order{mode}_{k}")
+                got = _dropin(gpu_lib, d, inds, vals, A, order, mode)
+                ref64 = oracle.mttkrp_coo(inds, vals, A, order, mode, dtype=np.float64)
+                _check(got, g.get(case, f"out{mode}_{k}"), ref64, R)
+
+
+@pytest.mark.parametrize("prepare", [False, True])
+def test_device_coo_all_modes(gpu_lib, golden_coo, prepare):
+    g = golden_coo
+    for case in g.cases():
+        d, inds, vals = g.tensor(case)
+        R = int(g.get(case, "R"))
+        A = [g.get(case, f"A{m}") for m in range(len(d))]
+        h = dev_coo(gpu_lib, d, inds, vals)
+        for mode in range(len(d)):
+            _say("device coo", case, mode, prepare)
+            order = g.get(case, f"order{mode}_0")
+            got = dev_mttkrp_coo(gpu_lib, d, inds, vals, A, order, mode, prepare=prepare, h=h)
+            ref64 = oracle.mttkrp_coo(inds, vals, A, order, mode, dtype=np.float64)
+            _check(got, g.get(case, f"out{mode}_0"), ref64, R)
+        gpu_lib.sptolDevCooFree(h)
+
+
+@pytest.mark.parametrize("R", [1, 5, 8, 16, 32, 64, 100, 300])
+def test_ranks_and_padding(gpu_lib, R):
+    d, inds, vals = gen.zipf_tensor([400, 300, 500], 30000, 1.1, seed=R)
+    A = gen.uniform_factors(d, R, seed=R + 1)
+    for mode in range(3):
+        _say("rank", R, mode)
+        order = [mode, (mode + 2) % 3, (mode + 1) % 3]
+        ref32 = oracle.mttkrp_coo(inds, vals, A, order, mode)
+        ref64 = oracle.mttkrp_coo(inds, vals, A, order, mode, dtype=np.float64)
+        for prepare in (False, True):
+            got = dev_mttkrp_coo(gpu_lib, d, inds, vals, A, order, mode, prepare=prepare)
+            _check(got, ref32, ref64, R)
+
+
+def test_higher_order_and_edge_cases(gpu_lib):
+    # 5-D tensor (generic-order kernel path)
+    d, inds, vals = gen.uniform_tensor([20, 15, 10, 12, 9], 20000, seed=3)
+    A = gen.uniform_factors(d, 16, seed=4)
+    for mode in range(5):
+        _say("5d", mode)
+        order = [mode] + [(mode + i) % 5 for i in range(1, 5)]
+        got = dev_mttkrp_coo(gpu_lib, d, inds, vals, A, order, mode, prepare=(mode % 2 == 0))
+        _check(got, oracle.mttkrp_coo(inds, vals, A, order, mode),
+               oracle.mttkrp_coo(inds, vals, A, order, mode, dtype=np.float64), 16)
+    # 2-D tensor
+    d, inds, vals = gen.uniform_tensor([300, 200], 5000, seed=5)
+    A = gen.uniform_factors(d, 8, seed=6)
+    for mode in range(2):
+        got = dev_mttkrp_coo(gpu_lib, d, inds, vals, A, [mode, 1 - mode], mode)
+        _check(got, oracle.mttkrp_coo(inds, vals, A, [mode, 1 - mode], mode),
+               oracle.mttkrp_coo(inds, vals, A, [mode, 1 - mode], mode, dtype=np.float64), 8)
+    # empty tensor: the output is all zeros
+    d = np.array([10, 20, 30], np.uint32)
+    e = [np.zeros(0, np.uint32)] * 3
+    A = gen.uniform_factors(d, 16, seed=7)
+    got = _dropin(gpu_lib, d, e, np.zeros(0, np.float32), A, [1, 2, 0], 1)
+    assert got.shape == (20, 16) and np.all(got == 0)
+    # one nonzero, and many repeats of the same coordinate (unsorted)
+    one = [np.array([3], np.uint32), np.array([7], np.uint32), np.array([11], np.uint32)]
+    got = _dropin(gpu_lib, d, one, np.array([2.5], np.float32), A, [0, 1, 2], 0)
+    want = np.zeros((10, 16), np.float32)
+    want[3] = (np.float32(2.5) * A[1][7]) * A[2][11]
+    assert np.array_equal(got, want)
+    rng = np.random.default_rng(0)
+    rep = [np.full(5000, 4, np.uint32), rng.integers(0, 20, 5000).astype(np.uint32),
+           rng.integers(0, 30, 5000).astype(np.uint32)]
+    rv = rng.random(5000).astype(np.float32)
+    got = _dropin(gpu_lib, d, rep, rv, A, [0, 1, 2], 0)
+    _check(got, oracle.mttkrp_coo(rep, rv, A, [0, 1, 2], 0),
+           oracle.mttkrp_coo(rep, rv, A, [0, 1, 2], 0, dtype=np.float64), 16)
+
+
+def test_c1_full_size(gpu_lib):
+    """2000x1500x1000, 1e6 nnz, R=16, mode 0 (the reference driver's case)."""
+    d, inds, vals = gen.uniform_tensor([2000, 1500, 1000], 1_000_000, seed=2024)
+    A = gen.uniform_factors(d, 16, seed=2025)
+    order = [0, 1, 2]
+    ref32 = oracle.mttkrp_coo(inds, vals, A, order, 0)
+    ref64 = oracle.mttkrp_coo(inds, vals, A, order, 0, dtype=np.float64)
+    for prepare in (False, True):
+        _say("c1 full", prepare)
+        got = dev_mttkrp_coo(gpu_lib, d, inds, vals, A, order, 0, prepare=prepare)
+        e32, e64 = _check(got, ref32, ref64, 16)
+        _say(f"  rel err vs FP32 reference {e32:.2e}, vs FP64 oracle {e64:.2e}; "
+             f"reference FP32 vs FP64 {rel_fro(ref32, ref64):.2e}")
+
+
+def test_bf16_factors(gpu_lib):
+    d, inds, vals = gen.zipf_tensor([3000, 2000, 4000], 100000, 1.1, seed=9)
+    A = gen.uniform_factors(d, 32, seed=10)
+    Ab = [bf16_round(a) for a in A]
+    for mode in range(3):
+        _say("bf16", mode)
+        order = [mode, (mode + 1) % 3, (mode + 2) % 3]
+        ref64b = oracle.mttkrp_coo(inds, vals, Ab, order, mode, dtype=np.float64)
+        ref64 = oracle.mttkrp_coo(inds, vals, A, order, mode, dtype=np.float64)
+        for prepare in (False, True):
+            got = dev_mttkrp_coo(gpu_lib, d, inds, vals, A, order, mode, prepare=prepare, dtype=SPTOL_BF16)
+            assert rel_fro(got[:, :32], ref64b[:, :32]) <= TOL
+            assert rel_fro(got[:, :32], ref64[:, :32]) <= 1e-2
+
+
+def test_dropin_hicoo_vs_reference_golden(gpu_lib, golden_hicoo):
+    g = golden_hicoo
+    for key in g.cases():
+        d, inds, vals = g.tensor(key, prefix="in_ind", vals="in_vals")
+        sb, sk = (int(x) for x in g.get(key, "sb_sk"))
+        coo = CooTensor(d, [a.copy() for a in inds], vals.copy())
+        h, _ = to_hicoo(gpu_lib, coo, sb, sk)
+        A = [g.get(key, f"hA{m}") for m in range(len(d))]
+        for mode in range(len(d)):
+            _say("dropin hicoo", key, mode)
+            order = g.get(key, f"horder{mode}")
+            mats = [Matrix(a) for a in A] + [Matrix(nrows=int(max(d)), ncols=8)]
+            check(gpu_lib, gpu_lib.sptMTTKRPHiCOO(C.byref(h), mats_array(mats), ptr(order, sptIndex), mode),
+                  "sptMTTKRPHiCOO")
+            got = mats[-1].buf[: int(d[mode])]
+            ref64 = oracle.mttkrp_coo(coo.inds, coo.vals, A, order, mode, dtype=np.float64)
+            _check(got, g.get(key, f"hout{mode}"), ref64, 8)
+        gpu_lib.sptFreeSparseTensorHiCOO(C.byref(h))
+
+
+@pytest.mark.parametrize("R,dtype", [(32, 0), (16, 0), (32, 1), (8, 0)])
+def test_device_hicoo_zipf(gpu_lib, R, dtype):
+    d, inds, vals = gen.zipf_tensor([120000, 90000, 280000], 300000, 1.1, seed=R + dtype)
+    coo = CooTensor(d, [a.copy() for a in inds], vals.copy())
+    h, _ = to_hicoo(gpu_lib, coo, 7, 14)
+    o = oracle.Hicoo(d, inds, vals, 7, 14)
+    A = gen.uniform_factors(d, R, seed=5)
+    Au = [bf16_round(a) for a in A] if dtype else A
+    for mode in range(3):
+        _say("device hicoo", R, dtype, mode)
+        order = [mode, (mode + 1) % 3, (mode + 2) % 3]
+        got = dev_mttkrp_hicoo(gpu_lib, h, d, A, order, mode, dtype=dtype)
+        ref32 = o.mttkrp(Au, order, mode)
+        ref64 = o.mttkrp(Au, order, mode, dtype=np.float64)
+        _check(got, ref32, ref64, R)
+    gpu_lib.sptFreeSparseTensorHiCOO(C.byref(h))

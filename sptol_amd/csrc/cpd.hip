@@ -1,0 +1,684 @@
+// CPD-ALS with tensor and factors resident on the GPU.
+//
+// Follows the reference loop src/sptensor/cpd.c:32-137 step for step:
+//   for it: for m: M = MTTKRP_m(X, A) with mats_order = [m, m+1, ...] mod N
+//               A_m = M V^{-1},  V = Hadamard_{p != m} A_p^T A_p   (matrix_solver.c:32-83)
+//               lambda = column 2-norms (it == 0) or max(column max, 1) (cpd.c:99-103)
+//               A_m /= lambda; ata[m] = A_m^T A_m
+//           fit = 1 - sqrt(||X||^2 + ||K||^2 - 2<X,K>) / ||X||          (ktensor.c:102-118)
+//   final: every A_m renormalised by its column 2-norms, folded into lambda (matrix.c:557-573)
+// Differences, all deliberate:
+//   * Gram matrices, the Hadamard product, the Cholesky factorisation, the
+//     row solves, the norms and the fit are FP64 (the reference uses FP32
+//     LAPACK); factors stay FP32, optionally with a BF16 copy for the MTTKRP
+//     gathers.
+//   * ||K||^2 uses the full symmetric Hadamard of the Grams; the reference
+//     reads an unwritten triangle (ktensor.c:151).
+//   * Reductions are two-stage with a fixed block count, so the solve, norm
+//     and fit steps are bitwise reproducible for a given MTTKRP result.
+#include "dev_common.hpp"
+#include <vector>
+#include <cmath>
+#include <cstring>
+#include <cstdio>
+#include <chrono>
+
+namespace sptol {
+
+constexpr int kRedBlocks = 240;   // fixed block count of the two-stage reductions
+constexpr int kMaxRank = 64;
+
+// ---------------------------------------------------------------------------
+// Kernels
+// ---------------------------------------------------------------------------
+
+// partial[blk][a*R+b] = sum over this block's rows of A[i][a]*A[i][b] (FP64).
+__global__ __launch_bounds__(256) void k_gram_partial(const float *__restrict__ A, uint32_t I, uint32_t R,
+                                                      uint32_t S, double *__restrict__ partial)
+{
+    __shared__ float tile[16][kMaxRank];
+    const uint32_t RR = R * R;
+    double acc[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) acc[q] = 0.0;
+    const uint64_t rows_per = ((uint64_t)I + gridDim.x - 1) / gridDim.x;
+    const uint64_t r0 = (uint64_t)blockIdx.x * rows_per;
+    const uint64_t r1 = min<uint64_t>(r0 + rows_per, I);
+    for (uint64_t base = r0; base < r1; base += 16) {
+        for (uint32_t t = threadIdx.x; t < 16 * R; t += 256) {
+            const uint32_t rr = t / R, c = t % R;
+            tile[rr][c] = (base + rr < r1) ? A[(base + rr) * S + c] : 0.0f;
+        }
+        __syncthreads();
+        const uint32_t nrow = (uint32_t)min<uint64_t>(16, r1 - base);
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const uint32_t e = threadIdx.x + 256u * q;
+            if (e < RR) {
+                const uint32_t a = e / R, b = e % R;
+                double s = acc[q];
+                for (uint32_t rr = 0; rr < nrow; ++rr) s += (double)tile[rr][a] * (double)tile[rr][b];
+                acc[q] = s;
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+        const uint32_t e = threadIdx.x + 256u * q;
+        if (e < RR) partial[(size_t)blockIdx.x * RR + e] = acc[q];
+    }
+}
+
+// out[e] = sum_blk partial[blk][e], summed in block order.
+__global__ void k_sum_partials(const double *__restrict__ partial, uint32_t nblk, uint32_t n, double *__restrict__ out)
+{
+    const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    double s = 0.0;
+    for (uint32_t b = 0; b < nblk; ++b) s += partial[(size_t)b * n + e];
+    out[e] = s;
+}
+
+// Column reduction partials: kind 0 = sum of squares, 1 = max (from 0).
+__global__ __launch_bounds__(256) void k_col_partial(const float *__restrict__ A, uint32_t I, uint32_t R, uint32_t S,
+                                                     int kind, double *__restrict__ partial)
+{
+    __shared__ double red[256];
+    const uint32_t lanes_per_col = 256 / R;          // R <= 64 so >= 4
+    const uint32_t c = threadIdx.x % R;
+    const uint32_t sub = threadIdx.x / R;
+    const bool active = sub < lanes_per_col;
+    const uint64_t rows_per = ((uint64_t)I + gridDim.x - 1) / gridDim.x;
+    const uint64_t r0 = (uint64_t)blockIdx.x * rows_per;
+    const uint64_t r1 = min<uint64_t>(r0 + rows_per, I);
+    double v = 0.0;
+    if (active) {
+        for (uint64_t i = r0 + sub; i < r1; i += lanes_per_col) {
+            const double x = (double)A[i * S + c];
+            if (kind == 0) v += x * x;
+            else v = x > v ? x : v;
+        }
+    }
+    red[threadIdx.x] = v;
+    __syncthreads();
+    if (threadIdx.x < R) {
+        double s = red[threadIdx.x];
+        for (uint32_t k = 1; k < lanes_per_col; ++k) {
+            const double y = red[threadIdx.x + k * R];
+            if (kind == 0) s += y;
+            else s = y > s ? y : s;
+        }
+        partial[(size_t)blockIdx.x * R + threadIdx.x] = s;
+    }
+}
+
+// lambda[c] from the partials: kind 0 -> sqrt(sum), 1 -> max(max, 1).
+// mult != 0 multiplies the existing lambda by the new norm instead
+// (GetFinalLambda); the norms are also written to norms[].
+__global__ void k_col_final(const double *__restrict__ partial, uint32_t nblk, uint32_t R, int kind, int mult,
+                            double *__restrict__ lambda, double *__restrict__ norms)
+{
+    const uint32_t c = threadIdx.x;
+    if (c >= R) return;
+    double s = 0.0;
+    for (uint32_t b = 0; b < nblk; ++b) {
+        const double y = partial[(size_t)b * R + c];
+        if (kind == 0) s += y;
+        else s = y > s ? y : s;
+    }
+    double nrm = kind == 0 ? sqrt(s) : (s < 1.0 ? 1.0 : s);
+    norms[c] = nrm;
+    lambda[c] = mult ? lambda[c] * nrm : nrm;
+}
+
+// A[i][c] /= norms[c] (FP64 divide, FP32 store); optional BF16 copy.
+__global__ void k_scale(float *__restrict__ A, uint16_t *__restrict__ Abf, uint32_t I, uint32_t R, uint32_t S,
+                        const double *__restrict__ norms)
+{
+    const uint64_t n = (uint64_t)I * S;
+    uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t step = (uint64_t)gridDim.x * blockDim.x;
+    for (; e < n; e += step) {
+        const uint32_t c = (uint32_t)(e % S);
+        float x = A[e];
+        if (c < R) {
+            x = (float)((double)x / norms[c]);
+            A[e] = x;
+        }
+        if (Abf) {
+            uint32_t u = __float_as_uint(x);
+            u += 0x7FFFu + ((u >> 16) & 1u);
+            Abf[e] = (uint16_t)(u >> 16);
+        }
+    }
+}
+
+// V = Hadamard of ata[p] for p = (m+k) % N, k = 1..N-1; then a Cholesky
+// factorisation V = L L^T.  If V is not positive definite, an LU
+// factorisation with partial pivoting of V replaces it (the gesv fallback
+// of matrix_solver.c:66-79).  Output: fac (R*R), piv (R), *kind = 0 (L) / 1 (LU).
+__global__ __launch_bounds__(256) void k_factor(const double *__restrict__ ata, uint32_t N, uint32_t m, uint32_t R,
+                                                double *__restrict__ fac, int *__restrict__ piv, int *__restrict__ kind)
+{
+    __shared__ double V[kMaxRank * kMaxRank];
+    __shared__ double W[kMaxRank * kMaxRank];
+    __shared__ int spd;
+    const uint32_t RR = R * R;
+    for (uint32_t e = threadIdx.x; e < RR; e += blockDim.x) {
+        double v = 1.0;
+        for (uint32_t k = 1; k < N; ++k) v *= ata[(size_t)((m + k) % N) * RR + e];
+        V[e] = v;
+        W[e] = v;
+    }
+    if (threadIdx.x == 0) spd = 1;
+    __syncthreads();
+    // Cholesky, column by column (lower triangle of W)
+    for (uint32_t j = 0; j < R; ++j) {
+        if (threadIdx.x == 0) {
+            double d = W[j * R + j];
+            for (uint32_t k = 0; k < j; ++k) d -= W[j * R + k] * W[j * R + k];
+            if (!(d > 0.0)) spd = 0;
+            W[j * R + j] = d > 0.0 ? sqrt(d) : 1.0;
+        }
+        __syncthreads();
+        if (!spd) break;
+        for (uint32_t i = j + 1 + threadIdx.x; i < R; i += blockDim.x) {
+            double s = W[i * R + j];
+            for (uint32_t k = 0; k < j; ++k) s -= W[i * R + k] * W[j * R + k];
+            W[i * R + j] = s / W[j * R + j];
+        }
+        __syncthreads();
+    }
+    __syncthreads();
+    if (spd) {
+        for (uint32_t e = threadIdx.x; e < RR; e += blockDim.x) {
+            const uint32_t i = e / R, k = e % R;
+            fac[e] = k <= i ? W[e] : 0.0;
+        }
+        if (threadIdx.x == 0) *kind = 0;
+        return;
+    }
+    // LU with partial pivoting on V (in place): at step j swap row j with the
+    // pivot row, then each thread eliminates one row below j.
+    __shared__ uint32_t p;
+    for (uint32_t j = 0; j < R; ++j) {
+        if (threadIdx.x == 0) {
+            uint32_t best = j;
+            for (uint32_t i = j + 1; i < R; ++i)
+                if (fabs(V[i * R + j]) > fabs(V[best * R + j])) best = i;
+            p = best;
+            piv[j] = (int)best;
+        }
+        __syncthreads();
+        if (p != j)
+            for (uint32_t k = threadIdx.x; k < R; k += blockDim.x) {
+                const double t = V[j * R + k];
+                V[j * R + k] = V[p * R + k];
+                V[p * R + k] = t;
+            }
+        __syncthreads();
+        for (uint32_t i = j + 1 + threadIdx.x; i < R; i += blockDim.x) {
+            const double l = V[i * R + j] / V[j * R + j];
+            V[i * R + j] = l;
+            for (uint32_t k = j + 1; k < R; ++k) V[i * R + k] -= l * V[j * R + k];
+        }
+        __syncthreads();
+    }
+    for (uint32_t e = threadIdx.x; e < RR; e += blockDim.x) fac[e] = V[e];
+    if (threadIdx.x == 0) *kind = 1;
+}
+
+// A[i] = M[i] V^{-1} for every row, one thread per row, FP64 arithmetic.
+template <int RM>
+__global__ __launch_bounds__(256) void k_solve_rows(const float *__restrict__ M, float *__restrict__ A, uint32_t I,
+                                                    uint32_t R, uint32_t S, const double *__restrict__ fac,
+                                                    const int *__restrict__ piv, const int *__restrict__ kind)
+{
+    __shared__ double F[RM * RM];
+    __shared__ int pv[RM];
+    for (uint32_t e = threadIdx.x; e < R * R; e += blockDim.x) F[(e / R) * RM + e % R] = fac[e];
+    for (uint32_t e = threadIdx.x; e < R; e += blockDim.x) pv[e] = piv[e];
+    __syncthreads();
+    const int lu = *kind;
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= I) return;
+    double x[RM];
+#pragma unroll
+    for (int a = 0; a < RM; ++a) x[a] = a < (int)R ? (double)M[i * S + a] : 0.0;
+    if (!lu) {
+        // L y = x, then L^T z = y
+#pragma unroll
+        for (int a = 0; a < RM; ++a) {
+            if (a < (int)R) {
+                double s = x[a];
+#pragma unroll
+                for (int k = 0; k < a; ++k) s -= F[a * RM + k] * x[k];
+                x[a] = s / F[a * RM + a];
+            }
+        }
+#pragma unroll
+        for (int a = RM - 1; a >= 0; --a) {
+            if (a < (int)R) {
+                double s = x[a];
+#pragma unroll
+                for (int k = a + 1; k < RM; ++k)
+                    if (k < (int)R) s -= F[k * RM + a] * x[k];
+                x[a] = s / F[a * RM + a];
+            }
+        }
+    } else {
+        // V^T = V: solve V z = x with P V = L U
+#pragma unroll
+        for (int a = 0; a < RM; ++a) {
+            if (a < (int)R) {
+                const int p = pv[a];
+                // swap x[a], x[p] (p >= a); done with a select loop to keep x in registers
+                double xa = x[a], xp = 0.0;
+#pragma unroll
+                for (int k = 0; k < RM; ++k) if (k == p) xp = x[k];
+#pragma unroll
+                for (int k = 0; k < RM; ++k) if (k == p) x[k] = xa;
+                x[a] = xp;
+            }
+        }
+#pragma unroll
+        for (int a = 0; a < RM; ++a) {
+            if (a < (int)R) {
+                double s = x[a];
+#pragma unroll
+                for (int k = 0; k < a; ++k) s -= F[a * RM + k] * x[k];
+                x[a] = s;
+            }
+        }
+#pragma unroll
+        for (int a = RM - 1; a >= 0; --a) {
+            if (a < (int)R) {
+                double s = x[a];
+#pragma unroll
+                for (int k = a + 1; k < RM; ++k)
+                    if (k < (int)R) s -= F[a * RM + k] * x[k];
+                x[a] = s / F[a * RM + a];
+            }
+        }
+    }
+#pragma unroll
+    for (int a = 0; a < RM; ++a)
+        if (a < (int)R) A[i * S + a] = (float)x[a];
+}
+
+// partial[blk][r] = sum over this block's rows of A[i][r] * M[i][r].
+__global__ __launch_bounds__(256) void k_inner_partial(const float *__restrict__ A, const float *__restrict__ M, uint32_t I,
+                                                       uint32_t R, uint32_t S, double *__restrict__ partial)
+{
+    __shared__ double red[256];
+    const uint32_t lanes_per_col = 256 / R;
+    const uint32_t c = threadIdx.x % R;
+    const uint32_t sub = threadIdx.x / R;
+    const uint64_t rows_per = ((uint64_t)I + gridDim.x - 1) / gridDim.x;
+    const uint64_t r0 = (uint64_t)blockIdx.x * rows_per;
+    const uint64_t r1 = min<uint64_t>(r0 + rows_per, I);
+    double v = 0.0;
+    if (sub < lanes_per_col)
+        for (uint64_t i = r0 + sub; i < r1; i += lanes_per_col) v += (double)A[i * S + c] * (double)M[i * S + c];
+    red[threadIdx.x] = v;
+    __syncthreads();
+    if (threadIdx.x < R) {
+        double s = red[threadIdx.x];
+        for (uint32_t k = 1; k < lanes_per_col; ++k) s += red[threadIdx.x + k * R];
+        partial[(size_t)blockIdx.x * R + threadIdx.x] = s;
+    }
+}
+
+// Sum of squares of the values (||X||^2), two-stage.
+__global__ __launch_bounds__(256) void k_sqnorm_partial(const float *__restrict__ v, uint64_t n, double *__restrict__ partial)
+{
+    __shared__ double red[256];
+    const uint64_t per = (n + gridDim.x - 1) / gridDim.x;
+    const uint64_t a = (uint64_t)blockIdx.x * per, b = min<uint64_t>(a + per, n);
+    double s = 0.0;
+    for (uint64_t i = a + threadIdx.x; i < b; i += 256) s += (double)v[i] * (double)v[i];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (uint32_t w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+// fit from ||X||^2, the Grams, lambda and the inner-product partials.
+__global__ __launch_bounds__(256) void k_fit(const double *__restrict__ ata, uint32_t N, uint32_t R,
+                                             const double *__restrict__ lambda, const double *__restrict__ inner_partial,
+                                             uint32_t nblk, const double *__restrict__ normx, double *__restrict__ fit)
+{
+    __shared__ double red[256];
+    const uint32_t RR = R * R;
+    double s = 0.0;
+    for (uint32_t e = threadIdx.x; e < RR; e += 256) {
+        double g = 1.0;
+        for (uint32_t m = 0; m < N; ++m) g *= ata[(size_t)m * RR + e];
+        s += g * lambda[e / R] * lambda[e % R];
+    }
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (uint32_t w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const double normk = fabs(red[0]);
+        double inner = 0.0;
+        for (uint32_t r = 0; r < R; ++r) {
+            double acc = 0.0;
+            for (uint32_t b = 0; b < nblk; ++b) acc += inner_partial[(size_t)b * R + r];
+            inner += acc * lambda[r];
+        }
+        double residual = *normx + normk - 2.0 * inner;
+        if (residual > 0.0) residual = sqrt(residual);
+        *fit = 1.0 - residual / sqrt(*normx);
+    }
+}
+
+__global__ void k_set(double *p, uint32_t n, double v)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+static unsigned grid_rows(uint64_t n, uint32_t per_block)
+{
+    uint64_t b = (n + per_block - 1) / per_block;
+    if (b < 1) b = 1;
+    if (b > 65535) b = 65535;
+    return (unsigned)b;
+}
+
+// ---------------------------------------------------------------------------
+// Workspace and host loop
+// ---------------------------------------------------------------------------
+struct CpdWork {
+    float *M = nullptr;            // MTTKRP output, maxI x S
+    uint16_t *bf[kMaxModes] = {};  // BF16 factor copies
+    double *ata = nullptr;         // N x R x R
+    double *partial = nullptr;     // kRedBlocks x R x R
+    double *fac = nullptr;         // R x R
+    int *piv = nullptr;            // R
+    int *kind = nullptr;           // 1
+    double *lambda = nullptr;      // R
+    double *norms = nullptr;       // R
+    double *normx = nullptr;       // 1
+    double *fit = nullptr;         // 1
+    ~CpdWork()
+    {
+        dfree(M);
+        for (auto p : bf) dfree(p);
+        dfree(ata); dfree(partial); dfree(fac); dfree(piv); dfree(kind);
+        dfree(lambda); dfree(norms); dfree(normx); dfree(fit);
+    }
+};
+
+static int gram(const float *A, uint32_t I, uint32_t R, uint32_t S, double *out, double *partial, hipStream_t s)
+{
+    int tok = prof_start("cpd_gram", s);
+    hipLaunchKernelGGL(k_gram_partial, dim3(kRedBlocks), dim3(256), 0, s, A, I, R, S, partial);
+    hipLaunchKernelGGL(k_sum_partials, dim3((R * R + 255) / 256), dim3(256), 0, s, partial, (uint32_t)kRedBlocks,
+                       R * R, out);
+    prof_stop(tok, s);
+    SPTOL_HIP(hipGetLastError(), "CPD gram");
+    return 0;
+}
+
+static int colnorm(float *A, uint16_t *Abf, uint32_t I, uint32_t R, uint32_t S, int kind, int mult, CpdWork &w,
+                   hipStream_t s)
+{
+    int tok = prof_start("cpd_norm", s);
+    hipLaunchKernelGGL(k_col_partial, dim3(kRedBlocks), dim3(256), 0, s, A, I, R, S, kind, w.partial);
+    hipLaunchKernelGGL(k_col_final, dim3(1), dim3(kMaxRank), 0, s, w.partial, (uint32_t)kRedBlocks, R, kind, mult,
+                       w.lambda, w.norms);
+    hipLaunchKernelGGL(k_scale, dim3(grid_rows((uint64_t)I * S, 256)), dim3(256), 0, s, A, Abf, I, R, S, w.norms);
+    prof_stop(tok, s);
+    SPTOL_HIP(hipGetLastError(), "CPD norm");
+    return 0;
+}
+
+static int solve(const float *M, float *A, uint32_t I, uint32_t R, uint32_t S, CpdWork &w, uint32_t N, uint32_t m,
+                 hipStream_t s)
+{
+    int tok = prof_start("cpd_solve", s);
+    hipLaunchKernelGGL(k_factor, dim3(1), dim3(256), 0, s, w.ata, N, m, R, w.fac, w.piv, w.kind);
+    const unsigned g = grid_rows(I, 256);
+    if (R <= 8) hipLaunchKernelGGL(k_solve_rows<8>, dim3(g), dim3(256), 0, s, M, A, I, R, S, w.fac, w.piv, w.kind);
+    else if (R <= 16) hipLaunchKernelGGL(k_solve_rows<16>, dim3(g), dim3(256), 0, s, M, A, I, R, S, w.fac, w.piv, w.kind);
+    else if (R <= 32) hipLaunchKernelGGL(k_solve_rows<32>, dim3(g), dim3(256), 0, s, M, A, I, R, S, w.fac, w.piv, w.kind);
+    else hipLaunchKernelGGL(k_solve_rows<64>, dim3(g), dim3(256), 0, s, M, A, I, R, S, w.fac, w.piv, w.kind);
+    prof_stop(tok, s);
+    SPTOL_HIP(hipGetLastError(), "CPD solve");
+    return 0;
+}
+
+}  // namespace sptol
+
+using namespace sptol;
+
+extern "C" {
+
+int sptolCpdAlsDevice(sptolDevCoo X, sptIndex rank, sptIndex niters, double tol, sptolDevMatrix *mats,
+                      sptValue *lambda_out, sptolCpdStats *stats, void *stream)
+{
+    const uint32_t N = X->nmodes, R = rank;
+    if (R < 1 || R > (uint32_t)kMaxRank) SPTOL_CHECK(SPTERR_VALUE_ERROR, "CPD", "rank must be in [1, 64]");
+    const int fdt = stats ? stats->factor_dtype : SPTOL_F32;
+    if (fdt != SPTOL_F32 && fdt != SPTOL_BF16) SPTOL_CHECK(SPTERR_VALUE_ERROR, "CPD", "bad factor dtype");
+    uint32_t maxI = 0;
+    for (uint32_t m = 0; m < N; ++m) {
+        if (mats[m].dtype != SPTOL_F32 || mats[m].ncols != R || mats[m].nrows != X->ndims[m])
+            SPTOL_CHECK(SPTERR_SHAPE_MISMATCH, "CPD", "factors must be FP32 ndims[m] x rank");
+        maxI = X->ndims[m] > maxI ? X->ndims[m] : maxI;
+    }
+    const uint32_t S = mats[0].stride;
+    hipStream_t s = stream_of(stream);
+    auto t0 = std::chrono::steady_clock::now();
+    for (uint32_t m = 0; m < N; ++m) SPTOL_TRY(sptolDevCooPrepareMode(X, m));
+
+    CpdWork w;
+    SPTOL_TRY(dmalloc((void **)&w.M, (size_t)(maxI ? maxI : 1) * S * 4, "CPD"));
+    SPTOL_TRY(dmalloc((void **)&w.ata, (size_t)N * R * R * 8, "CPD"));
+    SPTOL_TRY(dmalloc((void **)&w.partial, (size_t)kRedBlocks * R * R * 8, "CPD"));
+    SPTOL_TRY(dmalloc((void **)&w.fac, (size_t)R * R * 8, "CPD"));
+    SPTOL_TRY(dmalloc((void **)&w.piv, (size_t)R * 4, "CPD"));
+    SPTOL_TRY(dmalloc((void **)&w.kind, 4, "CPD"));
+    SPTOL_TRY(dmalloc((void **)&w.lambda, (size_t)R * 8, "CPD"));
+    SPTOL_TRY(dmalloc((void **)&w.norms, (size_t)R * 8, "CPD"));
+    SPTOL_TRY(dmalloc((void **)&w.normx, 8, "CPD"));
+    SPTOL_TRY(dmalloc((void **)&w.fit, 8, "CPD"));
+    if (fdt == SPTOL_BF16)
+        for (uint32_t m = 0; m < N; ++m)
+            SPTOL_TRY(dmalloc((void **)&w.bf[m], (size_t)(X->ndims[m] ? X->ndims[m] : 1) * S * 2, "CPD"));
+
+    sptolDevMatrix mt[kMaxModes];    // what the MTTKRP reads (FP32 or the BF16 copies)
+    for (uint32_t m = 0; m < N; ++m) {
+        mt[m] = mats[m];
+        if (fdt == SPTOL_BF16) { mt[m].dtype = SPTOL_BF16; mt[m].values = w.bf[m]; }
+    }
+    sptolDevMatrix Mout;
+    Mout.nrows = maxI; Mout.ncols = R; Mout.stride = S; Mout.dtype = SPTOL_F32; Mout.values = w.M;
+
+    // ||X||^2 and the initial Grams (cpd.c:65-70)
+    hipLaunchKernelGGL(k_sqnorm_partial, dim3(kRedBlocks), dim3(256), 0, s, X->vals, X->nnz, w.partial);
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, s, w.partial, (uint32_t)kRedBlocks, 1u, w.normx);
+    hipLaunchKernelGGL(k_set, dim3(1), dim3(kMaxRank), 0, s, w.lambda, R, 1.0);
+    SPTOL_HIP(hipGetLastError(), "CPD init");
+    for (uint32_t m = 0; m < N; ++m) {
+        SPTOL_TRY(gram((const float *)mats[m].values, X->ndims[m], R, S, w.ata + (size_t)m * R * R, w.partial, s));
+        if (fdt == SPTOL_BF16) {
+            hipLaunchKernelGGL(k_set, dim3(1), dim3(kMaxRank), 0, s, w.norms, R, 1.0);
+            hipLaunchKernelGGL(k_scale, dim3(grid_rows((uint64_t)X->ndims[m] * S, 256)), dim3(256), 0, s,
+                               (float *)mats[m].values, w.bf[m], X->ndims[m], R, S, w.norms);
+        }
+    }
+    double fit = 0.0, oldfit = 0.0;
+    double *h_fit = nullptr;
+    SPTOL_HIP(hipHostMalloc((void **)&h_fit, sizeof(double), hipHostMallocDefault), "CPD");
+    uint32_t order[kMaxModes];
+    int iters = 0;
+    int rc = 0;
+    for (uint32_t it = 0; it < niters && !rc; ++it) {
+        auto ti = std::chrono::steady_clock::now();
+        for (uint32_t m = 0; m < N && !rc; ++m) {
+            const uint32_t I = X->ndims[m];
+            order[0] = m;
+            for (uint32_t k = 1; k < N; ++k) order[k] = (m + k) % N;
+            Mout.nrows = I;
+            rc = sptolDevMTTKRP(X, mt, order, m, &Mout, stream);
+            if (rc) break;
+            rc = solve(w.M, (float *)mats[m].values, I, R, S, w, N, m, s);
+            if (rc) break;
+            rc = colnorm((float *)mats[m].values, fdt == SPTOL_BF16 ? w.bf[m] : nullptr, I, R, S, it == 0 ? 0 : 1, 0,
+                         w, s);
+            if (rc) break;
+            rc = gram((const float *)mats[m].values, I, R, S, w.ata + (size_t)m * R * R, w.partial, s);
+        }
+        if (rc) break;
+        // fit: the last mode's MTTKRP result is still in w.M
+        const uint32_t last = N - 1;
+        hipLaunchKernelGGL(k_inner_partial, dim3(kRedBlocks), dim3(256), 0, s, (const float *)mats[last].values, w.M,
+                           X->ndims[last], R, S, w.partial);
+        hipLaunchKernelGGL(k_fit, dim3(1), dim3(256), 0, s, w.ata, N, R, w.lambda, w.partial, (uint32_t)kRedBlocks,
+                           w.normx, w.fit);
+        hipError_t e = hipMemcpyAsync(h_fit, w.fit, sizeof(double), hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e != hipSuccess) {
+            sptol_complain("CPD", SPTERR_CUDA_ERROR + (int)e, __FILE__, __LINE__, hipGetErrorString(e));
+            rc = SPTERR_CUDA_ERROR + (int)e;
+            break;
+        }
+        fit = *h_fit;
+        iters = (int)it + 1;
+        if (verbose()) {
+            double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - ti).count();
+            printf("  its = %u ( %.3lf s ) fit = %0.5f  delta = %+0.4e\n", it + 1, sec, fit, fit - oldfit);
+        }
+        if (it > 0 && std::fabs(fit - oldfit) < tol) break;
+        oldfit = fit;
+    }
+    (void)hipHostFree(h_fit);
+    if (rc) return rc;
+    // GetFinalLambda (matrix.c:557-573)
+    for (uint32_t m = 0; m < N; ++m)
+        SPTOL_TRY(colnorm((float *)mats[m].values, nullptr, X->ndims[m], R, S, 0, 1, w, s));
+    std::vector<double> lam(R);
+    SPTOL_HIP(hipMemcpyAsync(lam.data(), w.lambda, R * 8, hipMemcpyDeviceToHost, s), "CPD");
+    SPTOL_HIP(hipStreamSynchronize(s), "CPD");
+    for (uint32_t r = 0; r < R; ++r) lambda_out[r] = (sptValue)lam[r];
+    if (stats) {
+        stats->iterations = iters;
+        stats->fit = fit;
+        stats->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    }
+    return 0;
+}
+
+// The reference's exported ALS body (cpd.c:32-137) with caller-supplied
+// factors: mats[0..N-1] in/out, mats[N] receives the last mode's MTTKRP.
+double CpdAlsStep(sptSparseTensor const * const spten, sptIndex const rank, sptIndex const niters,
+                  double const tol, sptMatrix ** mats, sptValue * const lambda)
+{
+    const uint32_t N = spten->nmodes;
+    for (uint32_t m = 0; m < N; ++m)
+        if (spten->ndims[m] != mats[m]->nrows || mats[m]->ncols != rank) {
+            sptol_complain("CPD", SPTERR_SHAPE_MISMATCH, __FILE__, __LINE__, "factor shape");
+            return NAN;
+        }
+    sptolDevCoo X = nullptr;
+    if (sptolDevCooCreate(&X, spten)) return NAN;
+    sptolDevMatrix dm[kMaxModes];
+    std::memset(dm, 0, sizeof(dm));
+    int r = 0;
+    for (uint32_t m = 0; m < N && !r; ++m) {
+        r = sptolDevMatrixCreate(&dm[m], mats[m]->nrows, rank, SPTOL_F32);
+        if (!r) r = sptolDevMatrixUpload(&dm[m], mats[m]);
+    }
+    sptolCpdStats st;
+    std::memset(&st, 0, sizeof(st));
+    st.factor_dtype = SPTOL_F32;
+    if (!r) r = sptolCpdAlsDevice(X, rank, niters, tol, dm, lambda, &st, nullptr);
+    for (uint32_t m = 0; m < N && !r; ++m) r = sptolDevMatrixDownload(mats[m], &dm[m]);
+    // mats[N]: the last mode's MTTKRP with the final factors, as the
+    // reference leaves its scratch matrix
+    if (!r && mats[N] && mats[N]->cap >= spten->ndims[N - 1]) {
+        sptolDevMatrix out;
+        r = sptolDevMatrixCreate(&out, spten->ndims[N - 1], rank, SPTOL_F32);
+        uint32_t order[kMaxModes];
+        order[0] = N - 1;
+        for (uint32_t k = 1; k < N; ++k) order[k] = (N - 1 + k) % N;
+        if (!r) r = sptolDevMTTKRP(X, dm, order, N - 1, &out, nullptr);
+        if (!r) r = sptolHipStreamSynchronize(nullptr);
+        if (!r) {
+            sptIndex keep = mats[N]->nrows;
+            mats[N]->nrows = spten->ndims[N - 1];
+            r = sptolDevMatrixDownload(mats[N], &out);
+            mats[N]->nrows = keep;
+        }
+        sptolDevMatrixFree(&out);
+    }
+    for (uint32_t m = 0; m < N; ++m) sptolDevMatrixFree(&dm[m]);
+    sptolDevCooFree(X);
+    return r ? NAN : st.fit;
+}
+
+int sptCpdAls(sptSparseTensor const * const spten, sptIndex const rank, sptIndex const niters, double const tol,
+              sptKruskalTensor * ktensor)
+{
+    const uint32_t N = spten->nmodes;
+    if (N < 2 || N > (uint32_t)kMaxModes) SPTOL_CHECK(SPTERR_VALUE_ERROR, "CPU SpTns CPD-ALS", "nmodes must be in [2, 8]");
+    if (!ktensor->lambda) SPTOL_CHECK(SPTERR_VALUE_ERROR, "CPU SpTns CPD-ALS", "ktensor not created by sptNewKruskalTensor");
+    sptIndex max_dim = sptMaxIndexArray(spten->ndims, N);
+    sptMatrix **mats = (sptMatrix **)calloc(N + 1, sizeof(sptMatrix *));
+    SPTOL_CHECK_OS(!mats, "CPU SpTns CPD-ALS");
+    int r = 0;
+    for (uint32_t m = 0; m <= N && !r; ++m) {
+        mats[m] = (sptMatrix *)malloc(sizeof(sptMatrix));
+        if (!mats[m]) { r = SPTERR_OS_ERROR + ENOMEM; break; }
+        if (m < N) {
+            r = sptNewMatrix(mats[m], spten->ndims[m], rank);
+            if (!r) r = sptRandomizeMatrix(mats[m], spten->ndims[m], rank);
+        } else {
+            r = sptNewMatrix(mats[m], max_dim, rank);
+            if (!r) r = sptConstantMatrix(mats[m], 0);
+        }
+    }
+    if (!r) {
+        auto t0 = std::chrono::steady_clock::now();
+        double fit = CpdAlsStep(spten, rank, niters, tol, mats, ktensor->lambda);
+        if (std::isnan(fit)) r = sptGetLastError(nullptr, nullptr, nullptr, nullptr);
+        if (!r && std::isnan(fit)) r = SPTERR_UNKNOWN;
+        ktensor->fit = fit;
+        if (verbose())
+            fprintf(stdout, "[GPU  SpTns CPD-ALS]: %.9lf s\n",
+                    std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count());
+    }
+    if (r) {
+        for (uint32_t m = 0; m <= N; ++m)
+            if (mats[m]) { sptFreeMatrix(mats[m]); free(mats[m]); }
+        free(mats);
+        return r;
+    }
+    if (ktensor->factors) {
+        for (uint32_t m = 0; m <= ktensor->nmodes; ++m)
+            if (ktensor->factors[m]) { sptFreeMatrix(ktensor->factors[m]); free(ktensor->factors[m]); }
+        free(ktensor->factors);
+    }
+    ktensor->factors = mats;   // owned by the Kruskal tensor (freed by sptFreeKruskalTensor)
+    return 0;
+}
+
+int sptOmpCpdAls(sptSparseTensor const * const spten, sptIndex const rank, sptIndex const niters, double const tol,
+                 const int tk, const int use_reduce, sptKruskalTensor * ktensor)
+{
+    (void)tk;
+    (void)use_reduce;
+    return sptCpdAls(spten, rank, niters, tol, ktensor);
+}
+
+}  // extern "C"

@@ -1,0 +1,140 @@
+"""GPU parity of CPD-ALS through the C ABI.
+
+Against the compiled reference's CpdAlsStep (golden factors/lambda after a
+fixed iteration count from committed initial factors, FP32 LAPACK): relative
+Frobenius error within 2e-3 (the reference's FP32 solve is the looser side,
+see the oracle test of the same bound).  Against the FP64 oracle on the same
+inputs: factors and lambda within 1e-4, fit within 1e-6 (FP32 factors and
+MTTKRP, FP64 solve/norms/fit).  BF16-stored factors: fit within 1e-4 of the
+FP64 oracle on a planted low-rank tensor."""
+import ctypes as C
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+from conftest import rel_fro
+from sptol_amd import gen
+from sptol_amd.parti import (CooTensor, Matrix, SPTOL_BF16, SPTOL_F32, check, mats_array, ptr,
+                             sptKruskalTensor, sptIndex, sptolCpdStats, sptolDevMatrix)
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print(*a, flush=True)
+
+
+def _cpd_step(lib, d, inds, vals, init, iters, tol=-1.0):
+    R = init[0].shape[1]
+    coo = CooTensor(d, inds, vals)
+    mats = [Matrix(a) for a in init] + [Matrix(nrows=int(max(d)), ncols=R)]
+    lam = np.zeros(R, np.float32)
+    fit = lib.CpdAlsStep(coo.ref(), R, iters, tol, mats_array(mats), ptr(lam, C.c_float))
+    assert np.isfinite(fit)
+    return [m.array().copy() for m in mats[:-1]], lam, fit, mats[-1]
+
+
+def test_cpd_step_vs_reference_and_oracle(gpu_lib, golden_cpd):
+    g = golden_cpd
+    for case in g.cases():
+        d, inds, vals = g.tensor(case)
+        R = int(g.get(case, "R"))
+        iters = int(g.get(case, "iters"))
+        init = [g.get(case, f"init{m}") for m in range(len(d))]
+        _say("cpd", case)
+        A, lam, fit, _ = _cpd_step(gpu_lib, d, inds, vals, init, iters)
+        om, olam, ofits = oracle.cpd_als(d, inds, vals, init, iters)
+        assert rel_fro(lam, g.get(case, "lambda")) < 2e-3, case
+        assert rel_fro(lam, olam) < 1e-4, case
+        for m in range(len(d)):
+            assert rel_fro(A[m], g.get(case, f"A{m}")) < 2e-3, (case, m)
+            assert rel_fro(A[m], om[m]) < 1e-4, (case, m)
+        assert abs(fit - ofits[-1]) < 1e-6, (case, fit, ofits[-1])
+
+
+def test_sptcpdals_dropin(gpu_lib):
+    d, inds, vals = gen.uniform_tensor([60, 50, 40], 8000, seed=31)
+    R, iters = 8, 6
+    gpu_lib.sptolSetSeed(1234)
+    kt = sptKruskalTensor()
+    nd = np.asarray(d, np.uint32)
+    gpu_lib.sptNewKruskalTensor.argtypes = [C.POINTER(sptKruskalTensor), sptIndex, C.POINTER(sptIndex), sptIndex]
+    gpu_lib.sptFreeKruskalTensor.argtypes = [C.POINTER(sptKruskalTensor)]
+    gpu_lib.sptRandomizeMatrix.argtypes = [C.c_void_p, sptIndex, sptIndex]
+    check(gpu_lib, gpu_lib.sptNewKruskalTensor(C.byref(kt), 3, ptr(nd, sptIndex), R), "new ktensor")
+    coo = CooTensor(d, inds, vals)
+    check(gpu_lib, gpu_lib.sptCpdAls(coo.ref(), R, iters, 0.0, C.byref(kt)), "sptCpdAls")
+    # the same seeded initial factors, regenerated
+    gpu_lib.sptolSetSeed(1234)
+    init = []
+    for m in range(3):
+        mm = Matrix(nrows=int(d[m]), ncols=R)
+        gpu_lib.sptRandomizeMatrix(C.byref(mm.m), int(d[m]), R)
+        init.append(mm.array().copy())
+    assert all(np.all((a >= 0) & (a < 1)) for a in init)
+    om, olam, ofits = oracle.cpd_als(d, inds, vals, init, iters)
+    lam = np.ctypeslib.as_array(kt.lambda_, shape=(R,)).copy()
+    assert rel_fro(lam, olam) < 1e-4
+    for m in range(3):
+        f = kt.factors[m].contents
+        got = np.ctypeslib.as_array(f.values, shape=(f.nrows, f.stride))[:, :R]
+        assert rel_fro(got, om[m]) < 1e-4
+    assert abs(kt.fit - ofits[-1]) < 1e-6
+    gpu_lib.sptFreeKruskalTensor(C.byref(kt))
+
+
+def _device_cpd(lib, d, inds, vals, init, iters, tol, dtype):
+    R = init[0].shape[1]
+    coo = CooTensor(d, inds, vals)
+    h = C.c_void_p()
+    check(lib, lib.sptolDevCooCreate(C.byref(h), coo.ref()), "DevCooCreate")
+    dm = (sptolDevMatrix * len(d))()
+    hm = [Matrix(a) for a in init]
+    for m in range(len(d)):
+        check(lib, lib.sptolDevMatrixCreate(C.byref(dm[m]), int(d[m]), R, SPTOL_F32), "create")
+        check(lib, lib.sptolDevMatrixUpload(C.byref(dm[m]), C.byref(hm[m].m)), "upload")
+    lam = np.zeros(R, np.float32)
+    st = sptolCpdStats()
+    st.factor_dtype = dtype
+    check(lib, lib.sptolCpdAlsDevice(h, R, iters, tol, dm, ptr(lam, C.c_float), C.byref(st), None), "CpdAlsDevice")
+    out = []
+    for m in range(len(d)):
+        check(lib, lib.sptolDevMatrixDownload(C.byref(hm[m].m), C.byref(dm[m])), "download")
+        out.append(hm[m].array().copy())
+        lib.sptolDevMatrixFree(C.byref(dm[m]))
+    lib.sptolDevCooFree(h)
+    return out, lam, st
+
+
+@pytest.mark.parametrize("R", [16, 32])
+def test_planted_fp32_and_bf16(gpu_lib, R):
+    d, inds, vals, planted = gen.planted_tensor([500, 400, 300, 60], 200000, R, 1.05, seed=R)
+    init = gen.uniform_factors(d, R, seed=R + 1)
+    iters = 20
+    om, olam, ofits = oracle.cpd_als(d, inds, vals, init, iters)
+    for dtype in (SPTOL_F32, SPTOL_BF16):
+        _say("planted", R, dtype)
+        A, lam, st = _device_cpd(gpu_lib, d, inds, vals, init, iters, -1.0, dtype)
+        assert st.iterations == iters
+        _say(f"  fit gpu {st.fit:.6f} oracle {ofits[-1]:.6f} diff {abs(st.fit - ofits[-1]):.2e}")
+        assert abs(st.fit - ofits[-1]) < 1e-4
+        if dtype == SPTOL_F32:
+            assert abs(st.fit - ofits[-1]) < 1e-5
+
+
+def test_tolerance_stops_early(gpu_lib):
+    d, inds, vals, _ = gen.planted_tensor([80, 70, 60], 20000, 4, 1.1, seed=5)
+    init = gen.uniform_factors(d, 4, seed=6)
+    _, _, st = _device_cpd(gpu_lib, d, inds, vals, init, 200, 1e-4, SPTOL_F32)
+    _, _, ofits = oracle.cpd_als(d, inds, vals, init, 200, tol=1e-4)
+    assert st.iterations == len(ofits)
+    assert st.iterations < 200
+
+
+def test_cpd_rejects_bad_rank(gpu_lib):
+    d, inds, vals = gen.uniform_tensor([20, 20, 20], 500, seed=1)
+    init = gen.uniform_factors(d, 65, seed=2)
+    with pytest.raises(RuntimeError):
+        _device_cpd(gpu_lib, d, inds, vals, init, 2, -1.0, SPTOL_F32)

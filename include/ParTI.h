@@ -249,6 +249,17 @@ int sptMTTKRPHiCOO_MatrixTiling(sptSparseTensorHiCOO const * const hitsr, sptRan
         sptIndex const mats_order[], sptIndex const mode);                                  /* hicoo/mttkrp.c:158 */
 int sptOmpMTTKRPHiCOO(sptSparseTensorHiCOO const * const hitsr, sptMatrix * mats[],
         sptIndex const mats_order[], sptIndex const mode, const int nt);                    /* hicoo/mttkrp_omp.c:110 */
+/* OpenMP-named variants keep their signatures; thread counts, private copies
+ * and the balanced flag are ignored because the work runs on the GPU. */
+int sptOmpMTTKRP_Reduce(sptSparseTensor const * const X, sptMatrix * mats[], sptMatrix * copy_mats[],
+        sptIndex const mats_order[], sptIndex const mode, const int tk);                    /* mttkrp_omp.c:401 */
+int sptOmpMTTKRPHiCOO_MatrixTiling(sptSparseTensorHiCOO const * const hitsr, sptRankMatrix * mats[],
+        sptIndex const mats_order[], sptIndex const mode, const int nt);                    /* hicoo/mttkrp_omp.c:134 */
+int sptOmpMTTKRPHiCOO_MatrixTiling_Scheduled(sptSparseTensorHiCOO const * const hitsr, sptRankMatrix * mats[],
+        sptIndex const mats_order[], sptIndex const mode, const int tk, int balanced);       /* hicoo/mttkrp_omp.c:158 */
+int sptOmpMTTKRPHiCOO_MatrixTiling_Scheduled_Reduce(sptSparseTensorHiCOO const * const hitsr,
+        sptRankMatrix * mats[], sptRankMatrix * copy_mats[], sptIndex const mats_order[],
+        sptIndex const mode, const int tk, int balanced);                                   /* hicoo/mttkrp_omp.c:190 */
 
 /* ---------------------------------------------------------------------- */
 /* Kruskal tensors and CPD-ALS (src/ktensor/ktensor.c, src/sptensor/cpd.c) */

@@ -106,6 +106,36 @@ int  sptolDevMTTKRPHiCOO(sptolDevHicoo X, const sptolDevMatrix *mats, const sptI
                          sptIndex mode, sptolDevMatrix *out, void *stream);
 
 /* ---------------------------------------------------------------------- */
+/* Batched MTTKRP: many device tensors in one persistent launch           */
+/* ---------------------------------------------------------------------- */
+typedef struct sptolBatch_s *sptolBatch;
+
+int  sptolBatchCreate(sptolBatch *b);
+/* Queue one MTTKRP (3-D or 4-D tensor, stride <= 256, all factors of a batch
+ * share one dtype).  Uses the tensor's mode-sorted copy when prepared.  The
+ * handles and matrices must outlive the batch. */
+int  sptolBatchAdd(sptolBatch b, sptolDevCoo X, const sptolDevMatrix *mats, const sptIndex mats_order[],
+                   sptIndex mode, sptolDevMatrix *out);
+/* Tile length multiplier (base: 131072/stride nonzeros per tile) and
+ * resident blocks per CU of the persistent kernel. */
+int  sptolBatchSetConfig(sptolBatch b, double tile_scale, int blocks_per_cu);
+int  sptolBatchGetConfig(sptolBatch b, double *tile_scale, int *blocks_per_cu);
+/* Time candidate configurations on this batch and keep the fastest; results
+ * are cached per batch signature for the life of the process. */
+int  sptolBatchAutotune(sptolBatch b, int reps, void *stream);
+/* Record the fill + persistent launch into a hipGraph on `stream` (a created
+ * stream, not NULL); sptolBatchRun on that stream then replays the graph. */
+int  sptolBatchCapture(sptolBatch b, void *stream);
+/* Zero every output and run all queued MTTKRPs (asynchronous on `stream`). */
+int  sptolBatchRun(sptolBatch b, void *stream);
+int  sptolBatchSize(sptolBatch b);
+void sptolBatchFree(sptolBatch b);
+
+/* Streams at the default priority (hipStreamCreate). */
+int  sptolHipStreamCreate(void **stream);
+int  sptolHipStreamDestroy(void *stream);
+
+/* ---------------------------------------------------------------------- */
 /* Device-resident CPD-ALS                                                */
 /* ---------------------------------------------------------------------- */
 typedef struct {

@@ -1,0 +1,429 @@
+// Batched MTTKRP: many independent COO tensors in one persistent launch.
+//
+// The reference runs one tensor per call and copies it to the device each
+// time (src/sptensor/mttkrp_cuda_onekernel.cu:100-149).  Here a batch of
+// device-resident tensors (each with its own factors, rank, mode and output)
+// is cut into tiles of consecutive nonzeros; one persistent kernel, sized to
+// the chip, pulls tiles from a global work queue (one atomic per tile), so
+// small and large tensors and different ranks share the launch without idle
+// CUs.  A preceding fill kernel zeroes every output and resets the queue, and
+// the pair can be captured once into a hipGraph and replayed.
+//
+// The tile length per rank class (nonzeros per tile) and the number of
+// resident blocks per CU come from an autotuner that times candidate
+// configurations on the batch itself and caches the winner per batch
+// signature.
+#include "dev_common.hpp"
+#include <vector>
+#include <map>
+#include <algorithm>
+#include <cstring>
+#include <cmath>
+#include <chrono>
+
+namespace sptol {
+
+struct BatchDesc {
+    uint64_t nnz;
+    uint64_t tile_nnz;         // nonzeros per tile (multiple of 64)
+    const uint32_t *inds[4];
+    const float *vals;
+    const void *mats[4];
+    float *out;
+    uint64_t out_len;          // floats in the output (rows * stride)
+    uint32_t nmodes, mode, R, stride;
+    uint32_t order[4];
+    uint32_t lpn;              // lanes per nonzero = stride / 4
+    uint32_t pad;
+};
+
+// Zero every output and reset the queue counter.  Work is split in chunks
+// of 64K floats; chunk c of the batch belongs to tensor t with
+// fill_prefix[t] <= c < fill_prefix[t+1].
+__global__ __launch_bounds__(256) void k_batch_fill(const BatchDesc *__restrict__ d, const uint64_t *__restrict__ fill_prefix,
+                                                    uint32_t ntensors, uint64_t nchunks, uint32_t *__restrict__ counter)
+{
+    if (blockIdx.x == 0 && threadIdx.x == 0) *counter = 0u;
+    for (uint64_t c = blockIdx.x; c < nchunks; c += gridDim.x) {
+        uint32_t lo = 0, hi = ntensors;
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (fill_prefix[mid] <= c) lo = mid; else hi = mid;
+        }
+        const uint64_t base = (c - fill_prefix[lo]) * 65536ull;
+        const uint64_t n = d[lo].out_len;
+        float4 *o = reinterpret_cast<float4 *>(d[lo].out + base);
+        const uint64_t n4 = (min<uint64_t>(base + 65536ull, n) - base) / 4;
+        for (uint64_t i = threadIdx.x; i < n4; i += 256) o[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+}
+
+__device__ __forceinline__ float4 bld4(const float *p) { return *reinterpret_cast<const float4 *>(p); }
+__device__ __forceinline__ float4 bld4(const uint16_t *p)
+{
+    const uint2 u = *reinterpret_cast<const uint2 *>(p);
+    return make_float4(__uint_as_float(u.x << 16), __uint_as_float(u.x & 0xFFFF0000u),
+                       __uint_as_float(u.y << 16), __uint_as_float(u.y & 0xFFFF0000u));
+}
+
+__device__ __forceinline__ void bflush(float *out, uint32_t row, uint32_t S, uint32_t c0, uint32_t R, const float4 a)
+{
+    float *o = out + (size_t)row * S + c0;
+    if (c0 + 0 < R) unsafeAtomicAdd(o + 0, a.x);
+    if (c0 + 1 < R) unsafeAtomicAdd(o + 1, a.y);
+    if (c0 + 2 < R) unsafeAtomicAdd(o + 2, a.z);
+    if (c0 + 3 < R) unsafeAtomicAdd(o + 3, a.w);
+}
+
+// Persistent kernel.  A wave takes one tile at a time from the queue; its
+// 64/lpn lane groups each walk a contiguous slice of the tile, summing
+// products of equal output rows in registers (as the single-tensor kernel).
+//
+// Every loop below has a wave-uniform trip count and per-lane work is
+// predicated, never skipped with a divergent continue or early loop exit:
+// the queue pull at the top relies on all 64 lanes arriving there together
+// (lane 0 takes the tile, readfirstlane broadcasts it).
+template <typename FT>
+__global__ __launch_bounds__(256) void k_batch_mttkrp(const BatchDesc *__restrict__ d,
+                                                      const uint64_t *__restrict__ tile_prefix, uint32_t ntensors,
+                                                      uint64_t ntiles, uint32_t *__restrict__ counter)
+{
+    const uint32_t lane = threadIdx.x & 63u;
+    for (;;) {
+        uint32_t tile = 0;
+        if (lane == 0) tile = atomicAdd(counter, 1u);
+        tile = __builtin_amdgcn_readfirstlane(tile);
+        if (tile >= ntiles) return;
+        uint32_t lo = 0, hi = ntensors;
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (tile_prefix[mid] <= tile) lo = mid; else hi = mid;
+        }
+        const BatchDesc &t = d[lo];
+        const uint32_t lpn = t.lpn, S = t.stride, R = t.R;
+        const uint32_t groups = 64u / lpn;
+        const uint32_t g = lane / lpn, l = lane % lpn;
+        const uint64_t tz0 = (tile - tile_prefix[lo]) * t.tile_nnz;
+        const uint64_t tz1 = min<uint64_t>(tz0 + t.tile_nnz, t.nnz);
+        const uint64_t slice = (t.tile_nnz + groups - 1) / groups;   // uniform
+        const uint64_t z0 = tz0 + (uint64_t)g * slice;
+        const uint64_t z1 = min<uint64_t>(z0 + slice, tz1);
+        const uint32_t c0 = l * 4;                                    // < S always
+        const bool lane_on = c0 < R;
+        const uint32_t *__restrict__ rowi = t.inds[t.mode];
+        const uint32_t o1 = t.order[1], o2 = t.order[2];
+        const uint32_t *__restrict__ i1 = t.inds[o1];
+        const uint32_t *__restrict__ i2 = t.inds[o2];
+        const FT *__restrict__ A1 = (const FT *)t.mats[o1];
+        const FT *__restrict__ A2 = (const FT *)t.mats[o2];
+        const bool four = t.nmodes == 4;
+        const uint32_t o3 = four ? t.order[3] : o2;
+        const uint32_t *__restrict__ i3 = t.inds[o3];
+        const FT *__restrict__ A3 = (const FT *)t.mats[o3];
+        float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+        uint32_t cur = 0xFFFFFFFFu;
+        constexpr int U = 4;
+        for (uint64_t k = 0; k < slice; k += U) {
+            uint32_t row[U];
+            float4 p[U];
+            bool valid[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const uint64_t z = z0 + k + u;
+                valid[u] = lane_on && z < z1;
+                const uint64_t zz = valid[u] ? z : tz0;   // tz0 < nnz: a safe index
+                row[u] = rowi[zz];
+                const float v = t.vals[zz];
+                float4 a = bld4(A1 + (size_t)i1[zz] * S + c0);
+                const float4 b = bld4(A2 + (size_t)i2[zz] * S + c0);
+                a.x = v * a.x; a.y = v * a.y; a.z = v * a.z; a.w = v * a.w;
+                a.x *= b.x; a.y *= b.y; a.z *= b.z; a.w *= b.w;
+                if (four) {
+                    const float4 c = bld4(A3 + (size_t)i3[zz] * S + c0);
+                    a.x *= c.x; a.y *= c.y; a.z *= c.z; a.w *= c.w;
+                }
+                p[u] = a;
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                if (valid[u]) {
+                    if (row[u] != cur) {
+                        if (cur != 0xFFFFFFFFu) bflush(t.out, cur, S, c0, R, acc);
+                        cur = row[u];
+                        acc = p[u];
+                    } else {
+                        acc.x += p[u].x; acc.y += p[u].y; acc.z += p[u].z; acc.w += p[u].w;
+                    }
+                }
+            }
+        }
+        if (cur != 0xFFFFFFFFu) bflush(t.out, cur, S, c0, R, acc);
+    }
+}
+
+struct TuneCfg {
+    double tile_scale = 1.0;     // multiplies the base tile length
+    int blocks_per_cu = 8;
+};
+
+static std::map<std::string, TuneCfg> g_tune_cache;
+
+static int cu_count()
+{
+    int dev = 0, n = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 256;
+    if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0) return 256;
+    return n;
+}
+
+}  // namespace sptol
+
+using namespace sptol;
+
+struct sptolBatch_s {
+    std::vector<BatchDesc> descs;
+    int dtype = -1;
+    TuneCfg cfg;
+    bool planned = false;
+    BatchDesc *d_desc = nullptr;
+    uint64_t *d_tile_prefix = nullptr, *d_fill_prefix = nullptr;
+    uint32_t *d_counter = nullptr;
+    uint64_t ntiles = 0, nchunks = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipStream_t graph_stream = nullptr;
+    ~sptolBatch_s()
+    {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        dfree(d_desc); dfree(d_tile_prefix); dfree(d_fill_prefix); dfree(d_counter);
+    }
+};
+
+namespace sptol {
+
+// Base tile: about 2^20 * 4 / stride rank-columns of work, i.e. fewer
+// nonzeros per tile for wide rows, scaled by the tuned factor.
+static uint64_t tile_len(uint32_t stride, double scale)
+{
+    double base = 8192.0 * 16.0 / (double)stride * scale;
+    uint64_t t = (uint64_t)base;
+    t = (t + 63) / 64 * 64;
+    return t < 64 ? 64 : t;
+}
+
+static int build_plan(sptolBatch b)
+{
+    const size_t n = b->descs.size();
+    std::vector<uint64_t> tp(n + 1, 0), fp(n + 1, 0);
+    for (size_t i = 0; i < n; ++i) {
+        BatchDesc &d = b->descs[i];
+        d.tile_nnz = tile_len(d.stride, b->cfg.tile_scale);
+        tp[i + 1] = tp[i] + (d.nnz + d.tile_nnz - 1) / d.tile_nnz;
+        fp[i + 1] = fp[i] + (d.out_len + 65535) / 65536;
+    }
+    if (tp[n] >= 0xFFFFFFFFull) SPTOL_CHECK(SPTERR_VALUE_ERROR, "Batch Plan", "too many tiles");
+    b->ntiles = tp[n];
+    b->nchunks = fp[n];
+    dfree(b->d_desc); dfree(b->d_tile_prefix); dfree(b->d_fill_prefix);
+    b->d_desc = nullptr; b->d_tile_prefix = nullptr; b->d_fill_prefix = nullptr;
+    SPTOL_TRY(sptolHipDuplicateMemory((void **)&b->d_desc, b->descs.data(), n * sizeof(BatchDesc)));
+    SPTOL_TRY(sptolHipDuplicateMemory((void **)&b->d_tile_prefix, tp.data(), (n + 1) * 8));
+    SPTOL_TRY(sptolHipDuplicateMemory((void **)&b->d_fill_prefix, fp.data(), (n + 1) * 8));
+    if (!b->d_counter) SPTOL_TRY(dmalloc((void **)&b->d_counter, 4, "Batch Plan"));
+    if (b->exec) { (void)hipGraphExecDestroy(b->exec); b->exec = nullptr; }
+    if (b->graph) { (void)hipGraphDestroy(b->graph); b->graph = nullptr; }
+    b->planned = true;
+    return 0;
+}
+
+static int launch_batch(sptolBatch b, hipStream_t s)
+{
+    const uint32_t n = (uint32_t)b->descs.size();
+    const int cus = cu_count();
+    unsigned fill_grid = (unsigned)std::min<uint64_t>(std::max<uint64_t>(b->nchunks, 1), (uint64_t)cus * 8);
+    int tok = prof_start("batch_fill", s);
+    hipLaunchKernelGGL(k_batch_fill, dim3(fill_grid), dim3(256), 0, s, b->d_desc, b->d_fill_prefix, n, b->nchunks,
+                       b->d_counter);
+    prof_stop(tok, s);
+    const unsigned grid = (unsigned)(cus * b->cfg.blocks_per_cu);
+    tok = prof_start("batch_mttkrp", s);
+    if (b->dtype == SPTOL_BF16)
+        hipLaunchKernelGGL(k_batch_mttkrp<uint16_t>, dim3(grid), dim3(256), 0, s, b->d_desc, b->d_tile_prefix, n,
+                           b->ntiles, b->d_counter);
+    else
+        hipLaunchKernelGGL(k_batch_mttkrp<float>, dim3(grid), dim3(256), 0, s, b->d_desc, b->d_tile_prefix, n,
+                           b->ntiles, b->d_counter);
+    prof_stop(tok, s);
+    SPTOL_HIP(hipGetLastError(), "Batch launch");
+    return 0;
+}
+
+static std::string signature(sptolBatch b)
+{
+    // rank mix and log2 total nonzeros per rank class
+    std::map<uint32_t, uint64_t> by;
+    for (auto &d : b->descs) by[d.stride] += d.nnz;
+    std::string s = std::to_string(b->dtype);
+    for (auto &kv : by) s += "/" + std::to_string(kv.first) + ":" + std::to_string((int)std::log2((double)kv.second + 1));
+    return s;
+}
+
+}  // namespace sptol
+
+extern "C" {
+
+int sptolBatchCreate(sptolBatch *b)
+{
+    *b = new sptolBatch_s();
+    return 0;
+}
+
+int sptolBatchAdd(sptolBatch b, sptolDevCoo X, const sptolDevMatrix *mats, const sptIndex mats_order[], sptIndex mode,
+                  sptolDevMatrix *out)
+{
+    if (X->nmodes != 3 && X->nmodes != 4) SPTOL_CHECK(SPTERR_VALUE_ERROR, "Batch Add", "batched MTTKRP takes 3-D or 4-D tensors");
+    if (mode >= X->nmodes) SPTOL_CHECK(SPTERR_VALUE_ERROR, "Batch Add", "mode out of range");
+    const uint32_t S = out->stride;
+    if (S > 256 || out->dtype != SPTOL_F32) SPTOL_CHECK(SPTERR_VALUE_ERROR, "Batch Add", "stride must be <= 256, output FP32");
+    for (uint32_t m = 0; m < X->nmodes; ++m) {
+        if (mats[m].nrows != X->ndims[m] || mats[m].stride != S || mats[m].ncols != out->ncols)
+            SPTOL_CHECK(SPTERR_SHAPE_MISMATCH, "Batch Add", "factor shape");
+        if (b->dtype >= 0 && mats[m].dtype != b->dtype) SPTOL_CHECK(SPTERR_VALUE_ERROR, "Batch Add", "mixed factor dtypes");
+    }
+    if (out->nrows < X->ndims[mode]) SPTOL_CHECK(SPTERR_SHAPE_MISMATCH, "Batch Add", "output rows");
+    uint32_t seen = 0;
+    for (uint32_t k = 1; k < X->nmodes; ++k) {
+        if (mats_order[k] >= X->nmodes || mats_order[k] == mode || ((seen >> mats_order[k]) & 1u))
+            SPTOL_CHECK(SPTERR_VALUE_ERROR, "Batch Add", "mats_order[1..N-1] must list the other modes");
+        seen |= 1u << mats_order[k];
+    }
+    b->dtype = mats[0].dtype;
+    BatchDesc d;
+    std::memset(&d, 0, sizeof(d));
+    d.nnz = X->nnz;
+    const SortedCopy &sc = X->sorted[mode];
+    for (uint32_t m = 0; m < X->nmodes; ++m) {
+        d.inds[m] = sc.ready ? sc.inds[m] : X->inds[m];
+        d.mats[m] = mats[m].values;
+        d.order[m] = mats_order[m];
+    }
+    d.vals = sc.ready ? sc.vals : X->vals;
+    d.out = (float *)out->values;
+    d.out_len = (uint64_t)X->ndims[mode] * S;
+    d.nmodes = X->nmodes;
+    d.mode = mode;
+    d.R = out->ncols;
+    d.stride = S;
+    d.lpn = S / 4;
+    b->descs.push_back(d);
+    b->planned = false;
+    return 0;
+}
+
+int sptolBatchSetConfig(sptolBatch b, double tile_scale, int blocks_per_cu)
+{
+    if (!(tile_scale > 0) || blocks_per_cu < 1 || blocks_per_cu > 32)
+        SPTOL_CHECK(SPTERR_VALUE_ERROR, "Batch Config", "tile_scale > 0, blocks_per_cu in [1, 32]");
+    b->cfg.tile_scale = tile_scale;
+    b->cfg.blocks_per_cu = blocks_per_cu;
+    return build_plan(b);
+}
+
+int sptolBatchGetConfig(sptolBatch b, double *tile_scale, int *blocks_per_cu)
+{
+    *tile_scale = b->cfg.tile_scale;
+    *blocks_per_cu = b->cfg.blocks_per_cu;
+    return 0;
+}
+
+int sptolBatchRun(sptolBatch b, void *stream)
+{
+    if (b->descs.empty()) return 0;
+    if (!b->planned) SPTOL_TRY(build_plan(b));
+    hipStream_t s = stream_of(stream);
+    if (b->exec && b->graph_stream == s) {
+        SPTOL_HIP(hipGraphLaunch(b->exec, s), "Batch Graph Launch");
+        return 0;
+    }
+    return launch_batch(b, s);
+}
+
+// Capture the fill + persistent launch into a hipGraph on `stream`; later
+// sptolBatchRun calls on the same stream replay it.
+int sptolBatchCapture(sptolBatch b, void *stream)
+{
+    if (b->descs.empty()) return 0;
+    if (!b->planned) SPTOL_TRY(build_plan(b));
+    hipStream_t s = stream_of(stream);
+    if (s == nullptr) SPTOL_CHECK(SPTERR_VALUE_ERROR, "Batch Capture", "capture needs a created (non-default) stream");
+    if (b->exec) { (void)hipGraphExecDestroy(b->exec); b->exec = nullptr; }
+    if (b->graph) { (void)hipGraphDestroy(b->graph); b->graph = nullptr; }
+    SPTOL_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "Batch Capture");
+    int r = launch_batch(b, s);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(s, &g);
+    if (r) { if (g) (void)hipGraphDestroy(g); return r; }
+    SPTOL_HIP(e, "Batch Capture");
+    b->graph = g;
+    SPTOL_HIP(hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0), "Batch Capture");
+    b->graph_stream = s;
+    return 0;
+}
+
+// Autotuner: time every (tile scale, blocks per CU) candidate on the batch
+// (median of `reps` runs after one warm-up), keep the fastest, and cache it
+// under the batch's signature (factor dtype, rank classes, log2 nonzeros per
+// class).  A cached signature is reused without timing.
+int sptolBatchAutotune(sptolBatch b, int reps, void *stream)
+{
+    if (b->descs.empty()) return 0;
+    const std::string key = signature(b);
+    auto it = g_tune_cache.find(key);
+    if (it != g_tune_cache.end()) {
+        b->cfg = it->second;
+        return build_plan(b);
+    }
+    hipStream_t s = stream_of(stream);
+    const double scales[] = {0.25, 0.5, 1.0, 2.0, 4.0};
+    const int occs[] = {2, 4, 8, 16};
+    double best = 1e300;
+    TuneCfg bestc = b->cfg;
+    hipEvent_t e0, e1;
+    SPTOL_HIP(hipEventCreate(&e0), "Autotune");
+    SPTOL_HIP(hipEventCreate(&e1), "Autotune");
+    if (reps < 1) reps = 1;
+    for (double sc : scales)
+        for (int oc : occs) {
+            b->cfg.tile_scale = sc;
+            b->cfg.blocks_per_cu = oc;
+            int r = build_plan(b);
+            if (r) { (void)hipEventDestroy(e0); (void)hipEventDestroy(e1); return r; }
+            r = launch_batch(b, s);   // warm-up
+            std::vector<float> t;
+            for (int k = 0; k < reps && !r; ++k) {
+                (void)hipEventRecord(e0, s);
+                r = launch_batch(b, s);
+                (void)hipEventRecord(e1, s);
+                (void)hipEventSynchronize(e1);
+                float ms = 0;
+                (void)hipEventElapsedTime(&ms, e0, e1);
+                t.push_back(ms);
+            }
+            if (r) { (void)hipEventDestroy(e0); (void)hipEventDestroy(e1); return r; }
+            std::sort(t.begin(), t.end());
+            const double med = t[t.size() / 2];
+            if (med < best) { best = med; bestc = b->cfg; }
+        }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    b->cfg = bestc;
+    g_tune_cache[key] = bestc;
+    return build_plan(b);
+}
+
+int sptolBatchSize(sptolBatch b) { return (int)b->descs.size(); }
+
+void sptolBatchFree(sptolBatch b) { delete b; }
+
+}  // extern "C"

@@ -91,6 +91,8 @@ static std::unordered_map<std::string, ProfTotal> g_prof_tot;
 int prof_start(const char *name, hipStream_t s)
 {
     if (!g_prof_on.load()) return -1;
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(s, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return -1;
     ProfRec r;
     r.name = name;
     if (hipEventCreate(&r.a) != hipSuccess || hipEventCreate(&r.b) != hipSuccess) return -1;
@@ -182,6 +184,20 @@ int sptolHipDuplicateMemory(void **dst, const void *src, size_t bytes)
     int r = sptolHipMemcpyH2D(*dst, src, bytes, nullptr);
     if (r) { dfree(*dst); *dst = nullptr; }
     return r;
+}
+
+int sptolHipStreamCreate(void **stream)
+{
+    hipStream_t s = nullptr;
+    SPTOL_HIP(hipStreamCreate(&s), "HipWrap StreamCreate");
+    *stream = (void *)s;
+    return 0;
+}
+
+int sptolHipStreamDestroy(void *stream)
+{
+    if (stream) SPTOL_HIP(hipStreamDestroy((hipStream_t)stream), "HipWrap StreamDestroy");
+    return 0;
 }
 
 int sptolHipStreamSynchronize(void *stream)

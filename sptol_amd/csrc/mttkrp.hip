@@ -740,4 +740,41 @@ int sptMTTKRPHiCOO_MatrixTiling(sptSparseTensorHiCOO const * const hitsr, sptRan
     return sptMTTKRPHiCOO(hitsr, pv, mats_order, mode);
 }
 
+// The reference's OpenMP variants keep their signatures and run the same
+// device path; thread counts, reduction buffers and the balanced flag have
+// no meaning on the GPU (mttkrp_omp.c:401, hicoo/mttkrp_omp.c:134-207).
+int sptOmpMTTKRP_Reduce(sptSparseTensor const * const X, sptMatrix * mats[], sptMatrix * copy_mats[],
+                        sptIndex const mats_order[], sptIndex const mode, const int tk)
+{
+    (void)copy_mats;
+    (void)tk;
+    return sptMTTKRP(X, mats, mats_order, mode);
+}
+
+int sptOmpMTTKRPHiCOO_MatrixTiling(sptSparseTensorHiCOO const * const hitsr, sptRankMatrix * mats[],
+                                   sptIndex const mats_order[], sptIndex const mode, const int nt)
+{
+    (void)nt;
+    return sptMTTKRPHiCOO_MatrixTiling(hitsr, mats, mats_order, mode);
+}
+
+int sptOmpMTTKRPHiCOO_MatrixTiling_Scheduled(sptSparseTensorHiCOO const * const hitsr, sptRankMatrix * mats[],
+                                             sptIndex const mats_order[], sptIndex const mode, const int tk,
+                                             int balanced)
+{
+    (void)tk;
+    (void)balanced;
+    return sptMTTKRPHiCOO_MatrixTiling(hitsr, mats, mats_order, mode);
+}
+
+int sptOmpMTTKRPHiCOO_MatrixTiling_Scheduled_Reduce(sptSparseTensorHiCOO const * const hitsr, sptRankMatrix * mats[],
+                                                    sptRankMatrix * copy_mats[], sptIndex const mats_order[],
+                                                    sptIndex const mode, const int tk, int balanced)
+{
+    (void)copy_mats;
+    (void)tk;
+    (void)balanced;
+    return sptMTTKRPHiCOO_MatrixTiling(hitsr, mats, mats_order, mode);
+}
+
 }  // extern "C"

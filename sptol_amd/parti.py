@@ -239,6 +239,19 @@ def _declare(lib):
         "sptCpdAls": (C.c_int, [P(sptSparseTensor), sptIndex, sptIndex, C.c_double, P(sptKruskalTensor)]),
         "sptolCpdAlsDevice": (C.c_int, [C.c_void_p, sptIndex, sptIndex, C.c_double, P(sptolDevMatrix),
                                         P(sptValue), P(sptolCpdStats), C.c_void_p]),
+        "sptolBatchCreate": (C.c_int, [P(C.c_void_p)]),
+        "sptolBatchAdd": (C.c_int, [C.c_void_p, C.c_void_p, P(sptolDevMatrix), P(sptIndex), sptIndex,
+                                    P(sptolDevMatrix)]),
+        "sptolBatchSetConfig": (C.c_int, [C.c_void_p, C.c_double, C.c_int]),
+        "sptolBatchGetConfig": (C.c_int, [C.c_void_p, P(C.c_double), P(C.c_int)]),
+        "sptolBatchAutotune": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p]),
+        "sptolBatchCapture": (C.c_int, [C.c_void_p, C.c_void_p]),
+        "sptolBatchRun": (C.c_int, [C.c_void_p, C.c_void_p]),
+        "sptolBatchSize": (C.c_int, [C.c_void_p]),
+        "sptolBatchFree": (None, [C.c_void_p]),
+        "sptolHipStreamCreate": (C.c_int, [P(C.c_void_p)]),
+        "sptolHipStreamDestroy": (C.c_int, [C.c_void_p]),
+        "sptolHipSetDevice": (C.c_int, [C.c_int]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name, None)

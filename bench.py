@@ -165,6 +165,10 @@ def main():
 
     dh = C.c_void_p()
     check(lib, lib.sptolDevHicooCreate(C.byref(dh), C.byref(h)), "DevHicooCreate")
+    t0 = time.time()
+    for m in range(3):
+        check(lib, lib.sptolDevHicooPrepareMode(dh, m), "DevHicooPrepareMode")
+    log(f"HiCOO per-mode views built in {time.time() - t0:.1f}s")
     dm = (sptolDevMatrix * 3)()
     hm = [Matrix(f) for f in factors]
     for m in range(3):

@@ -101,6 +101,13 @@ int  sptolDevMTTKRP(sptolDevCoo X, const sptolDevMatrix *mats, const sptIndex ma
 typedef struct sptolDevHicoo_s *sptolDevHicoo;
 
 int  sptolDevHicooCreate(sptolDevHicoo *h, const sptSparseTensorHiCOO *X);
+/* Build, on the device, a copy of the tensor whose blocks are stably ordered
+ * by their mode-th block coordinate (entries move with their blocks) plus
+ * the work list of the LDS-tiled kernel; later MTTKRPs on that mode
+ * accumulate each 2^sb_bits x R output tile in LDS instead of issuing one
+ * global atomic per nonzero.  Needs sb_bits <= 8 and stride <= 64; other
+ * shapes keep the block-order kernel. */
+int  sptolDevHicooPrepareMode(sptolDevHicoo h, sptIndex mode);
 void sptolDevHicooFree(sptolDevHicoo h);
 int  sptolDevMTTKRPHiCOO(sptolDevHicoo X, const sptolDevMatrix *mats, const sptIndex mats_order[],
                          sptIndex mode, sptolDevMatrix *out, void *stream);
@@ -139,10 +146,14 @@ int  sptolHipStreamDestroy(void *stream);
 /* Device-resident CPD-ALS                                                */
 /* ---------------------------------------------------------------------- */
 typedef struct {
-    int      factor_dtype;   /* SPTOL_F32 or SPTOL_BF16 storage for MTTKRP gathers */
-    int      iterations;     /* out: iterations executed                            */
-    double   fit;            /* out: last fit (FP64, correct ||K||^2)               */
-    double   seconds;        /* out: wall time of the ALS loop                      */
+    int      factor_dtype;   /* in: SPTOL_F32 or SPTOL_BF16 storage for MTTKRP gathers       */
+    int      exact_fit;      /* in: 1 = <X,K> by an FP64 pass over the nonzeros; 0 = from the
+                                last mode's FP32 MTTKRP, as the reference (ktensor.c:173-257),
+                                which loses digits to cancellation when the fit nears 1     */
+    int      iterations;     /* out: iterations executed                                    */
+    int      reserved;
+    double   fit;            /* out: last fit (FP64, correct ||K||^2)                       */
+    double   seconds;        /* out: wall time of the ALS loop                              */
 } sptolCpdStats;
 
 /* ALS on a device tensor.  mats[0..nmodes-1] (FP32 device matrices) hold the
@@ -151,6 +162,54 @@ typedef struct {
  * norms and fit in FP64.  tol <= 0 runs all niters. */
 int  sptolCpdAlsDevice(sptolDevCoo X, sptIndex rank, sptIndex niters, double tol,
                        sptolDevMatrix *mats, sptValue *lambda, sptolCpdStats *stats, void *stream);
+
+/* ---------------------------------------------------------------------- */
+/* Continuous-batching CPD-ALS server                                     */
+/* ---------------------------------------------------------------------- */
+typedef struct sptolServer_s *sptolServer;
+
+typedef struct {
+    size_t hbm_budget;      /* bytes of device memory admitted requests may reserve         */
+    int    max_active;      /* requests on the device at once (slots)                       */
+    int    batch_mttkrp;    /* 1 = one persistent launch per step for every active request  */
+    int    exact_fit;       /* 1 = FP64 pass for <X,K> (see sptolCpdStats.exact_fit)         */
+    int    blocks_per_cu;   /* persistent-kernel residency (0 = default 8)                  */
+    double tile_scale;      /* batched tile length multiplier (0 = default 1.0)             */
+} sptolServerConfig;
+
+#define SPTOL_REQ_QUEUED    0
+#define SPTOL_REQ_PREPARING 1
+#define SPTOL_REQ_RUNNING   2
+#define SPTOL_REQ_DONE      3
+#define SPTOL_REQ_FAILED    4
+
+typedef struct {
+    int    status;          /* SPTOL_REQ_*                                   */
+    int    error;           /* SptError of a failed request                  */
+    int    iterations;
+    int    rank;
+    double fit;
+    double latency_s;       /* submit -> result available                    */
+    double queue_s;         /* submit -> admitted to the device              */
+} sptolRequestInfo;
+
+/* Start the admission and compute threads on the current device. */
+int  sptolServerCreate(sptolServer *s, const sptolServerConfig *cfg);
+/* Queue a CPD-ALS request (copies the tensor; the caller may free it on
+ * return).  Factors start U[0,1) from `seed`; the solve stops after niters
+ * iterations or when the fit changes by less than tol (tol <= 0: never). */
+int  sptolServerSubmit(sptolServer s, const sptSparseTensor *X, sptIndex rank, sptIndex niters, double tol,
+                       uint64_t seed, int factor_dtype, uint64_t *request_id);
+/* Block until the request finished (timeout_s < 0: no limit).  Returns
+ * SPTERR_NO_MORE on timeout. */
+int  sptolServerWait(sptolServer s, uint64_t request_id, double timeout_s, sptolRequestInfo *info);
+/* Copy a finished request's factors (mats[m]: ndims[m] x rank, stride as
+ * sptMatrix) and lambda (rank floats); either may be NULL. */
+int  sptolServerFetch(sptolServer s, uint64_t request_id, sptMatrix **mats, sptValue *lambda);
+int  sptolServerRelease(sptolServer s, uint64_t request_id);
+int  sptolServerDrain(sptolServer s);
+uint64_t sptolServerTicks(sptolServer s);
+void sptolServerDestroy(sptolServer s);
 
 #ifdef __cplusplus
 }

@@ -23,19 +23,6 @@
 
 namespace sptol {
 
-struct BatchDesc {
-    uint64_t nnz;
-    uint64_t tile_nnz;         // nonzeros per tile (multiple of 64)
-    const uint32_t *inds[4];
-    const float *vals;
-    const void *mats[4];
-    float *out;
-    uint64_t out_len;          // floats in the output (rows * stride)
-    uint32_t nmodes, mode, R, stride;
-    uint32_t order[4];
-    uint32_t lpn;              // lanes per nonzero = stride / 4
-    uint32_t pad;
-};
 
 // Zero every output and reset the queue counter.  Work is split in chunks
 // of 64K floats; chunk c of the batch belongs to tensor t with
@@ -259,6 +246,76 @@ static int launch_batch(sptolBatch b, hipStream_t s)
     return 0;
 }
 
+void make_desc(sptolDevCoo X, const sptolDevMatrix *mats, const uint32_t *mats_order, uint32_t mode,
+               const sptolDevMatrix *out, BatchDesc *d)
+{
+    std::memset(d, 0, sizeof(*d));
+    d->nnz = X->nnz;
+    const SortedCopy &sc = X->sorted[mode];
+    for (uint32_t m = 0; m < X->nmodes; ++m) {
+        d->inds[m] = sc.ready ? sc.inds[m] : X->inds[m];
+        d->mats[m] = mats[m].values;
+        d->order[m] = mats_order[m];
+    }
+    d->vals = sc.ready ? sc.vals : X->vals;
+    d->out = (float *)out->values;
+    d->out_len = (uint64_t)X->ndims[mode] * out->stride;
+    d->nmodes = X->nmodes;
+    d->mode = mode;
+    d->R = out->ncols;
+    d->stride = out->stride;
+    d->lpn = out->stride / 4;
+}
+
+BatchBuffers::~BatchBuffers()
+{
+    dfree(desc); dfree(tile_prefix); dfree(fill_prefix); dfree(counter);
+}
+
+int batch_launch(std::vector<BatchDesc> &descs, int dtype, double tile_scale, int blocks_per_cu, BatchBuffers &bb,
+                 hipStream_t s)
+{
+    const size_t n = descs.size();
+    if (n == 0) return 0;
+    std::vector<uint64_t> tp(n + 1, 0), fp(n + 1, 0);
+    for (size_t i = 0; i < n; ++i) {
+        BatchDesc &d = descs[i];
+        d.tile_nnz = tile_len(d.stride, tile_scale);
+        tp[i + 1] = tp[i] + (d.nnz + d.tile_nnz - 1) / d.tile_nnz;
+        fp[i + 1] = fp[i] + (d.out_len + 65535) / 65536;
+    }
+    if (n > bb.cap) {
+        dfree(bb.desc); dfree(bb.tile_prefix); dfree(bb.fill_prefix);
+        bb.desc = nullptr; bb.tile_prefix = nullptr; bb.fill_prefix = nullptr; bb.cap = 0;
+        const size_t cap = n * 2;
+        SPTOL_TRY(dmalloc((void **)&bb.desc, cap * sizeof(BatchDesc), "Batch"));
+        SPTOL_TRY(dmalloc((void **)&bb.tile_prefix, (cap + 1) * 8, "Batch"));
+        SPTOL_TRY(dmalloc((void **)&bb.fill_prefix, (cap + 1) * 8, "Batch"));
+        bb.cap = cap;
+    }
+    if (!bb.counter) SPTOL_TRY(dmalloc((void **)&bb.counter, 4, "Batch"));
+    // pageable sources: each copy is staged before the call returns, so the
+    // host vectors may be reused immediately; the copies stay stream-ordered
+    SPTOL_HIP(hipMemcpyAsync(bb.desc, descs.data(), n * sizeof(BatchDesc), hipMemcpyHostToDevice, s), "Batch");
+    SPTOL_HIP(hipMemcpyAsync(bb.tile_prefix, tp.data(), (n + 1) * 8, hipMemcpyHostToDevice, s), "Batch");
+    SPTOL_HIP(hipMemcpyAsync(bb.fill_prefix, fp.data(), (n + 1) * 8, hipMemcpyHostToDevice, s), "Batch");
+    const int cus = cu_count();
+    const unsigned fill_grid = (unsigned)std::min<uint64_t>(std::max<uint64_t>(fp[n], 1), (uint64_t)cus * 8);
+    hipLaunchKernelGGL(k_batch_fill, dim3(fill_grid), dim3(256), 0, s, bb.desc, bb.fill_prefix, (uint32_t)n, fp[n],
+                       bb.counter);
+    const unsigned grid = (unsigned)(cus * blocks_per_cu);
+    int tok = prof_start("batch_mttkrp", s);
+    if (dtype == SPTOL_BF16)
+        hipLaunchKernelGGL(k_batch_mttkrp<uint16_t>, dim3(grid), dim3(256), 0, s, bb.desc, bb.tile_prefix, (uint32_t)n,
+                           tp[n], bb.counter);
+    else
+        hipLaunchKernelGGL(k_batch_mttkrp<float>, dim3(grid), dim3(256), 0, s, bb.desc, bb.tile_prefix, (uint32_t)n,
+                           tp[n], bb.counter);
+    prof_stop(tok, s);
+    SPTOL_HIP(hipGetLastError(), "Batch launch");
+    return 0;
+}
+
 static std::string signature(sptolBatch b)
 {
     // rank mix and log2 total nonzeros per rank class
@@ -300,22 +357,7 @@ int sptolBatchAdd(sptolBatch b, sptolDevCoo X, const sptolDevMatrix *mats, const
     }
     b->dtype = mats[0].dtype;
     BatchDesc d;
-    std::memset(&d, 0, sizeof(d));
-    d.nnz = X->nnz;
-    const SortedCopy &sc = X->sorted[mode];
-    for (uint32_t m = 0; m < X->nmodes; ++m) {
-        d.inds[m] = sc.ready ? sc.inds[m] : X->inds[m];
-        d.mats[m] = mats[m].values;
-        d.order[m] = mats_order[m];
-    }
-    d.vals = sc.ready ? sc.vals : X->vals;
-    d.out = (float *)out->values;
-    d.out_len = (uint64_t)X->ndims[mode] * S;
-    d.nmodes = X->nmodes;
-    d.mode = mode;
-    d.R = out->ncols;
-    d.stride = S;
-    d.lpn = S / 4;
+    make_desc(X, mats, mats_order, mode, out, &d);
     b->descs.push_back(d);
     b->planned = false;
     return 0;

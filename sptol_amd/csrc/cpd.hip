@@ -16,7 +16,7 @@
 //     reads an unwritten triangle (ktensor.c:151).
 //   * Reductions are two-stage with a fixed block count, so the solve, norm
 //     and fit steps are bitwise reproducible for a given MTTKRP result.
-#include "dev_common.hpp"
+#include "cpd_job.hpp"
 #include <vector>
 #include <cmath>
 #include <cstring>
@@ -25,8 +25,6 @@
 
 namespace sptol {
 
-constexpr int kRedBlocks = 240;   // fixed block count of the two-stage reductions
-constexpr int kMaxRank = 64;
 
 // ---------------------------------------------------------------------------
 // Kernels
@@ -330,6 +328,47 @@ __global__ __launch_bounds__(256) void k_inner_partial(const float *__restrict__
     }
 }
 
+// <X, K> = sum_nnz x * sum_r lambda_r prod_m A_m[i_m][r], every product and
+// sum in FP64 from the FP32 factors: no cancellation-prone reuse of an FP32
+// MTTKRP.  One thread per nonzero; per-block partial sums (fixed order).
+struct InnerArgs {
+    const uint32_t *inds[kMaxModes];
+    const float *vals;
+    const float *mats[kMaxModes];
+    uint64_t nnz;
+    uint32_t N, R, S;
+};
+
+__global__ __launch_bounds__(256) void k_inner_exact_partial(const InnerArgs a, const double *__restrict__ lambda,
+                                                             double *__restrict__ partial)
+{
+    __shared__ double red[256];
+    __shared__ double lam[kMaxRank];
+    if (threadIdx.x < a.R) lam[threadIdx.x] = lambda[threadIdx.x];
+    __syncthreads();
+    const uint64_t per = (a.nnz + gridDim.x - 1) / gridDim.x;
+    const uint64_t z0 = (uint64_t)blockIdx.x * per, z1 = min<uint64_t>(z0 + per, a.nnz);
+    double s = 0.0;
+    for (uint64_t z = z0 + threadIdx.x; z < z1; z += 256) {
+        const float *row[kMaxModes];
+        for (uint32_t m = 0; m < a.N; ++m) row[m] = a.mats[m] + (size_t)a.inds[m][z] * a.S;
+        double k = 0.0;
+        for (uint32_t r = 0; r < a.R; ++r) {
+            double p = lam[r];
+            for (uint32_t m = 0; m < a.N; ++m) p *= (double)row[m][r];
+            k += p;
+        }
+        s += (double)a.vals[z] * k;
+    }
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (uint32_t w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
 // Sum of squares of the values (||X||^2), two-stage.
 __global__ __launch_bounds__(256) void k_sqnorm_partial(const float *__restrict__ v, uint64_t n, double *__restrict__ partial)
 {
@@ -348,9 +387,12 @@ __global__ __launch_bounds__(256) void k_sqnorm_partial(const float *__restrict_
 }
 
 // fit from ||X||^2, the Grams, lambda and the inner-product partials.
+// inner_scalar = 1: inner_partial holds nblk scalar partials of <X,K>
+// (exact pass); 0: nblk x R partials of sum_i A_last[i][r] M[i][r].
 __global__ __launch_bounds__(256) void k_fit(const double *__restrict__ ata, uint32_t N, uint32_t R,
                                              const double *__restrict__ lambda, const double *__restrict__ inner_partial,
-                                             uint32_t nblk, const double *__restrict__ normx, double *__restrict__ fit)
+                                             uint32_t nblk, int inner_scalar, const double *__restrict__ normx,
+                                             double *__restrict__ fit)
 {
     __shared__ double red[256];
     const uint32_t RR = R * R;
@@ -369,10 +411,14 @@ __global__ __launch_bounds__(256) void k_fit(const double *__restrict__ ata, uin
     if (threadIdx.x == 0) {
         const double normk = fabs(red[0]);
         double inner = 0.0;
-        for (uint32_t r = 0; r < R; ++r) {
-            double acc = 0.0;
-            for (uint32_t b = 0; b < nblk; ++b) acc += inner_partial[(size_t)b * R + r];
-            inner += acc * lambda[r];
+        if (inner_scalar) {
+            for (uint32_t b = 0; b < nblk; ++b) inner += inner_partial[b];
+        } else {
+            for (uint32_t r = 0; r < R; ++r) {
+                double acc = 0.0;
+                for (uint32_t b = 0; b < nblk; ++b) acc += inner_partial[(size_t)b * R + r];
+                inner += acc * lambda[r];
+            }
         }
         double residual = *normx + normk - 2.0 * inner;
         if (residual > 0.0) residual = sqrt(residual);
@@ -395,28 +441,15 @@ static unsigned grid_rows(uint64_t n, uint32_t per_block)
 }
 
 // ---------------------------------------------------------------------------
-// Workspace and host loop
+// One ALS solve as a resumable job (used directly below and by the server)
 // ---------------------------------------------------------------------------
-struct CpdWork {
-    float *M = nullptr;            // MTTKRP output, maxI x S
-    uint16_t *bf[kMaxModes] = {};  // BF16 factor copies
-    double *ata = nullptr;         // N x R x R
-    double *partial = nullptr;     // kRedBlocks x R x R
-    double *fac = nullptr;         // R x R
-    int *piv = nullptr;            // R
-    int *kind = nullptr;           // 1
-    double *lambda = nullptr;      // R
-    double *norms = nullptr;       // R
-    double *normx = nullptr;       // 1
-    double *fit = nullptr;         // 1
-    ~CpdWork()
-    {
-        dfree(M);
-        for (auto p : bf) dfree(p);
-        dfree(ata); dfree(partial); dfree(fac); dfree(piv); dfree(kind);
-        dfree(lambda); dfree(norms); dfree(normx); dfree(fit);
-    }
-};
+CpdWork::~CpdWork()
+{
+    dfree(M);
+    for (auto p : bf) dfree(p);
+    dfree(ata); dfree(partial); dfree(fac); dfree(piv); dfree(kind);
+    dfree(lambda); dfree(norms); dfree(normx); dfree(fit);
+}
 
 static int gram(const float *A, uint32_t I, uint32_t R, uint32_t S, double *out, double *partial, hipStream_t s)
 {
@@ -457,31 +490,35 @@ static int solve(const float *M, float *A, uint32_t I, uint32_t R, uint32_t S, C
     return 0;
 }
 
-}  // namespace sptol
-
-using namespace sptol;
-
-extern "C" {
-
-int sptolCpdAlsDevice(sptolDevCoo X, sptIndex rank, sptIndex niters, double tol, sptolDevMatrix *mats,
-                      sptValue *lambda_out, sptolCpdStats *stats, void *stream)
+CpdJob::~CpdJob()
 {
-    const uint32_t N = X->nmodes, R = rank;
+    if (h_fit) (void)hipHostFree(h_fit);
+}
+
+size_t CpdJob::workspace_bytes(const uint32_t *ndims, uint32_t N, uint32_t R, int fdt)
+{
+    const uint32_t S = R ? ((R - 1) / 8 + 1) * 8 : 8;
+    uint64_t maxI = 1, sumI = 0;
+    for (uint32_t m = 0; m < N; ++m) { maxI = ndims[m] > maxI ? ndims[m] : maxI; sumI += ndims[m]; }
+    size_t b = (size_t)maxI * S * 4 + (size_t)N * R * R * 8 + (size_t)kRedBlocks * R * R * 8 + (size_t)R * R * 8 + 4096;
+    if (fdt == SPTOL_BF16) b += (size_t)sumI * S * 2;
+    return b;
+}
+
+int CpdJob::setup(sptolDevCoo X_, uint32_t R_, uint32_t niters_, double tol_, sptolDevMatrix *mats_, int fdt_,
+                  hipStream_t s)
+{
+    X = X_; N = X->nmodes; R = R_; niters = niters_; tol = tol_; mats = mats_; fdt = fdt_;
     if (R < 1 || R > (uint32_t)kMaxRank) SPTOL_CHECK(SPTERR_VALUE_ERROR, "CPD", "rank must be in [1, 64]");
-    const int fdt = stats ? stats->factor_dtype : SPTOL_F32;
     if (fdt != SPTOL_F32 && fdt != SPTOL_BF16) SPTOL_CHECK(SPTERR_VALUE_ERROR, "CPD", "bad factor dtype");
-    uint32_t maxI = 0;
+    maxI = 0;
     for (uint32_t m = 0; m < N; ++m) {
-        if (mats[m].dtype != SPTOL_F32 || mats[m].ncols != R || mats[m].nrows != X->ndims[m])
+        if (mats[m].dtype != SPTOL_F32 || mats[m].ncols != R || mats[m].nrows != X->ndims[m] ||
+            mats[m].stride != mats[0].stride)
             SPTOL_CHECK(SPTERR_SHAPE_MISMATCH, "CPD", "factors must be FP32 ndims[m] x rank");
         maxI = X->ndims[m] > maxI ? X->ndims[m] : maxI;
     }
-    const uint32_t S = mats[0].stride;
-    hipStream_t s = stream_of(stream);
-    auto t0 = std::chrono::steady_clock::now();
-    for (uint32_t m = 0; m < N; ++m) SPTOL_TRY(sptolDevCooPrepareMode(X, m));
-
-    CpdWork w;
+    S = mats[0].stride;
     SPTOL_TRY(dmalloc((void **)&w.M, (size_t)(maxI ? maxI : 1) * S * 4, "CPD"));
     SPTOL_TRY(dmalloc((void **)&w.ata, (size_t)N * R * R * 8, "CPD"));
     SPTOL_TRY(dmalloc((void **)&w.partial, (size_t)kRedBlocks * R * R * 8, "CPD"));
@@ -495,15 +532,12 @@ int sptolCpdAlsDevice(sptolDevCoo X, sptIndex rank, sptIndex niters, double tol,
     if (fdt == SPTOL_BF16)
         for (uint32_t m = 0; m < N; ++m)
             SPTOL_TRY(dmalloc((void **)&w.bf[m], (size_t)(X->ndims[m] ? X->ndims[m] : 1) * S * 2, "CPD"));
-
-    sptolDevMatrix mt[kMaxModes];    // what the MTTKRP reads (FP32 or the BF16 copies)
     for (uint32_t m = 0; m < N; ++m) {
         mt[m] = mats[m];
         if (fdt == SPTOL_BF16) { mt[m].dtype = SPTOL_BF16; mt[m].values = w.bf[m]; }
     }
-    sptolDevMatrix Mout;
     Mout.nrows = maxI; Mout.ncols = R; Mout.stride = S; Mout.dtype = SPTOL_F32; Mout.values = w.M;
-
+    SPTOL_HIP(hipHostMalloc((void **)&h_fit, sizeof(double), hipHostMallocDefault), "CPD");
     // ||X||^2 and the initial Grams (cpd.c:65-70)
     hipLaunchKernelGGL(k_sqnorm_partial, dim3(kRedBlocks), dim3(256), 0, s, X->vals, X->nnz, w.partial);
     hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, s, w.partial, (uint32_t)kRedBlocks, 1u, w.normx);
@@ -517,63 +551,115 @@ int sptolCpdAlsDevice(sptolDevCoo X, sptIndex rank, sptIndex niters, double tol,
                                (float *)mats[m].values, w.bf[m], X->ndims[m], R, S, w.norms);
         }
     }
-    double fit = 0.0, oldfit = 0.0;
-    double *h_fit = nullptr;
-    SPTOL_HIP(hipHostMalloc((void **)&h_fit, sizeof(double), hipHostMallocDefault), "CPD");
-    uint32_t order[kMaxModes];
-    int iters = 0;
-    int rc = 0;
-    for (uint32_t it = 0; it < niters && !rc; ++it) {
-        auto ti = std::chrono::steady_clock::now();
-        for (uint32_t m = 0; m < N && !rc; ++m) {
-            const uint32_t I = X->ndims[m];
-            order[0] = m;
-            for (uint32_t k = 1; k < N; ++k) order[k] = (m + k) % N;
-            Mout.nrows = I;
-            rc = sptolDevMTTKRP(X, mt, order, m, &Mout, stream);
-            if (rc) break;
-            rc = solve(w.M, (float *)mats[m].values, I, R, S, w, N, m, s);
-            if (rc) break;
-            rc = colnorm((float *)mats[m].values, fdt == SPTOL_BF16 ? w.bf[m] : nullptr, I, R, S, it == 0 ? 0 : 1, 0,
-                         w, s);
-            if (rc) break;
-            rc = gram((const float *)mats[m].values, I, R, S, w.ata + (size_t)m * R * R, w.partial, s);
-        }
-        if (rc) break;
-        // fit: the last mode's MTTKRP result is still in w.M
-        const uint32_t last = N - 1;
+    it = 0; m = 0; fit = oldfit = 0.0; iters = 0;
+    done = niters == 0;
+    fit_pending = false;
+    t_iter = std::chrono::steady_clock::now();
+    return 0;
+}
+
+void CpdJob::order(uint32_t *o) const
+{
+    o[0] = m;
+    for (uint32_t k = 1; k < N; ++k) o[k] = (m + k) % N;
+}
+
+int CpdJob::mttkrp(hipStream_t s)
+{
+    uint32_t o[kMaxModes];
+    order(o);
+    Mout.nrows = X->ndims[m];
+    return coo_mttkrp(X, mt, o, m, &Mout, s);
+}
+
+int CpdJob::after_mttkrp(hipStream_t s)
+{
+    const uint32_t I = X->ndims[m];
+    SPTOL_TRY(solve(w.M, (float *)mats[m].values, I, R, S, w, N, m, s));
+    SPTOL_TRY(colnorm((float *)mats[m].values, fdt == SPTOL_BF16 ? w.bf[m] : nullptr, I, R, S, it == 0 ? 0 : 1, 0,
+                      w, s));
+    SPTOL_TRY(gram((const float *)mats[m].values, I, R, S, w.ata + (size_t)m * R * R, w.partial, s));
+    if (++m < N) return 0;
+    // end of an iteration: fit from the last mode's MTTKRP, still in w.M
+    const uint32_t last = N - 1;
+    if (exact_fit) {
+        InnerArgs ia;
+        std::memset(&ia, 0, sizeof(ia));
+        for (uint32_t q = 0; q < N; ++q) { ia.inds[q] = X->inds[q]; ia.mats[q] = (const float *)mats[q].values; }
+        ia.vals = X->vals;
+        ia.nnz = X->nnz;
+        ia.N = N; ia.R = R; ia.S = S;
+        hipLaunchKernelGGL(k_inner_exact_partial, dim3(kRedBlocks), dim3(256), 0, s, ia, w.lambda, w.partial);
+    } else {
         hipLaunchKernelGGL(k_inner_partial, dim3(kRedBlocks), dim3(256), 0, s, (const float *)mats[last].values, w.M,
                            X->ndims[last], R, S, w.partial);
-        hipLaunchKernelGGL(k_fit, dim3(1), dim3(256), 0, s, w.ata, N, R, w.lambda, w.partial, (uint32_t)kRedBlocks,
-                           w.normx, w.fit);
-        hipError_t e = hipMemcpyAsync(h_fit, w.fit, sizeof(double), hipMemcpyDeviceToHost, s);
-        if (e == hipSuccess) e = hipStreamSynchronize(s);
-        if (e != hipSuccess) {
-            sptol_complain("CPD", SPTERR_CUDA_ERROR + (int)e, __FILE__, __LINE__, hipGetErrorString(e));
-            rc = SPTERR_CUDA_ERROR + (int)e;
-            break;
-        }
-        fit = *h_fit;
-        iters = (int)it + 1;
-        if (verbose()) {
-            double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - ti).count();
-            printf("  its = %u ( %.3lf s ) fit = %0.5f  delta = %+0.4e\n", it + 1, sec, fit, fit - oldfit);
-        }
-        if (it > 0 && std::fabs(fit - oldfit) < tol) break;
-        oldfit = fit;
     }
-    (void)hipHostFree(h_fit);
-    if (rc) return rc;
+    hipLaunchKernelGGL(k_fit, dim3(1), dim3(256), 0, s, w.ata, N, R, w.lambda, w.partial, (uint32_t)kRedBlocks,
+                       exact_fit ? 1 : 0, w.normx, w.fit);
+    SPTOL_HIP(hipGetLastError(), "CPD fit");
+    SPTOL_HIP(hipMemcpyAsync(h_fit, w.fit, sizeof(double), hipMemcpyDeviceToHost, s), "CPD fit");
+    fit_pending = true;
+    return 0;
+}
+
+void CpdJob::fit_ready()
+{
+    fit = *h_fit;
+    fit_pending = false;
+    iters = (int)it + 1;
+    if (verbose()) {
+        const double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_iter).count();
+        printf("  its = %u ( %.3lf s ) fit = %0.5f  delta = %+0.4e\n", it + 1, sec, fit, fit - oldfit);
+    }
+    t_iter = std::chrono::steady_clock::now();
+    if ((it > 0 && std::fabs(fit - oldfit) < tol) || it + 1 >= niters) {
+        done = true;
+    } else {
+        oldfit = fit;
+        ++it;
+        m = 0;
+    }
+}
+
+int CpdJob::finish(hipStream_t s, sptValue *lambda_out)
+{
     // GetFinalLambda (matrix.c:557-573)
-    for (uint32_t m = 0; m < N; ++m)
-        SPTOL_TRY(colnorm((float *)mats[m].values, nullptr, X->ndims[m], R, S, 0, 1, w, s));
+    for (uint32_t q = 0; q < N; ++q)
+        SPTOL_TRY(colnorm((float *)mats[q].values, nullptr, X->ndims[q], R, S, 0, 1, w, s));
     std::vector<double> lam(R);
     SPTOL_HIP(hipMemcpyAsync(lam.data(), w.lambda, R * 8, hipMemcpyDeviceToHost, s), "CPD");
     SPTOL_HIP(hipStreamSynchronize(s), "CPD");
     for (uint32_t r = 0; r < R; ++r) lambda_out[r] = (sptValue)lam[r];
+    return 0;
+}
+
+}  // namespace sptol
+
+using namespace sptol;
+
+extern "C" {
+
+int sptolCpdAlsDevice(sptolDevCoo X, sptIndex rank, sptIndex niters, double tol, sptolDevMatrix *mats,
+                      sptValue *lambda_out, sptolCpdStats *stats, void *stream)
+{
+    hipStream_t s = stream_of(stream);
+    auto t0 = std::chrono::steady_clock::now();
+    for (uint32_t m = 0; m < X->nmodes; ++m) SPTOL_TRY(coo_prepare(X, m, s));
+    CpdJob job;
+    job.exact_fit = stats ? stats->exact_fit != 0 : false;
+    SPTOL_TRY(job.setup(X, rank, niters, tol, mats, stats ? stats->factor_dtype : SPTOL_F32, s));
+    while (!job.done) {
+        SPTOL_TRY(job.mttkrp(s));
+        SPTOL_TRY(job.after_mttkrp(s));
+        if (job.fit_pending) {
+            SPTOL_HIP(hipStreamSynchronize(s), "CPD");
+            job.fit_ready();
+        }
+    }
+    SPTOL_TRY(job.finish(s, lambda_out));
     if (stats) {
-        stats->iterations = iters;
-        stats->fit = fit;
+        stats->iterations = job.iters;
+        stats->fit = job.fit;
         stats->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     }
     return 0;

@@ -1,0 +1,61 @@
+// One CPD-ALS solve as a resumable job: the standalone loop and the server
+// drive the same steps.  Follows src/sptensor/cpd.c:32-137.
+#pragma once
+#include "dev_common.hpp"
+#include <chrono>
+
+namespace sptol {
+
+constexpr int kRedBlocks = 240;   // fixed block count of the two-stage reductions
+constexpr int kMaxRank = 64;
+
+struct CpdWork {
+    float *M = nullptr;            // MTTKRP output, maxI x S
+    uint16_t *bf[kMaxModes] = {};  // BF16 factor copies
+    double *ata = nullptr;         // N x R x R Grams
+    double *partial = nullptr;     // kRedBlocks x R x R
+    double *fac = nullptr;         // R x R Cholesky or LU factor
+    int *piv = nullptr;            // R
+    int *kind = nullptr;           // 0 Cholesky, 1 LU
+    double *lambda = nullptr;      // R
+    double *norms = nullptr;       // R
+    double *normx = nullptr;       // ||X||^2
+    double *fit = nullptr;         // last fit
+    ~CpdWork();
+};
+
+struct CpdJob {
+    sptolDevCoo X = nullptr;
+    uint32_t N = 0, R = 0, S = 0, maxI = 0;
+    int fdt = SPTOL_F32;
+    uint32_t niters = 0;
+    double tol = 0.0;
+    sptolDevMatrix *mats = nullptr;       // FP32 factors, owned by the caller
+    CpdWork w;
+    sptolDevMatrix mt[kMaxModes] = {};    // what the MTTKRP reads (FP32 or the BF16 copies)
+    sptolDevMatrix Mout = {};
+    uint32_t it = 0, m = 0;               // current iteration and mode
+    double fit = 0.0, oldfit = 0.0;
+    int iters = 0;
+    bool done = false, fit_pending = false;
+    bool exact_fit = false;               // <X,K> by an FP64 pass over the nonzeros
+    double *h_fit = nullptr;              // pinned
+    std::chrono::steady_clock::time_point t_iter;
+
+    ~CpdJob();
+    // Device bytes setup() allocates beyond the tensor and the factors.
+    static size_t workspace_bytes(const uint32_t *ndims, uint32_t N, uint32_t R, int fdt);
+    // Allocate, ||X||^2, initial Grams (and BF16 copies).  Every mode of X
+    // must already have its sorted copy (coo_prepare).
+    int setup(sptolDevCoo X, uint32_t R, uint32_t niters, double tol, sptolDevMatrix *mats, int fdt, hipStream_t s);
+    void order(uint32_t *o) const;        // mats_order of the current mode
+    int mttkrp(hipStream_t s);            // the current mode's MTTKRP into Mout
+    // Solve, normalise and Gram for the current mode, then advance; at the
+    // end of an iteration queue the fit and its copy to the host and set
+    // fit_pending (call fit_ready() after the stream synchronises).
+    int after_mttkrp(hipStream_t s);
+    void fit_ready();
+    int finish(hipStream_t s, sptValue *lambda_out);   // final lambda
+};
+
+}  // namespace sptol

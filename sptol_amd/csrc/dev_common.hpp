@@ -5,6 +5,7 @@
 #include <hip/hip_bf16.h>
 #include <cstdint>
 #include <cstddef>
+#include <vector>
 #include "../../include/sptol_hip.h"
 #include "host_internal.h"
 
@@ -55,6 +56,8 @@ struct sptolDevCoo_s {
     sptol::SortedCopy sorted[sptol::kMaxModes];
 };
 
+struct sptolHicooViews;
+
 struct sptolDevHicoo_s {
     uint32_t nmodes = 0;
     uint32_t ndims[sptol::kMaxModes] = {};
@@ -64,6 +67,7 @@ struct sptolDevHicoo_s {
     uint32_t *binds[sptol::kMaxModes] = {};         // nb each
     uint8_t *einds[sptol::kMaxModes] = {};          // nnz each
     float *vals = nullptr;                          // nnz
+    sptolHicooViews *views = nullptr;               // per-mode reordered copies (hicoo_view.hip)
 };
 
 namespace sptol {
@@ -93,6 +97,50 @@ struct MttkrpArgs {
 // prof_start returns a token (or -1 when profiling is off); prof_stop closes it.
 int prof_start(const char *name, hipStream_t s);
 void prof_stop(int token, hipStream_t s);
+
+// Allocate + copy on stream s (synchronises s).
+int dup_on(void **dst, const void *src, size_t bytes, hipStream_t s, const char *module);
+// Stream-aware forms of sptolDevCooCreate / sptolDevCooPrepareMode.
+int coo_create(sptolDevCoo *h, const sptSparseTensor *X, hipStream_t s);
+int coo_prepare(sptolDevCoo h, uint32_t mode, hipStream_t s);
+// Device MTTKRP on stream s (sptolDevMTTKRP without the argument checks).
+int coo_mttkrp(sptolDevCoo X, const sptolDevMatrix *mats, const uint32_t *order, uint32_t mode,
+               sptolDevMatrix *out, hipStream_t s);
+
+// One MTTKRP of a batch (device pointers; shared by the batch API and the server).
+struct BatchDesc {
+    uint64_t nnz;
+    uint64_t tile_nnz;         // nonzeros per tile (multiple of 64)
+    const uint32_t *inds[4];
+    const float *vals;
+    const void *mats[4];
+    float *out;
+    uint64_t out_len;          // floats in the output (rows * stride)
+    uint32_t nmodes, mode, R, stride;
+    uint32_t order[4];
+    uint32_t lpn;              // lanes per nonzero = stride / 4
+    uint32_t pad;
+};
+
+void make_desc(sptolDevCoo X, const sptolDevMatrix *mats, const uint32_t *mats_order, uint32_t mode,
+               const sptolDevMatrix *out, BatchDesc *d);
+
+// Reusable device buffers of a per-step batched launch.
+struct BatchBuffers {
+    BatchDesc *desc = nullptr;
+    uint64_t *tile_prefix = nullptr, *fill_prefix = nullptr;
+    uint32_t *counter = nullptr;
+    size_t cap = 0;
+    ~BatchBuffers();
+};
+
+// Fill + persistent batched MTTKRP of `descs` (all with factor dtype `dtype`) on s.
+int batch_launch(std::vector<BatchDesc> &descs, int dtype, double tile_scale, int blocks_per_cu, BatchBuffers &bb,
+                 hipStream_t s);
+
+// HiCOO per-mode views (hicoo_view.hip).
+void free_hicoo_views(sptolDevHicoo h);
+int launch_hicoo_tiled(sptolDevHicoo X, const MttkrpArgs &a, int dtype, hipStream_t s, int *used);
 
 int launch_mttkrp_coo(const MttkrpArgs &a, int dtype, hipStream_t s);
 int launch_mttkrp_hicoo(const MttkrpArgs &a, int dtype, hipStream_t s);

@@ -222,7 +222,9 @@ __global__ __launch_bounds__(256) void k_mttkrp_hicoo(const MttkrpArgs a)
             float v[U];
 #pragma unroll
             for (int u = 0; u < U; ++u) {
-                const uint64_t zz = valid[u] ? z + u : z0;
+                // padding lanes repeat the chunk's first nonzero z (block blk[0]):
+                // a block coordinate and an element offset of the same nonzero
+                const uint64_t zz = valid[u] ? z + u : z;
                 const uint64_t bb = valid[u] ? blk[u] : blk[0];
                 v[u] = a.vals[zz];
 #pragma unroll
@@ -408,7 +410,25 @@ using namespace sptol;
 
 extern "C" {
 
-int sptolDevCooCreate(sptolDevCoo *h, const sptSparseTensor *X)
+}  // extern "C"
+
+namespace sptol {
+
+int dup_on(void **dst, const void *src, size_t bytes, hipStream_t s, const char *module)
+{
+    SPTOL_TRY(dmalloc(dst, bytes, module));
+    if (!bytes) return 0;
+    hipError_t e = hipMemcpyAsync(*dst, src, bytes, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) {
+        dfree(*dst);
+        *dst = nullptr;
+        SPTOL_HIP(e, module);
+    }
+    return 0;
+}
+
+int coo_create(sptolDevCoo *h, const sptSparseTensor *X, hipStream_t s)
 {
     *h = nullptr;
     if (X->nmodes < 2 || X->nmodes > kMaxModes)
@@ -428,21 +448,20 @@ int sptolDevCooCreate(sptolDevCoo *h, const sptSparseTensor *X)
     for (uint32_t m = 0; m < t->nmodes; ++m) t->ndims[m] = X->ndims[m];
     int r = 0;
     for (uint32_t m = 0; m < t->nmodes && !r; ++m)
-        r = sptolHipDuplicateMemory((void **)&t->inds[m], X->inds[m].data, X->nnz * sizeof(uint32_t));
-    if (!r) r = sptolHipDuplicateMemory((void **)&t->vals, X->values.data, X->nnz * sizeof(float));
+        r = dup_on((void **)&t->inds[m], X->inds[m].data, X->nnz * sizeof(uint32_t), s, "DevCoo New");
+    if (!r) r = dup_on((void **)&t->vals, X->values.data, X->nnz * sizeof(float), s, "DevCoo New");
     if (r) { sptolDevCooFree(t); return r; }
     *h = t;
     return 0;
 }
 
-int sptolDevCooPrepareMode(sptolDevCoo h, sptIndex mode)
+int coo_prepare(sptolDevCoo h, uint32_t mode, hipStream_t s)
 {
     if (mode >= h->nmodes) SPTOL_CHECK(SPTERR_VALUE_ERROR, "DevCoo Prepare", "mode out of range");
     SortedCopy &sc = h->sorted[mode];
     if (sc.ready || h->nnz == 0) { sc.ready = true; return 0; }
     if (h->nnz >= 0xFFFFFFFFull) SPTOL_CHECK(SPTERR_VALUE_ERROR, "DevCoo Prepare", "nnz >= 2^32");
     const uint64_t n = h->nnz;
-    hipStream_t s = nullptr;
     uint32_t *perm = nullptr, *perm2 = nullptr, *keys2 = nullptr;
     void *tmp = nullptr;
     size_t tmp_bytes = 0;
@@ -479,6 +498,14 @@ int sptolDevCooPrepareMode(sptolDevCoo h, sptIndex mode)
     sc.ready = true;
     return 0;
 }
+
+}  // namespace sptol
+
+extern "C" {
+
+int sptolDevCooCreate(sptolDevCoo *h, const sptSparseTensor *X) { return coo_create(h, X, nullptr); }
+
+int sptolDevCooPrepareMode(sptolDevCoo h, sptIndex mode) { return coo_prepare(h, mode, nullptr); }
 
 void sptolDevCooFree(sptolDevCoo h)
 {
@@ -517,10 +544,11 @@ static int check_dev_mats(uint32_t nmodes, const uint32_t *ndims, const sptolDev
     return 0;
 }
 
-int sptolDevMTTKRP(sptolDevCoo X, const sptolDevMatrix *mats, const sptIndex mats_order[], sptIndex mode,
-                   sptolDevMatrix *out, void *stream)
+}  // extern "C"
+
+int sptol::coo_mttkrp(sptolDevCoo X, const sptolDevMatrix *mats, const uint32_t *mats_order, uint32_t mode,
+                      sptolDevMatrix *out, hipStream_t s)
 {
-    SPTOL_TRY(check_dev_mats(X->nmodes, X->ndims, mats, mats_order, mode, out, "DevCoo MTTKRP"));
     MttkrpArgs a;
     std::memset(&a, 0, sizeof(a));
     a.nnz = X->nnz;
@@ -536,9 +564,17 @@ int sptolDevMTTKRP(sptolDevCoo X, const sptolDevMatrix *mats, const sptIndex mat
     }
     a.vals = sc.ready ? sc.vals : X->vals;
     a.out = (float *)out->values;
-    hipStream_t s = stream_of(stream);
     SPTOL_TRY(launch_fill_zero(a.out, (uint64_t)X->ndims[mode] * out->stride, s));
     return launch_mttkrp_coo(a, mats[0].dtype, s);
+}
+
+extern "C" {
+
+int sptolDevMTTKRP(sptolDevCoo X, const sptolDevMatrix *mats, const sptIndex mats_order[], sptIndex mode,
+                   sptolDevMatrix *out, void *stream)
+{
+    SPTOL_TRY(check_dev_mats(X->nmodes, X->ndims, mats, mats_order, mode, out, "DevCoo MTTKRP"));
+    return coo_mttkrp(X, mats, mats_order, mode, out, stream_of(stream));
 }
 
 int sptolDevHicooCreate(sptolDevHicoo *h, const sptSparseTensorHiCOO *X)
@@ -563,6 +599,7 @@ int sptolDevHicooCreate(sptolDevHicoo *h, const sptSparseTensorHiCOO *X)
                 const uint64_t base = (uint64_t)X->binds[m].data[b] << X->sb_bits;
                 uint32_t emax = 0;
                 for (uint64_t z = z0; z < z1; ++z) emax = X->einds[m].data[z] > emax ? X->einds[m].data[z] : emax;
+                if (emax >> X->sb_bits) SPTOL_CHECK(SPTERR_VALUE_ERROR, "DevHiCOO New", "element offset >= 2^sb_bits");
                 if (base + emax >= X->ndims[m]) SPTOL_CHECK(SPTERR_VALUE_ERROR, "DevHiCOO New", "index >= ndims");
             }
         }
@@ -589,6 +626,7 @@ int sptolDevHicooCreate(sptolDevHicoo *h, const sptSparseTensorHiCOO *X)
 void sptolDevHicooFree(sptolDevHicoo h)
 {
     if (!h) return;
+    free_hicoo_views(h);
     dfree(h->bptr);
     for (uint32_t m = 0; m < kMaxModes; ++m) { dfree(h->binds[m]); dfree(h->einds[m]); }
     dfree(h->vals);
@@ -619,6 +657,9 @@ int sptolDevMTTKRPHiCOO(sptolDevHicoo X, const sptolDevMatrix *mats, const sptIn
     a.out = (float *)out->values;
     hipStream_t s = stream_of(stream);
     SPTOL_TRY(launch_fill_zero(a.out, (uint64_t)X->ndims[mode] * out->stride, s));
+    int used = 0;
+    SPTOL_TRY(launch_hicoo_tiled(X, a, mats[0].dtype, s, &used));
+    if (used) return 0;
     return launch_mttkrp_hicoo(a, mats[0].dtype, s);
 }
 
@@ -707,7 +748,9 @@ int sptMTTKRPHiCOO(sptSparseTensorHiCOO const * const hitsr, sptMatrix * mats[],
     SPTOL_TRY(check_host_mats(hitsr->nmodes, hitsr->ndims, mats, mode, "GPU HiCOO SpTns MTTKRP"));
     sptolDevHicoo dx = nullptr;
     SPTOL_TRY(sptolDevHicooCreate(&dx, hitsr));
-    int r = with_device_factors(hitsr->nmodes, hitsr->ndims, mats, mode, [&](sptolDevMatrix *dm, sptolDevMatrix *dout) {
+    int r = sptolDevHicooPrepareMode(dx, mode);
+    if (r) { sptolDevHicooFree(dx); return r; }
+    r = with_device_factors(hitsr->nmodes, hitsr->ndims, mats, mode, [&](sptolDevMatrix *dm, sptolDevMatrix *dout) {
         return sptolDevMTTKRPHiCOO(dx, dm, mats_order, mode, dout, nullptr);
     });
     sptolDevHicooFree(dx);

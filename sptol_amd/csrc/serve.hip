@@ -1,0 +1,477 @@
+// Continuous-batching CPD-ALS server.
+//
+// The reference solves one tensor at a time on the host (src/sptensor/cpd.c).
+// Here requests (tensor, rank, iteration cap, tolerance, seed) arrive at any
+// time and are served concurrently on one GPU:
+//   * an admission thread takes queued requests in arrival order, reserves
+//     their device footprint against an HBM budget (waiting while admitted
+//     work holds the budget or all slots are full, rejecting requests that
+//     could never fit), uploads and prepares them on its own stream, and
+//     hands them to the compute thread -- so uploads overlap with compute;
+//   * a compute thread advances every active request by one ALS mode-step
+//     per tick: the MTTKRPs of all of them run as one persistent batched
+//     launch, followed by each request's FP64 solve / normalise / Gram
+//     kernels; at the end of an iteration the request's fit is copied back
+//     behind an event, and the request sits out ticks until the event has
+//     completed, so the host never blocks the device to read a fit;
+//   * finished requests copy their factors to host memory, release their
+//     device memory and budget, and wake the admission thread and waiters.
+// A request that fails (malformed input, allocation failure, budget) is
+// reported with its error code and never affects the others: each request
+// owns its tensor, factors and workspace, and the batched kernel writes only
+// each request's own output.
+#include "cpd_job.hpp"
+#include <atomic>
+#include <condition_variable>
+#include <deque>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+#include <cmath>
+#include <algorithm>
+#include <cstring>
+#include <chrono>
+
+namespace sptol {
+
+using clk = std::chrono::steady_clock;
+
+enum { REQ_QUEUED = 0, REQ_PREPARING = 1, REQ_RUNNING = 2, REQ_DONE = 3, REQ_FAILED = 4 };
+
+struct Request {
+    uint64_t id = 0;
+    // input (host copy, released after upload)
+    uint32_t nmodes = 0;
+    uint32_t ndims[kMaxModes] = {};
+    uint64_t nnz = 0;
+    std::vector<uint32_t> inds[kMaxModes];
+    std::vector<float> vals;
+    uint32_t R = 0, niters = 0;
+    double tol = 0.0;
+    uint64_t seed = 0;
+    int fdt = SPTOL_F32;
+    // state
+    int status = REQ_QUEUED;
+    int error = 0;
+    size_t reserved = 0;
+    bool admitted_once = false;
+    clk::time_point t_submit, t_admit, t_done;
+    // device
+    sptolDevCoo X = nullptr;
+    sptolDevMatrix mats[kMaxModes] = {};
+    CpdJob *job = nullptr;
+    hipEvent_t fit_event = nullptr;
+    // results
+    std::vector<float> lambda;
+    std::vector<std::vector<float>> factors;   // per mode, ndims[m] x stride
+    double fit = 0.0;
+    int iterations = 0;
+
+    void free_device()
+    {
+        delete job;
+        job = nullptr;
+        for (auto &m : mats) { dfree(m.values); m.values = nullptr; }
+        if (X) { sptolDevCooFree(X); X = nullptr; }
+        if (fit_event) { (void)hipEventDestroy(fit_event); fit_event = nullptr; }
+    }
+};
+
+static size_t estimate_bytes(const Request &r)
+{
+    const uint32_t N = r.nmodes;
+    const uint32_t S = ((r.R - 1) / 8 + 1) * 8;
+    size_t b = (size_t)r.nnz * (4 * N + 4) * (1 + N);   // tensor + one sorted copy per mode
+    b += (size_t)r.nnz * 16 + (1u << 20);               // sort temporaries of the preparation
+    for (uint32_t m = 0; m < N; ++m) b += (size_t)r.ndims[m] * S * 4;
+    b += CpdJob::workspace_bytes(r.ndims, N, r.R, r.fdt);
+    return b;
+}
+
+// Seeded U[0,1) factor initialisation local to a request (splitmix64).
+static void seeded_uniform(uint64_t &state, float *dst, size_t n)
+{
+    for (size_t i = 0; i < n; ++i) {
+        uint64_t z = (state += 0x9E3779B97F4A7C15ull);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        z ^= z >> 31;
+        dst[i] = (float)(z >> 40) * (1.0f / 16777216.0f);
+    }
+}
+
+}  // namespace sptol
+
+using namespace sptol;
+
+struct sptolServer_s {
+    sptolServerConfig cfg;
+    std::mutex mu;
+    std::condition_variable cv_admit, cv_compute, cv_done;
+    std::deque<Request *> queued;
+    std::vector<Request *> ready;
+    std::unordered_map<uint64_t, Request *> all;
+    size_t reserved = 0;
+    int admitted = 0;           // preparing + ready + active
+    uint64_t next_id = 1;
+    bool stop = false;
+    std::thread admit_thr, compute_thr;
+    hipStream_t admit_stream = nullptr, compute_stream = nullptr;
+    int device = 0;
+    std::atomic<uint64_t> ticks{0};
+};
+
+namespace sptol {
+
+static void finish_request(sptolServer srv, Request *r, int err)
+{
+    std::lock_guard<std::mutex> lk(srv->mu);
+    r->status = err ? REQ_FAILED : REQ_DONE;
+    r->error = err;
+    r->t_done = clk::now();
+    srv->reserved -= r->reserved;
+    r->reserved = 0;
+    srv->admitted -= 1;
+    srv->cv_admit.notify_all();
+    srv->cv_done.notify_all();
+}
+
+static int prepare_request(sptolServer srv, Request *r)
+{
+    hipStream_t s = srv->admit_stream;
+    // host view of the copied tensor
+    sptIndexVector iv[kMaxModes];
+    sptIndex nd[kMaxModes], so[kMaxModes];
+    for (uint32_t m = 0; m < r->nmodes; ++m) {
+        iv[m].len = iv[m].cap = r->nnz;
+        iv[m].data = r->inds[m].data();
+        nd[m] = r->ndims[m];
+        so[m] = m;
+    }
+    sptSparseTensor X;
+    X.nmodes = r->nmodes;
+    X.sortorder = so;
+    X.ndims = nd;
+    X.nnz = r->nnz;
+    X.inds = iv;
+    X.values.len = X.values.cap = r->nnz;
+    X.values.data = r->vals.data();
+    SPTOL_TRY(coo_create(&r->X, &X, s));
+    for (uint32_t m = 0; m < r->nmodes; ++m) SPTOL_TRY(coo_prepare(r->X, m, s));
+    uint64_t st = r->seed;
+    for (uint32_t m = 0; m < r->nmodes; ++m) {
+        SPTOL_TRY(sptolDevMatrixCreate(&r->mats[m], r->ndims[m], r->R, SPTOL_F32));
+        const uint32_t S = r->mats[m].stride;
+        std::vector<float> h((size_t)r->ndims[m] * S, 0.0f);
+        for (uint32_t i = 0; i < r->ndims[m]; ++i) seeded_uniform(st, h.data() + (size_t)i * S, r->R);
+        SPTOL_HIP(hipMemcpyAsync(r->mats[m].values, h.data(), h.size() * 4, hipMemcpyHostToDevice, s), "Serve upload");
+        SPTOL_HIP(hipStreamSynchronize(s), "Serve upload");
+    }
+    r->job = new CpdJob();
+    r->job->exact_fit = srv->cfg.exact_fit != 0;
+    SPTOL_TRY(r->job->setup(r->X, r->R, r->niters, r->tol, r->mats, r->fdt, s));
+    SPTOL_HIP(hipEventCreateWithFlags(&r->fit_event, hipEventDisableTiming), "Serve");
+    SPTOL_HIP(hipStreamSynchronize(s), "Serve prepare");
+    return 0;
+}
+
+static void admit_loop(sptolServer srv)
+{
+    (void)hipSetDevice(srv->device);
+    for (;;) {
+        Request *r = nullptr;
+        {
+            std::unique_lock<std::mutex> lk(srv->mu);
+            srv->cv_admit.wait(lk, [&] { return srv->stop || !srv->queued.empty(); });
+            if (srv->stop) return;
+            r = srv->queued.front();
+            const size_t need = estimate_bytes(*r);
+            if (need > srv->cfg.hbm_budget) {
+                srv->queued.pop_front();
+                srv->admitted += 1;   // balanced by finish_request
+                lk.unlock();
+                sptol_complain("Serve", SPTERR_CUDA_ERROR + (int)hipErrorOutOfMemory, __FILE__, __LINE__,
+                               "request exceeds the HBM budget");
+                finish_request(srv, r, SPTERR_CUDA_ERROR + (int)hipErrorOutOfMemory);
+                continue;
+            }
+            srv->cv_admit.wait(lk, [&] {
+                return srv->stop || (srv->reserved + need <= srv->cfg.hbm_budget && srv->admitted < srv->cfg.max_active);
+            });
+            if (srv->stop) return;
+            srv->queued.pop_front();
+            srv->reserved += need;
+            r->reserved = need;
+            srv->admitted += 1;
+            r->status = REQ_PREPARING;
+            r->admitted_once = true;
+            r->t_admit = clk::now();
+        }
+        int err = prepare_request(srv, r);
+        for (auto &v : r->inds) std::vector<uint32_t>().swap(v);
+        std::vector<float>().swap(r->vals);
+        if (err) {
+            (void)hipStreamSynchronize(srv->admit_stream);
+            r->free_device();
+            finish_request(srv, r, err);
+            continue;
+        }
+        std::lock_guard<std::mutex> lk(srv->mu);
+        r->status = REQ_RUNNING;
+        srv->ready.push_back(r);
+        srv->cv_compute.notify_all();
+    }
+}
+
+static void complete(sptolServer srv, Request *r)
+{
+    hipStream_t s = srv->compute_stream;
+    r->lambda.resize(r->R);
+    int err = r->job->finish(s, r->lambda.data());
+    if (!err) {
+        r->fit = r->job->fit;
+        r->iterations = r->job->iters;
+        r->factors.resize(r->nmodes);
+        for (uint32_t m = 0; m < r->nmodes && !err; ++m) {
+            r->factors[m].resize((size_t)r->ndims[m] * r->mats[m].stride);
+            hipError_t e = hipMemcpyAsync(r->factors[m].data(), r->mats[m].values, r->factors[m].size() * 4,
+                                          hipMemcpyDeviceToHost, s);
+            if (e == hipSuccess) e = hipStreamSynchronize(s);
+            if (e != hipSuccess) err = SPTERR_CUDA_ERROR + (int)e;
+        }
+    }
+    r->free_device();
+    finish_request(srv, r, err);
+}
+
+static void compute_loop(sptolServer srv)
+{
+    (void)hipSetDevice(srv->device);
+    hipStream_t s = srv->compute_stream;
+    std::vector<Request *> active;
+    BatchBuffers bb;
+    std::vector<BatchDesc> descs[2];
+    std::vector<Request *> stepped;
+    for (;;) {
+        {
+            std::unique_lock<std::mutex> lk(srv->mu);
+            if (active.empty())
+                srv->cv_compute.wait(lk, [&] { return srv->stop || !srv->ready.empty(); });
+            if (srv->stop && srv->ready.empty() && active.empty()) return;
+            for (auto *r : srv->ready) active.push_back(r);
+            srv->ready.clear();
+        }
+        // collect fits that have arrived; finish converged requests
+        bool any_ready = false;
+        for (size_t i = 0; i < active.size();) {
+            Request *r = active[i];
+            if (r->job->fit_pending) {
+                if (hipEventQuery(r->fit_event) != hipSuccess) { ++i; continue; }
+                r->job->fit_ready();
+            }
+            if (r->job->done) {
+                complete(srv, r);
+                active[i] = active.back();
+                active.pop_back();
+                continue;
+            }
+            any_ready = true;
+            ++i;
+        }
+        if (active.empty()) continue;
+        if (!any_ready) {
+            // every active request waits on its fit: block on the oldest
+            for (auto *r : active)
+                if (r->job->fit_pending) { (void)hipEventSynchronize(r->fit_event); break; }
+            continue;
+        }
+        // one tick: the current MTTKRP of every ready request
+        descs[0].clear();
+        descs[1].clear();
+        stepped.clear();
+        for (auto *r : active) {
+            if (r->job->fit_pending || r->job->done) continue;
+            CpdJob &j = *r->job;
+            uint32_t o[kMaxModes];
+            j.order(o);
+            j.Mout.nrows = r->ndims[j.m];
+            if (srv->cfg.batch_mttkrp) {
+                BatchDesc d;
+                make_desc(r->X, j.mt, o, j.m, &j.Mout, &d);
+                descs[j.fdt == SPTOL_BF16 ? 1 : 0].push_back(d);
+            }
+            stepped.push_back(r);
+        }
+        int err = 0;
+        if (srv->cfg.batch_mttkrp) {
+            for (int k = 0; k < 2 && !err; ++k)
+                err = batch_launch(descs[k], k ? SPTOL_BF16 : SPTOL_F32, srv->cfg.tile_scale, srv->cfg.blocks_per_cu, bb, s);
+        }
+        for (auto *r : stepped) {
+            int e = err;
+            if (!e && !srv->cfg.batch_mttkrp) e = r->job->mttkrp(s);
+            if (!e) e = r->job->after_mttkrp(s);
+            if (!e && r->job->fit_pending) {
+                hipError_t he = hipEventRecord(r->fit_event, s);
+                if (he != hipSuccess) e = SPTERR_CUDA_ERROR + (int)he;
+            }
+            if (e) {
+                (void)hipStreamSynchronize(s);
+                r->job->done = true;
+                r->free_device();
+                finish_request(srv, r, e);
+                active.erase(std::find(active.begin(), active.end(), r));
+            }
+        }
+        srv->ticks++;
+    }
+}
+
+}  // namespace sptol
+
+extern "C" {
+
+int sptolServerCreate(sptolServer *out, const sptolServerConfig *cfg)
+{
+    *out = nullptr;
+    if (!cfg || cfg->hbm_budget == 0 || cfg->max_active < 1)
+        SPTOL_CHECK(SPTERR_VALUE_ERROR, "Serve", "hbm_budget > 0 and max_active >= 1 required");
+    sptolServer s = new sptolServer_s();
+    s->cfg = *cfg;
+    if (!(s->cfg.tile_scale > 0)) s->cfg.tile_scale = 1.0;
+    if (s->cfg.blocks_per_cu < 1) s->cfg.blocks_per_cu = 8;
+    (void)hipGetDevice(&s->device);
+    hipError_t e = hipStreamCreateWithFlags(&s->admit_stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->compute_stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete s;
+        SPTOL_HIP(e, "Serve");
+    }
+    s->admit_thr = std::thread(admit_loop, s);
+    s->compute_thr = std::thread(compute_loop, s);
+    *out = s;
+    return 0;
+}
+
+int sptolServerSubmit(sptolServer srv, const sptSparseTensor *X, sptIndex rank, sptIndex niters, double tol,
+                      uint64_t seed, int factor_dtype, uint64_t *id)
+{
+    if (X->nmodes < 2 || X->nmodes > (uint32_t)kMaxModes) SPTOL_CHECK(SPTERR_VALUE_ERROR, "Serve", "nmodes must be in [2, 8]");
+    if (rank < 1 || rank > (uint32_t)kMaxRank) SPTOL_CHECK(SPTERR_VALUE_ERROR, "Serve", "rank must be in [1, 64]");
+    if (factor_dtype != SPTOL_F32 && factor_dtype != SPTOL_BF16) SPTOL_CHECK(SPTERR_VALUE_ERROR, "Serve", "bad dtype");
+    for (uint64_t z = 0; z < X->nnz; ++z)
+        if (!std::isfinite(X->values.data[z])) SPTOL_CHECK(SPTERR_VALUE_ERROR, "Serve", "non-finite value");
+    Request *r = new Request();
+    r->nmodes = X->nmodes;
+    r->nnz = X->nnz;
+    for (uint32_t m = 0; m < X->nmodes; ++m) {
+        r->ndims[m] = X->ndims[m];
+        r->inds[m].assign(X->inds[m].data, X->inds[m].data + X->nnz);
+    }
+    r->vals.assign(X->values.data, X->values.data + X->nnz);
+    r->R = rank;
+    r->niters = niters;
+    r->tol = tol;
+    r->seed = seed;
+    r->fdt = factor_dtype;
+    r->t_submit = clk::now();
+    std::lock_guard<std::mutex> lk(srv->mu);
+    r->id = srv->next_id++;
+    srv->all[r->id] = r;
+    srv->queued.push_back(r);
+    srv->cv_admit.notify_all();
+    *id = r->id;
+    return 0;
+}
+
+int sptolServerWait(sptolServer srv, uint64_t id, double timeout_s, sptolRequestInfo *info)
+{
+    std::unique_lock<std::mutex> lk(srv->mu);
+    auto it = srv->all.find(id);
+    if (it == srv->all.end()) SPTOL_CHECK(SPTERR_VALUE_ERROR, "Serve", "unknown request id");
+    Request *r = it->second;
+    auto pred = [&] { return r->status == REQ_DONE || r->status == REQ_FAILED; };
+    if (timeout_s < 0) srv->cv_done.wait(lk, pred);
+    else if (!srv->cv_done.wait_for(lk, std::chrono::duration<double>(timeout_s), pred)) {
+        if (info) info->status = r->status;
+        return SPTERR_NO_MORE;
+    }
+    if (info) {
+        info->status = r->status;
+        info->error = r->error;
+        info->iterations = r->iterations;
+        info->rank = (int)r->R;
+        info->fit = r->fit;
+        info->latency_s = std::chrono::duration<double>(r->t_done - r->t_submit).count();
+        info->queue_s = r->admitted_once ? std::chrono::duration<double>(r->t_admit - r->t_submit).count()
+                                         : info->latency_s;
+    }
+    return 0;
+}
+
+int sptolServerFetch(sptolServer srv, uint64_t id, sptMatrix **mats, sptValue *lambda)
+{
+    std::lock_guard<std::mutex> lk(srv->mu);
+    auto it = srv->all.find(id);
+    if (it == srv->all.end()) SPTOL_CHECK(SPTERR_VALUE_ERROR, "Serve", "unknown request id");
+    Request *r = it->second;
+    if (r->status != REQ_DONE) SPTOL_CHECK(SPTERR_VALUE_ERROR, "Serve", "request not completed");
+    for (uint32_t m = 0; m < r->nmodes && mats; ++m) {
+        const uint32_t S = ((r->R - 1) / 8 + 1) * 8;
+        if (mats[m]->stride != S || mats[m]->cap < r->ndims[m] || mats[m]->ncols != r->R)
+            SPTOL_CHECK(SPTERR_SHAPE_MISMATCH, "Serve", "factor matrix shape");
+        std::memcpy(mats[m]->values, r->factors[m].data(), r->factors[m].size() * 4);
+        mats[m]->nrows = r->ndims[m];
+    }
+    if (lambda) std::memcpy(lambda, r->lambda.data(), r->R * sizeof(float));
+    return 0;
+}
+
+int sptolServerRelease(sptolServer srv, uint64_t id)
+{
+    std::lock_guard<std::mutex> lk(srv->mu);
+    auto it = srv->all.find(id);
+    if (it == srv->all.end()) return 0;
+    Request *r = it->second;
+    if (r->status != REQ_DONE && r->status != REQ_FAILED) SPTOL_CHECK(SPTERR_VALUE_ERROR, "Serve", "request still running");
+    delete r;
+    srv->all.erase(it);
+    return 0;
+}
+
+int sptolServerDrain(sptolServer srv)
+{
+    std::unique_lock<std::mutex> lk(srv->mu);
+    srv->cv_done.wait(lk, [&] {
+        for (auto &kv : srv->all)
+            if (kv.second->status != REQ_DONE && kv.second->status != REQ_FAILED) return false;
+        return true;
+    });
+    return 0;
+}
+
+uint64_t sptolServerTicks(sptolServer srv) { return srv->ticks.load(); }
+
+void sptolServerDestroy(sptolServer srv)
+{
+    if (!srv) return;
+    sptolServerDrain(srv);
+    {
+        std::lock_guard<std::mutex> lk(srv->mu);
+        srv->stop = true;
+        srv->cv_admit.notify_all();
+        srv->cv_compute.notify_all();
+    }
+    srv->admit_thr.join();
+    srv->compute_thr.join();
+    for (auto &kv : srv->all) {
+        kv.second->free_device();
+        delete kv.second;
+    }
+    (void)hipStreamDestroy(srv->admit_stream);
+    (void)hipStreamDestroy(srv->compute_stream);
+    delete srv;
+}
+
+}  // extern "C"

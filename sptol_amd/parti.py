@@ -79,12 +79,45 @@ class sptolDevMatrix(C.Structure):
 
 
 class sptolCpdStats(C.Structure):
-    _fields_ = [("factor_dtype", C.c_int), ("iterations", C.c_int),
-                ("fit", C.c_double), ("seconds", C.c_double)]
+    _fields_ = [("factor_dtype", C.c_int), ("exact_fit", C.c_int), ("iterations", C.c_int),
+                ("reserved", C.c_int), ("fit", C.c_double), ("seconds", C.c_double)]
+
+
+class sptolServerConfig(C.Structure):
+    _fields_ = [("hbm_budget", C.c_size_t), ("max_active", C.c_int), ("batch_mttkrp", C.c_int),
+                ("exact_fit", C.c_int), ("blocks_per_cu", C.c_int), ("tile_scale", C.c_double)]
+
+
+class sptolRequestInfo(C.Structure):
+    _fields_ = [("status", C.c_int), ("error", C.c_int), ("iterations", C.c_int), ("rank", C.c_int),
+                ("fit", C.c_double), ("latency_s", C.c_double), ("queue_s", C.c_double)]
 
 
 SPTOL_F32 = 0
 SPTOL_BF16 = 1
+SPTOL_REQ_DONE = 3
+SPTOL_REQ_FAILED = 4
+SPTERR_NO_MORE = 99
+
+
+def seeded_uniform_factors(seed, ndims, R):
+    """The factor initialisation of a served request: one splitmix64 stream
+    from `seed`, U[0,1) with 24 random bits, rows of mode 0, then mode 1, ..."""
+    mask = (1 << 64) - 1
+    st = seed & mask
+    out = []
+    for d in ndims:
+        n = int(d) * R
+        a = np.empty(n, dtype=np.float32)
+        for i in range(n):
+            st = (st + 0x9E3779B97F4A7C15) & mask
+            z = st
+            z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & mask
+            z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & mask
+            z ^= z >> 31
+            a[i] = np.float32((z >> 40) * (1.0 / 16777216.0))
+        out.append(a.reshape(int(d), R))
+    return out
 
 
 def ptr(a: np.ndarray, ctype):
@@ -232,6 +265,7 @@ def _declare(lib):
                                      P(sptolDevMatrix), C.c_void_p]),
         "sptolDevHicooCreate": (C.c_int, [P(C.c_void_p), P(sptSparseTensorHiCOO)]),
         "sptolDevHicooFree": (None, [C.c_void_p]),
+        "sptolDevHicooPrepareMode": (C.c_int, [C.c_void_p, sptIndex]),
         "sptolDevMTTKRPHiCOO": (C.c_int, [C.c_void_p, P(sptolDevMatrix), P(sptIndex), sptIndex,
                                           P(sptolDevMatrix), C.c_void_p]),
         "CpdAlsStep": (C.c_double, [P(sptSparseTensor), sptIndex, sptIndex, C.c_double,
@@ -252,6 +286,15 @@ def _declare(lib):
         "sptolHipStreamCreate": (C.c_int, [P(C.c_void_p)]),
         "sptolHipStreamDestroy": (C.c_int, [C.c_void_p]),
         "sptolHipSetDevice": (C.c_int, [C.c_int]),
+        "sptolServerCreate": (C.c_int, [P(C.c_void_p), P(sptolServerConfig)]),
+        "sptolServerSubmit": (C.c_int, [C.c_void_p, P(sptSparseTensor), sptIndex, sptIndex, C.c_double,
+                                        C.c_uint64, C.c_int, P(C.c_uint64)]),
+        "sptolServerWait": (C.c_int, [C.c_void_p, C.c_uint64, C.c_double, P(sptolRequestInfo)]),
+        "sptolServerFetch": (C.c_int, [C.c_void_p, C.c_uint64, P(P(sptMatrix)), P(sptValue)]),
+        "sptolServerRelease": (C.c_int, [C.c_void_p, C.c_uint64]),
+        "sptolServerDrain": (C.c_int, [C.c_void_p]),
+        "sptolServerTicks": (C.c_uint64, [C.c_void_p]),
+        "sptolServerDestroy": (None, [C.c_void_p]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name, None)

@@ -64,9 +64,11 @@ def dev_mttkrp_coo(lib, ndims, inds, vals, factors, order, mode, prepare=False, 
     return res
 
 
-def dev_mttkrp_hicoo(lib, hicoo_struct, ndims, factors, order, mode, dtype=SPTOL_F32):
+def dev_mttkrp_hicoo(lib, hicoo_struct, ndims, factors, order, mode, dtype=SPTOL_F32, prepare=False):
     h = C.c_void_p()
     check(lib, lib.sptolDevHicooCreate(C.byref(h), C.byref(hicoo_struct)), "DevHicooCreate")
+    if prepare:
+        check(lib, lib.sptolDevHicooPrepareMode(h, mode), "DevHicooPrepareMode")
     F = DevFactors(lib, factors, dtype)
     out = DevOut(lib, int(ndims[mode]), factors[0].shape[1])
     o = np.ascontiguousarray(order, dtype=np.uint32)

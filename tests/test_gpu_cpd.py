@@ -37,6 +37,14 @@ def _cpd_step(lib, d, inds, vals, init, iters, tol=-1.0):
 
 
 def test_cpd_step_vs_reference_and_oracle(gpu_lib, golden_cpd):
+    """The reference-method fit reuses the last FP32 MTTKRP for <X,K>; when
+    the fit nears 1, ||X||^2 + ||K||^2 - 2<X,K> cancels and a relative error
+    e of <X,K> becomes an absolute fit error of about sqrt(2e): for FP32
+    (e ~ 1e-7) that is ~5e-4, hence the looser bound on that path.  The
+    exact-fit pass (FP64 products and sums) measures the fit of the FP32
+    factors themselves, whose rounding (~6e-8 relative per entry) bounds
+    ||X - K|| / ||X|| near 1e-6 on an exactly low-rank tensor (the FP64
+    oracle reaches 1 - 1e-14 there): it is held to 1e-5."""
     g = golden_cpd
     for case in g.cases():
         d, inds, vals = g.tensor(case)
@@ -51,7 +59,9 @@ def test_cpd_step_vs_reference_and_oracle(gpu_lib, golden_cpd):
         for m in range(len(d)):
             assert rel_fro(A[m], g.get(case, f"A{m}")) < 2e-3, (case, m)
             assert rel_fro(A[m], om[m]) < 1e-4, (case, m)
-        assert abs(fit - ofits[-1]) < 1e-6, (case, fit, ofits[-1])
+        assert abs(fit - ofits[-1]) < 1e-3, (case, fit, ofits[-1])
+        _, _, st = _device_cpd(gpu_lib, d, inds, vals, init, iters, -1.0, SPTOL_F32, exact_fit=1)
+        assert abs(st.fit - ofits[-1]) < 1e-5, (case, st.fit, ofits[-1])
 
 
 def test_sptcpdals_dropin(gpu_lib):
@@ -85,7 +95,7 @@ def test_sptcpdals_dropin(gpu_lib):
     gpu_lib.sptFreeKruskalTensor(C.byref(kt))
 
 
-def _device_cpd(lib, d, inds, vals, init, iters, tol, dtype):
+def _device_cpd(lib, d, inds, vals, init, iters, tol, dtype, exact_fit=0):
     R = init[0].shape[1]
     coo = CooTensor(d, inds, vals)
     h = C.c_void_p()
@@ -98,6 +108,7 @@ def _device_cpd(lib, d, inds, vals, init, iters, tol, dtype):
     lam = np.zeros(R, np.float32)
     st = sptolCpdStats()
     st.factor_dtype = dtype
+    st.exact_fit = exact_fit
     check(lib, lib.sptolCpdAlsDevice(h, R, iters, tol, dm, ptr(lam, C.c_float), C.byref(st), None), "CpdAlsDevice")
     out = []
     for m in range(len(d)):
@@ -116,7 +127,7 @@ def test_planted_fp32_and_bf16(gpu_lib, R):
     om, olam, ofits = oracle.cpd_als(d, inds, vals, init, iters)
     for dtype in (SPTOL_F32, SPTOL_BF16):
         _say("planted", R, dtype)
-        A, lam, st = _device_cpd(gpu_lib, d, inds, vals, init, iters, -1.0, dtype)
+        A, lam, st = _device_cpd(gpu_lib, d, inds, vals, init, iters, -1.0, dtype, exact_fit=1)
         assert st.iterations == iters
         _say(f"  fit gpu {st.fit:.6f} oracle {ofits[-1]:.6f} diff {abs(st.fit - ofits[-1]):.2e}")
         assert abs(st.fit - ofits[-1]) < 1e-4

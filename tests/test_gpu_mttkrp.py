@@ -176,8 +176,10 @@ def test_dropin_hicoo_vs_reference_golden(gpu_lib, golden_hicoo):
         gpu_lib.sptFreeSparseTensorHiCOO(C.byref(h))
 
 
-@pytest.mark.parametrize("R,dtype", [(32, 0), (16, 0), (32, 1), (8, 0)])
-def test_device_hicoo_zipf(gpu_lib, R, dtype):
+@pytest.mark.parametrize("R,dtype,prepare", [(32, 0, False), (16, 0, False), (32, 1, False), (8, 0, False),
+                                            (32, 0, True), (16, 0, True), (32, 1, True), (8, 0, True),
+                                            (5, 0, True), (64, 0, True), (100, 0, True)])
+def test_device_hicoo_zipf(gpu_lib, R, dtype, prepare):
     d, inds, vals = gen.zipf_tensor([120000, 90000, 280000], 300000, 1.1, seed=R + dtype)
     coo = CooTensor(d, [a.copy() for a in inds], vals.copy())
     h, _ = to_hicoo(gpu_lib, coo, 7, 14)
@@ -185,10 +187,30 @@ def test_device_hicoo_zipf(gpu_lib, R, dtype):
     A = gen.uniform_factors(d, R, seed=5)
     Au = [bf16_round(a) for a in A] if dtype else A
     for mode in range(3):
-        _say("device hicoo", R, dtype, mode)
+        _say("device hicoo", R, dtype, prepare, mode)
         order = [mode, (mode + 1) % 3, (mode + 2) % 3]
-        got = dev_mttkrp_hicoo(gpu_lib, h, d, A, order, mode, dtype=dtype)
+        got = dev_mttkrp_hicoo(gpu_lib, h, d, A, order, mode, dtype=dtype, prepare=prepare)
         ref32 = o.mttkrp(Au, order, mode)
         ref64 = o.mttkrp(Au, order, mode, dtype=np.float64)
         _check(got, ref32, ref64, R)
     gpu_lib.sptFreeSparseTensorHiCOO(C.byref(h))
+
+
+@pytest.mark.parametrize("sb,sk", [(1, 3), (4, 6), (7, 14), (8, 12)])
+def test_hicoo_tiled_block_sizes_4d(gpu_lib, sb, sk):
+    """LDS-tiled HiCOO path across block sizes (tile rows 2..256), 3-D and 4-D,
+    including a heavy output row shared by many work items."""
+    for dims, s in (([20000, 9000, 30000], 1.3), ([3000, 2000, 2500, 400], 1.05)):
+        d, inds, vals = gen.zipf_tensor(dims, 200000, s, seed=sb * 7 + sk)
+        coo = CooTensor(d, [a.copy() for a in inds], vals.copy())
+        h, _ = to_hicoo(gpu_lib, coo, sb, sk)
+        A = gen.uniform_factors(d, 16, seed=3)
+        N = len(d)
+        for mode in range(N):
+            _say("tiled", sb, sk, N, mode)
+            order = [mode] + [(mode + i) % N for i in range(1, N)]
+            got = dev_mttkrp_hicoo(gpu_lib, h, d, A, order, mode, prepare=True)
+            ref32 = oracle.mttkrp_coo(coo.inds, coo.vals, A, order, mode)
+            ref64 = oracle.mttkrp_coo(coo.inds, coo.vals, A, order, mode, dtype=np.float64)
+            _check(got, ref32, ref64, 16)
+        gpu_lib.sptFreeSparseTensorHiCOO(C.byref(h))

@@ -27,7 +27,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 from sptol_amd import gen  # noqa: E402
-from sptol_amd.parti import (CooTensor, Matrix, SPTOL_F32, check, load_library, ptr,  # noqa: E402
+from sptol_amd.parti import (CooTensor, Matrix, SPTOL_BF16, SPTOL_F32, check, load_library, ptr,  # noqa: E402
                              sptIndex, sptolDevMatrix, to_hicoo)
 
 HBM_PEAK = 8.0e12
@@ -126,25 +126,31 @@ def cpu_baseline(lib, coo_inds, vals, dims, factors, order_of, budget_s=20.0):
                       f"{t_tot:.1f} s of CPU time"}
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--nnz", type=float, default=1e8)
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--no-coo", action="store_true", help="skip the COO comparison")
-    args = ap.parse_args()
-    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path (see DESIGN.md, "
-              "'How the path would shard'); run with --gpus 1", file=sys.stderr)
-        sys.exit(2)
+def timed_steps(lib, step, kname, steps, warmup, extra=("fill_zero",)):
+    """Warm up, then time exactly `steps` calls of step() between device
+    synchronisations; kernel durations come from hipEvents recorded on the
+    launch stream (sptolProfile*)."""
+    for _ in range(warmup):
+        step()
+    sync(lib)
+    lib.sptolProfileReset()
+    lib.sptolProfileEnable(1)
+    sync(lib)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    sync(lib)
+    dt = time.perf_counter() - t0
+    lib.sptolProfileEnable(0)
+    kms, kn = prof(lib, kname)
+    out = {"dt": dt, "kernel_ms": kms / max(kn, 1), "launches": kn}
+    for e in extra:
+        ems, en = prof(lib, e)
+        out[e + "_ms"] = ems / max(en, 1)
+    return out
 
-    lib = load_library()
-    n = C.c_int(0)
-    if lib.sptolHipDeviceCount(C.byref(n)) != 0 or n.value < 1:
-        print("no GPU visible to HIP", file=sys.stderr)
-        sys.exit(1)
+
+def run_hicoo(args, lib):
     nnz = int(args.nnz)
     R = 32
     t0 = time.time()
@@ -186,21 +192,8 @@ def main():
                   "DevMTTKRPHiCOO")
 
     def timed(step, kname):
-        for _ in range(args.warmup):
-            step()
-        sync(lib)
-        lib.sptolProfileReset()
-        lib.sptolProfileEnable(1)
-        sync(lib)
-        t0 = time.perf_counter()
-        for _ in range(args.steps):
-            step()
-        sync(lib)
-        dt = time.perf_counter() - t0
-        lib.sptolProfileEnable(0)
-        kms, kn = prof(lib, kname)
-        fms, fn = prof(lib, "fill_zero")
-        return dt, kms / max(kn, 1), fms / max(fn, 1)
+        t = timed_steps(lib, step, kname, args.steps, args.warmup)
+        return t["dt"], t["kernel_ms"], t["fill_zero_ms"]
 
     dt, k_ms, fill_ms = timed(step_hicoo, "mttkrp_hicoo")
     ms_step = dt / args.steps * 1e3
@@ -275,6 +268,333 @@ def main():
         log(f"CPU baseline in {time.time() - t0:.1f}s: {result['cpu_baseline']}")
     else:
         result["cpu_baseline"] = None
+    return result
+
+
+# ---------------------------------------------------------------------------
+# Other workloads (parity-test configurations measured on request)
+# ---------------------------------------------------------------------------
+def base_result(args, metric, value, unit, ms_step, config, higher=True, dtype="f32"):
+    return {"metric": metric, "value": value, "unit": unit, "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": higher, "scaling": "weak",
+            "vs_baseline": None, "dtype": dtype, "data": "synthetic", "config": config}
+
+
+def dev_factors(lib, factors, dtype=SPTOL_F32):
+    arr = (sptolDevMatrix * len(factors))()
+    for m, f in enumerate(factors):
+        hm = Matrix(f)
+        check(lib, lib.sptolDevMatrixCreate(C.byref(arr[m]), f.shape[0], f.shape[1], dtype), "create")
+        check(lib, lib.sptolDevMatrixUpload(C.byref(arr[m]), C.byref(hm.m)), "upload")
+    return arr
+
+
+def free_factors(lib, arr):
+    for m in range(len(arr)):
+        lib.sptolDevMatrixFree(C.byref(arr[m]))
+
+
+def run_coo(args, lib):
+    """3-D COO MTTKRP, mode 0: 2000x1500x1000, 1e6 uniform nonzeros, R=16
+    (the reference driver's configuration, examples/mttkrp.c)."""
+    dims = [2000, 1500, 1000]
+    nnz = int(args.nnz or 1_000_000)
+    R = 16
+    d, inds, vals = gen.uniform_tensor(dims, nnz, seed=2024, sort_fn=lib_sort(lib))
+    nnz = len(vals)
+    factors = gen.uniform_factors(d, R, seed=2025)
+    coo = CooTensor(d, inds, vals)
+    h = C.c_void_p()
+    check(lib, lib.sptolDevCooCreate(C.byref(h), coo.ref()), "DevCooCreate")
+    check(lib, lib.sptolDevCooPrepareMode(h, 0), "prepare")
+    dm = dev_factors(lib, factors)
+    out = sptolDevMatrix()
+    check(lib, lib.sptolDevMatrixCreate(C.byref(out), dims[0], R, SPTOL_F32), "create")
+    order = np.array([0, 1, 2], np.uint32)
+    steps = max(args.steps, 200)
+
+    def step():
+        check(lib, lib.sptolDevMTTKRP(h, dm, ptr(order, sptIndex), 0, C.byref(out), None), "mttkrp")
+
+    t = timed_steps(lib, step, "mttkrp_coo", steps, args.warmup)
+    U = [distinct_count(inds[m], d[m]) for m in range(3)]
+    B = nnz * 16 + (U[1] + U[2]) * R * 4 + dims[0] * R * 4
+    k_ms = t["kernel_ms"]
+    res = base_result(args, "MTTKRP Gnnz/s & %HBM roofline", round(nnz * steps / t["dt"] / 1e9, 4), "Gnnz/s",
+                      round(t["dt"] / steps * 1e3, 5),
+                      {"workload": "coo_mttkrp_mode0", "dims": dims, "nnz": nnz, "R": R, "values": "U(0,1)"})
+    res["steps"] = steps
+    res["roofline"] = {"bound": "hbm", "achieved": round(B / (k_ms * 1e-3) / 1e9, 2), "peak": HBM_PEAK / 1e9,
+                       "unit": "GB/s", "frac": round(B / (k_ms * 1e-3) / HBM_PEAK, 4), "traffic": None,
+                       "kernel": "k_mttkrp_coo", "kernel_avg_ms": round(k_ms, 5), "algorithmic_bytes_per_launch": B,
+                       "fill_zero_avg_ms": round(t["fill_zero_ms"], 5)}
+    if not args.no_cpu:
+        res["cpu_baseline"] = cpu_baseline_coo(inds, vals, factors, order)
+    lib.sptolDevMatrixFree(C.byref(out))
+    free_factors(lib, dm)
+    lib.sptolDevCooFree(h)
+    return res
+
+
+def cpu_baseline_coo(inds, vals, factors, order):
+    """The reference's OpenMP COO MTTKRP sptOmpMTTKRP (mttkrp_omp.c:125) from
+    oracle/_ref, one warm-up and the mean of 5 calls (examples/mttkrp.c:240-283);
+    the oracle's restatement when oracle/_ref is absent."""
+    ncpu = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    tk = max(1, min(16, ncpu))
+    import oracle
+    ts = []
+    for k in range(6):
+        t0 = time.perf_counter()
+        oracle.mttkrp_coo(inds, vals, factors, order, 0, threads=tk)
+        if k:
+            ts.append(time.perf_counter() - t0)
+    nnz = len(vals)
+    return {"value": nnz / float(np.mean(ts)) / 1e9, "unit": "Gnnz/s", "cores": tk, "kind": "port",
+            "sample": f"oracle restatement of sptOmpMTTKRP (atomic, mttkrp_omp.c:125), tk={tk}, the full "
+                      f"{nnz}-nonzero tensor, mean of 5 calls after one warm-up"}
+
+
+def run_cpd(args, lib):
+    """4-D CPD-ALS, 20 iterations, R=32: 5000x5000x5000x500, 5e7 nonzeros,
+    Zipf(1.05) indices, values from a planted rank-32 model plus 1% noise;
+    FP32 factors and BF16-stored factors (FP32 accumulation), FP64 solve and
+    fit.  Reports seconds per iteration and the fit of both variants."""
+    dims = [5000, 5000, 5000, 500]
+    nnz = int(args.nnz or 50_000_000)
+    R, iters = 32, 20
+    t0 = time.time()
+    d, inds, vals, _ = gen.planted_tensor(dims, nnz, R, 1.05, seed=33, sort_fn=lib_sort(lib))
+    nnz = len(vals)
+    log(f"cpd: generated {nnz} nnz in {time.time() - t0:.1f}s")
+    init = gen.uniform_factors(d, R, seed=34)
+    coo = CooTensor(d, inds, vals)
+    h = C.c_void_p()
+    check(lib, lib.sptolDevCooCreate(C.byref(h), coo.ref()), "DevCooCreate")
+    for m in range(4):
+        check(lib, lib.sptolDevCooPrepareMode(h, m), "prepare")
+    out = {}
+    from sptol_amd.parti import sptolCpdStats
+    for name, dt_ in (("f32", SPTOL_F32), ("bf16", SPTOL_BF16)):
+        dm = dev_factors(lib, init)
+        lam = np.zeros(R, np.float32)
+        st = sptolCpdStats()
+        st.factor_dtype = dt_
+        st.exact_fit = 1
+        sync(lib)
+        t0 = time.perf_counter()
+        check(lib, lib.sptolCpdAlsDevice(h, R, iters, -1.0, dm, ptr(lam, C.c_float), C.byref(st), None), "cpd")
+        sync(lib)
+        dt = time.perf_counter() - t0
+        out[name] = {"seconds": dt, "s_per_iter": dt / st.iterations, "fit": st.fit, "iterations": st.iterations}
+        free_factors(lib, dm)
+        log(f"cpd {name}: {out[name]}")
+    lib.sptolDevCooFree(h)
+    res = base_result(args, "CPD-ALS seconds per iteration", round(out["f32"]["s_per_iter"], 5), "s/iter",
+                      round(out["f32"]["s_per_iter"] * 1e3, 4),
+                      {"workload": "cpd_als_4d", "dims": dims, "nnz": nnz, "R": R, "iterations": iters,
+                       "zipf_s": 1.05, "planted_rank": R, "noise": 0.01}, higher=False)
+    res["variants"] = out
+    res["bf16_minus_f32_fit"] = out["bf16"]["fit"] - out["f32"]["fit"]
+    return res
+
+
+def run_batch(args, lib):
+    """256 independent 3-D COO tensors (nnz log-uniform in [1e4, 1e7], dims
+    log-uniform in [1e3, 1e6], R in {8, 16, 32, 64}, random mode): one
+    autotuned persistent launch replayed from a hipGraph versus 256 plain
+    launches in sequence."""
+    rng = np.random.default_rng(404)
+    ntens = 256
+    scale = float(args.nnz or 1.0)   # --nnz scales the nonzero counts (1.0 = as specified)
+    specs = []
+    for t in range(ntens):
+        nnz = int(np.exp(rng.uniform(np.log(1e4), np.log(1e7))) * scale)
+        dims = [int(np.exp(rng.uniform(np.log(1e3), np.log(1e6)))) for _ in range(3)]
+        specs.append((dims, max(nnz, 1), int(rng.choice([8, 16, 32, 64])), int(rng.integers(0, 3)), t))
+    t0 = time.time()
+    items = []
+    total = 0
+    for dims, nnz, R, mode, t in specs:
+        d, inds, vals = gen.uniform_tensor(dims, nnz, seed=5000 + t, sort_fn=lib_sort(lib))
+        coo = CooTensor(d, inds, vals)
+        h = C.c_void_p()
+        check(lib, lib.sptolDevCooCreate(C.byref(h), coo.ref()), "DevCooCreate")
+        check(lib, lib.sptolDevCooPrepareMode(h, mode), "prepare")
+        dm = dev_factors(lib, gen.uniform_factors(d, R, seed=6000 + t))
+        out = sptolDevMatrix()
+        check(lib, lib.sptolDevMatrixCreate(C.byref(out), int(d[mode]), R, SPTOL_F32), "create")
+        order = np.array([mode, (mode + 1) % 3, (mode + 2) % 3], np.uint32)
+        items.append((h, dm, out, order, mode, len(vals)))
+        total += len(vals)
+    log(f"batch: {ntens} tensors, {total} nnz prepared in {time.time() - t0:.1f}s")
+
+    def plain():
+        for h, dm, out, order, mode, _ in items:
+            check(lib, lib.sptolDevMTTKRP(h, dm, ptr(order, sptIndex), mode, C.byref(out), None), "mttkrp")
+
+    tp = timed_steps(lib, plain, "mttkrp_coo", args.steps, args.warmup)
+    b = C.c_void_p()
+    check(lib, lib.sptolBatchCreate(C.byref(b)), "BatchCreate")
+    for h, dm, out, order, mode, _ in items:
+        check(lib, lib.sptolBatchAdd(b, h, dm, ptr(order, sptIndex), mode, C.byref(out)), "BatchAdd")
+    s = C.c_void_p()
+    check(lib, lib.sptolHipStreamCreate(C.byref(s)), "stream")
+    check(lib, lib.sptolBatchAutotune(b, 3, s), "autotune")
+    sc, oc = C.c_double(), C.c_int()
+    lib.sptolBatchGetConfig(b, C.byref(sc), C.byref(oc))
+    check(lib, lib.sptolBatchCapture(b, s), "capture")
+
+    def batched():
+        check(lib, lib.sptolBatchRun(b, s), "BatchRun")
+
+    def sync_s():
+        check(lib, lib.sptolHipStreamSynchronize(s), "sync")
+
+    for _ in range(args.warmup):
+        batched()
+    sync_s()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        batched()
+    sync_s()
+    dtb = time.perf_counter() - t0
+    gb = total * args.steps / dtb / 1e9
+    gp = total * args.steps / tp["dt"] / 1e9
+    res = base_result(args, "batched MTTKRP Gnnz/s", round(gb, 4), "Gnnz/s", round(dtb / args.steps * 1e3, 4),
+                      {"workload": "batched_coo_mttkrp", "tensors": ntens, "total_nnz": total,
+                       "ranks": [8, 16, 32, 64], "nnz_range": [1e4 * scale, 1e7 * scale],
+                       "dims_range": [1e3, 1e6]})
+    res["plain_sequential"] = {"value": round(gp, 4), "unit": "Gnnz/s", "ms_per_step": round(tp["dt"] / args.steps * 1e3, 4),
+                               "launches_per_step": 2 * ntens}
+    res["speedup_vs_plain"] = round(gb / gp, 3)
+    res["autotuned"] = {"tile_scale": sc.value, "blocks_per_cu": oc.value}
+    lib.sptolBatchFree(b)
+    lib.sptolHipStreamDestroy(s)
+    for h, dm, out, _, _, _ in items:
+        lib.sptolDevMatrixFree(C.byref(out))
+        free_factors(lib, dm)
+        lib.sptolDevCooFree(h)
+    return res
+
+
+def run_serve(args, lib):
+    """Served CPD-ALS stream: requests are 3-D or 4-D (50/50) tensors with nnz
+    log-uniform in [1e5, 5e7] (drawn from a pool of 24 pre-generated tensors
+    spanning that range), Zipf(1.1) indices, planted rank-16 values plus 1%
+    noise, R in {16, 32}, stopping at a fit change below 1e-4 or 50
+    iterations; 200 GB admission budget.  The same stream is then run through
+    the plain one-request-at-a-time device path."""
+    from sptol_amd.parti import SPTOL_REQ_DONE, sptolCpdStats, sptolRequestInfo, sptolServerConfig
+    rng = np.random.default_rng(505)
+    npool = 24
+    scale = float(args.nnz or 1.0)
+    pool = []
+    t0 = time.time()
+    for k in range(npool):
+        nnz = int(np.exp(np.log(1e5) + (np.log(5e7) - np.log(1e5)) * (k + 0.5) / npool) * scale)
+        three = k % 2 == 0
+        dims = [20000, 15000, 10000] if three else [5000, 4000, 3000, 400]
+        d, inds, vals, _ = gen.planted_tensor(dims, max(nnz, 1000), 16, 1.1, seed=700 + k, sort_fn=lib_sort(lib))
+        pool.append(CooTensor(d, inds, vals))
+    log(f"serve: pool of {npool} tensors generated in {time.time() - t0:.1f}s")
+    nreq = args.requests
+    reqs = [(int(rng.integers(0, npool)), int(rng.choice([16, 32])), 9000 + i) for i in range(nreq)]
+    gaps = rng.exponential(1.0 / args.rate, size=nreq) if args.rate > 0 else np.zeros(nreq)
+    cfg = sptolServerConfig()
+    cfg.hbm_budget = 200 << 30
+    cfg.max_active = 64
+    cfg.batch_mttkrp = 1
+    cfg.exact_fit = 0
+    srv = C.c_void_p()
+    check(lib, lib.sptolServerCreate(C.byref(srv), C.byref(cfg)), "ServerCreate")
+    ids = []
+    t0 = time.perf_counter()
+    tnext = t0
+    for (p, R, seed), gap in zip(reqs, gaps):
+        tnext += gap
+        now = time.perf_counter()
+        if tnext > now:
+            time.sleep(tnext - now)
+        rid = C.c_uint64()
+        check(lib, lib.sptolServerSubmit(srv, pool[p].ref(), R, 50, 1e-4, seed, SPTOL_F32, C.byref(rid)), "submit")
+        ids.append(rid.value)
+    lat, iters, fits = [], [], []
+    for rid in ids:
+        info = sptolRequestInfo()
+        check(lib, lib.sptolServerWait(srv, rid, -1.0, C.byref(info)), "wait")
+        if info.status == SPTOL_REQ_DONE:
+            lat.append(info.latency_s)
+            iters.append(info.iterations)
+            fits.append(info.fit)
+        lib.sptolServerRelease(srv, rid)
+    t_served = time.perf_counter() - t0
+    lib.sptolServerDestroy(srv)
+    done = len(lat)
+    log(f"serve: {done}/{nreq} done in {t_served:.1f}s")
+    # the plain path: one request at a time (upload, prepare, solve, download)
+    t0 = time.perf_counter()
+    plat = []
+    for (p, R, seed) in reqs:
+        ts = time.perf_counter()
+        coo = pool[p]
+        h = C.c_void_p()
+        check(lib, lib.sptolDevCooCreate(C.byref(h), coo.ref()), "DevCooCreate")
+        init = gen.uniform_factors(coo.ndims, R, seed=seed)
+        dm = dev_factors(lib, init)
+        lam = np.zeros(R, np.float32)
+        st = sptolCpdStats()
+        check(lib, lib.sptolCpdAlsDevice(h, R, 50, 1e-4, dm, ptr(lam, C.c_float), C.byref(st), None), "cpd")
+        for m in range(len(coo.ndims)):
+            hm = Matrix(nrows=int(coo.ndims[m]), ncols=R)
+            check(lib, lib.sptolDevMatrixDownload(C.byref(hm.m), C.byref(dm[m])), "download")
+        free_factors(lib, dm)
+        lib.sptolDevCooFree(h)
+        plat.append(time.perf_counter() - ts)
+    t_plain = time.perf_counter() - t0
+    sps = done / t_served
+    pps = nreq / t_plain
+    res = base_result(args, "served CPD-ALS solves/s", round(sps, 4), "solves/s", round(t_served / max(done, 1) * 1e3, 3),
+                      {"workload": "served_cpd_als", "requests": nreq, "pool": npool,
+                       "nnz_range": [1e5 * scale, 5e7 * scale], "ranks": [16, 32], "max_iters": 50, "tol": 1e-4,
+                       "arrival_rate": args.rate or "all at t=0", "hbm_budget_gb": 200, "slots": cfg.max_active})
+    res["steps"] = nreq
+    res["latency_s"] = {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))} if lat else None
+    res["plain_one_at_a_time"] = {"value": round(pps, 4), "unit": "solves/s",
+                                  "latency_s": {"p50": float(np.percentile(plat, 50)), "p99": float(np.percentile(plat, 99))}}
+    res["speedup_vs_plain"] = round(sps / pps, 3)
+    res["mean_iterations"] = float(np.mean(iters)) if iters else None
+    return res
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="hicoo", choices=["hicoo", "coo", "cpd", "batch", "serve"],
+                    help="hicoo: the headline HiCOO MTTKRP (default); coo: 3-D COO mode 0 (2000x1500x1000, 1e6 nnz); "
+                         "cpd: 4-D CPD-ALS FP32 and BF16; batch: 256 tensors in one persistent launch; "
+                         "serve: CPD request stream through the server")
+    ap.add_argument("--nnz", type=float, default=None, help="override the workload's nonzero count")
+    ap.add_argument("--requests", type=int, default=2000, help="serve: number of requests")
+    ap.add_argument("--rate", type=float, default=0.0, help="serve: Poisson arrival rate (req/s); 0 = all at once")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-coo", action="store_true", help="hicoo: skip the COO comparison")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path (see DESIGN.md, "
+              "'How the path would shard'); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    lib = load_library()
+    n = C.c_int(0)
+    if lib.sptolHipDeviceCount(C.byref(n)) != 0 or n.value < 1:
+        print("no GPU visible to HIP", file=sys.stderr)
+        sys.exit(1)
+    if args.workload == "hicoo" and args.nnz is None:
+        args.nnz = 1e8
+    result = {"hicoo": run_hicoo, "coo": run_coo, "cpd": run_cpd, "batch": run_batch,
+              "serve": run_serve}[args.workload](args, lib)
     print(json.dumps(result), flush=True)
 
 

@@ -126,7 +126,35 @@ def cpu_baseline(lib, coo_inds, vals, dims, factors, order_of, budget_s=20.0):
                       f"{t_tot:.1f} s of CPU time"}
 
 
-def timed_steps(lib, step, kname, steps, warmup, extra=("fill_zero",)):
+KERNEL_SYMBOL = {"mttkrp_seg": "k_mttkrp_seg", "mttkrp_hicoo": "k_mttkrp_hicoo_tiled", "mttkrp_coo": "k_mttkrp_coo"}
+
+
+def alg_bytes(kname, nnz, dims, U, mode, R, nb=0, nk=0, factor_bytes=4):
+    """Algorithmic HBM bytes of one MTTKRP launch on `mode`: the nonzero
+    stream, one read of each distinct factor row gathered (U[m] rows of R
+    values), and the output rows the kernel writes.
+      mttkrp_seg:   16-byte records (N=3; 20 with the value array for N=4)
+                    + the row bitmap; writes all I_n rows (zeros included).
+      mttkrp_hicoo: HiCOO stream (u8 element offsets, values, per-block u32
+                    block coordinates and u64 pointers, kernel pointers);
+                    writes the U[mode] touched rows (the zero fill is a
+                    separate launch).
+      mttkrp_coo:   u32 indices + values; writes the touched rows."""
+    N = len(dims)
+    gather = sum(U[m] * R * factor_bytes for m in range(N) if m != mode)
+    if kname == "mttkrp_seg":
+        stream = nnz * (16 if N == 3 else 20) + (int(dims[mode]) + 31) // 32 * 4
+        write = int(dims[mode]) * R * 4
+    elif kname == "mttkrp_hicoo":
+        stream = nnz * (N + 4) + nb * (4 * N + 8) + (nk + 1) * 8 + nk * 4
+        write = U[mode] * R * 4
+    else:
+        stream = nnz * (4 * N + 4)
+        write = U[mode] * R * 4
+    return stream + gather + write
+
+
+def timed_steps(lib, step, kname, steps, warmup, extra=("fill_zero", "mttkrp_seg_carry")):
     """Warm up, then time exactly `steps` calls of step() between device
     synchronisations; kernel durations come from hipEvents recorded on the
     launch stream (sptolProfile*)."""
@@ -143,7 +171,10 @@ def timed_steps(lib, step, kname, steps, warmup, extra=("fill_zero",)):
     dt = time.perf_counter() - t0
     lib.sptolProfileEnable(0)
     kms, kn = prof(lib, kname)
-    out = {"dt": dt, "kernel_ms": kms / max(kn, 1), "launches": kn}
+    if kn == 0:   # the segmented kernel ran instead
+        kname = "mttkrp_seg"
+        kms, kn = prof(lib, kname)
+    out = {"dt": dt, "kernel_ms": kms / max(kn, 1), "launches": kn, "kernel": kname}
     for e in extra:
         ems, en = prof(lib, e)
         out[e + "_ms"] = ems / max(en, 1)
@@ -191,23 +222,13 @@ def run_hicoo(args, lib):
             check(lib, lib.sptolDevMTTKRPHiCOO(dh, dm, ptr(orders[m], sptIndex), m, C.byref(outs[m]), None),
                   "DevMTTKRPHiCOO")
 
-    def timed(step, kname):
-        t = timed_steps(lib, step, kname, args.steps, args.warmup)
-        return t["dt"], t["kernel_ms"], t["fill_zero_ms"]
-
-    dt, k_ms, fill_ms = timed(step_hicoo, "mttkrp_hicoo")
+    t = timed_steps(lib, step_hicoo, "mttkrp_hicoo", args.steps, args.warmup)
+    dt, k_ms, kname = t["dt"], t["kernel_ms"], t["kernel"]
     ms_step = dt / args.steps * 1e3
     value = 3 * nnz * args.steps / dt / 1e9
-    log(f"HiCOO: {ms_step:.3f} ms/step, {value:.2f} Gnnz/s, kernel avg {k_ms:.3f} ms, fill avg {fill_ms:.3f} ms")
-
-    # algorithmic bytes per launch (averaged over the 3 modes)
-    S_h = nnz * (3 + 4) + nb * (4 * 3 + 8) + (nk + 1) * 8 + nk * 4
-    B = []
-    for mode in range(3):
-        G = sum(U[m] * R * 4 for m in range(3) if m != mode)
-        W = int(dims[mode]) * R * 4
-        B.append(S_h + G + W)
-    B_avg = float(np.mean(B))
+    log(f"HiCOO: {ms_step:.3f} ms/step, {value:.2f} Gnnz/s, {kname} avg {k_ms:.3f} ms, "
+        f"carry avg {t['mttkrp_seg_carry_ms']:.4f} ms, fill avg {t['fill_zero_ms']:.3f} ms")
+    B_avg = float(np.mean([alg_bytes(kname, nnz, dims, U, mode, R, nb=nb, nk=nk) for mode in range(3)]))
     achieved = B_avg / (k_ms * 1e-3)
 
     result = {
@@ -228,8 +249,10 @@ def run_hicoo(args, lib):
                    "nb": nb, "nk": nk, "distinct_per_mode": U},
         "roofline": {"bound": "hbm", "achieved": round(achieved / 1e9, 2), "peak": HBM_PEAK / 1e9,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 4), "traffic": None,
-                     "kernel": "k_mttkrp_hicoo", "kernel_avg_ms": round(k_ms, 4),
-                     "algorithmic_bytes_per_launch": int(B_avg), "fill_zero_avg_ms": round(fill_ms, 4)},
+                     "kernel": KERNEL_SYMBOL[kname], "kernel_avg_ms": round(k_ms, 4),
+                     "algorithmic_bytes_per_launch": int(B_avg),
+                     "carry_kernel_avg_ms": round(t["mttkrp_seg_carry_ms"], 4),
+                     "fill_zero_avg_ms": round(t["fill_zero_ms"], 4)},
     }
 
     if not args.no_coo:
@@ -244,14 +267,13 @@ def run_hicoo(args, lib):
                 check(lib, lib.sptolDevMTTKRP(dc, dm, ptr(orders[m], sptIndex), m, C.byref(outs[m]), None),
                       "DevMTTKRP")
 
-        dt_c, kc_ms, _ = timed(step_coo, "mttkrp_coo")
-        S_c = nnz * (4 * 3 + 4)
-        Bc = float(np.mean([S_c + sum(U[m] * R * 4 for m in range(3) if m != mode) + int(dims[mode]) * R * 4
-                            for mode in range(3)]))
+        tc = timed_steps(lib, step_coo, "mttkrp_coo", args.steps, args.warmup)
+        dt_c, kc_ms = tc["dt"], tc["kernel_ms"]
+        Bc = float(np.mean([alg_bytes(tc["kernel"], nnz, dims, U, mode, R) for mode in range(3)]))
         result["coo_same_shape"] = {
             "value": round(3 * nnz * args.steps / dt_c / 1e9, 4), "unit": "Gnnz/s",
-            "ms_per_step": round(dt_c / args.steps * 1e3, 4), "kernel_avg_ms": round(kc_ms, 4),
-            "roofline_frac": round(Bc / (kc_ms * 1e-3) / HBM_PEAK, 4),
+            "ms_per_step": round(dt_c / args.steps * 1e3, 4), "kernel": KERNEL_SYMBOL[tc["kernel"]],
+            "kernel_avg_ms": round(kc_ms, 4), "roofline_frac": round(Bc / (kc_ms * 1e-3) / HBM_PEAK, 4),
             "layout": "one copy per mode, sorted by that mode on the device"}
         lib.sptolDevCooFree(dc)
         log(f"COO: {result['coo_same_shape']}")
@@ -318,7 +340,7 @@ def run_coo(args, lib):
 
     t = timed_steps(lib, step, "mttkrp_coo", steps, args.warmup)
     U = [distinct_count(inds[m], d[m]) for m in range(3)]
-    B = nnz * 16 + (U[1] + U[2]) * R * 4 + dims[0] * R * 4
+    B = alg_bytes(t["kernel"], nnz, dims, U, 0, R)
     k_ms = t["kernel_ms"]
     res = base_result(args, "MTTKRP Gnnz/s & %HBM roofline", round(nnz * steps / t["dt"] / 1e9, 4), "Gnnz/s",
                       round(t["dt"] / steps * 1e3, 5),
@@ -326,7 +348,8 @@ def run_coo(args, lib):
     res["steps"] = steps
     res["roofline"] = {"bound": "hbm", "achieved": round(B / (k_ms * 1e-3) / 1e9, 2), "peak": HBM_PEAK / 1e9,
                        "unit": "GB/s", "frac": round(B / (k_ms * 1e-3) / HBM_PEAK, 4), "traffic": None,
-                       "kernel": "k_mttkrp_coo", "kernel_avg_ms": round(k_ms, 5), "algorithmic_bytes_per_launch": B,
+                       "kernel": KERNEL_SYMBOL[t["kernel"]], "kernel_avg_ms": round(k_ms, 5),
+                       "algorithmic_bytes_per_launch": B, "carry_kernel_avg_ms": round(t["mttkrp_seg_carry_ms"], 5),
                        "fill_zero_avg_ms": round(t["fill_zero_ms"], 5)}
     if not args.no_cpu:
         res["cpu_baseline"] = cpu_baseline_coo(inds, vals, factors, order)
@@ -567,15 +590,41 @@ def run_serve(args, lib):
     return res
 
 
+def run_copy(args, lib):
+    """Device-to-device copy of 1 GiB with hipMemcpy (the runtime's blit
+    kernel): the known-size reference for scaling rocprofv3 FETCH_SIZE /
+    WRITE_SIZE readings to bytes.  Not a product path."""
+    hip = C.CDLL("libamdhip64.so")
+    hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    n = 1 << 30
+    src, dst = C.c_void_p(), C.c_void_p()
+    check(lib, lib.sptolHipMalloc(C.byref(src), n), "malloc")
+    check(lib, lib.sptolHipMalloc(C.byref(dst), n), "malloc")
+    for _ in range(args.warmup):
+        assert hip.hipMemcpy(dst, src, n, 3) == 0
+    sync(lib)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        assert hip.hipMemcpy(dst, src, n, 3) == 0
+    sync(lib)
+    dt = time.perf_counter() - t0
+    lib.sptolHipFree(src)
+    lib.sptolHipFree(dst)
+    gbs = 2 * n * args.steps / dt / 1e9
+    return base_result(args, "device copy GB/s", round(gbs, 2), "GB/s", round(dt / args.steps * 1e3, 4),
+                       {"workload": "copy_1GiB", "bytes_read_per_launch": n, "bytes_written_per_launch": n},
+                       dtype="u8")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="hicoo", choices=["hicoo", "coo", "cpd", "batch", "serve"],
+    ap.add_argument("--workload", default="hicoo", choices=["hicoo", "coo", "cpd", "batch", "serve", "copy"],
                     help="hicoo: the headline HiCOO MTTKRP (default); coo: 3-D COO mode 0 (2000x1500x1000, 1e6 nnz); "
                          "cpd: 4-D CPD-ALS FP32 and BF16; batch: 256 tensors in one persistent launch; "
-                         "serve: CPD request stream through the server")
+                         "serve: CPD request stream through the server; copy: 1 GiB device copy (counter scale)")
     ap.add_argument("--nnz", type=float, default=None, help="override the workload's nonzero count")
     ap.add_argument("--requests", type=int, default=2000, help="serve: number of requests")
     ap.add_argument("--rate", type=float, default=0.0, help="serve: Poisson arrival rate (req/s); 0 = all at once")
@@ -594,7 +643,7 @@ def main():
     if args.workload == "hicoo" and args.nnz is None:
         args.nnz = 1e8
     result = {"hicoo": run_hicoo, "coo": run_coo, "cpd": run_cpd, "batch": run_batch,
-              "serve": run_serve}[args.workload](args, lib)
+              "serve": run_serve, "copy": run_copy}[args.workload](args, lib)
     print(json.dumps(result), flush=True)
 
 

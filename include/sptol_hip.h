@@ -101,13 +101,26 @@ int  sptolDevMTTKRP(sptolDevCoo X, const sptolDevMatrix *mats, const sptIndex ma
 typedef struct sptolDevHicoo_s *sptolDevHicoo;
 
 int  sptolDevHicooCreate(sptolDevHicoo *h, const sptSparseTensorHiCOO *X);
-/* Build, on the device, a copy of the tensor whose blocks are stably ordered
- * by their mode-th block coordinate (entries move with their blocks) plus
- * the work list of the LDS-tiled kernel; later MTTKRPs on that mode
- * accumulate each 2^sb_bits x R output tile in LDS instead of issuing one
- * global atomic per nonzero.  Needs sb_bits <= 8 and stride <= 64; other
- * shapes keep the block-order kernel. */
+/* Build, on the device, a per-mode view that later MTTKRPs on that mode use
+ * (built once; the HiCOO arrays themselves are not changed).  Two kinds:
+ *  - tiled: blocks stably ordered by their mode-th block coordinate
+ *    (entries move with their blocks) plus the work list of the LDS-tiled
+ *    kernel, which accumulates each 2^sb_bits x R output tile in LDS.  Pays
+ *    off when blocks hold many nonzeros.  Needs sb_bits <= 8, stride <= 64.
+ *  - segmented: coordinates rebuilt from block coordinates and element
+ *    offsets, sorted by the mode, packed 16 bytes per nonzero, for the
+ *    segmented kernel (one plain store per output row, no atomics,
+ *    deterministic).  For 3- and 4-mode tensors with stride <= 64; chosen
+ *    when blocks are sparse (fewer than 8 nonzeros per block on average),
+ *    where blocking saves no index bytes.
+ * Shapes that neither kind handles keep the block-order kernel. */
 int  sptolDevHicooPrepareMode(sptolDevHicoo h, sptIndex mode);
+/* View kind for later sptolDevHicooPrepareMode calls (process-wide):
+ * 0 = by block density (default), 1 = tiled, 2 = segmented. */
+#define SPTOL_HICOO_VIEW_AUTO 0
+#define SPTOL_HICOO_VIEW_TILED 1
+#define SPTOL_HICOO_VIEW_SEG 2
+int  sptolSetHicooViewKind(int kind);
 void sptolDevHicooFree(sptolDevHicoo h);
 int  sptolDevMTTKRPHiCOO(sptolDevHicoo X, const sptolDevMatrix *mats, const sptIndex mats_order[],
                          sptIndex mode, sptolDevMatrix *out, void *stream);

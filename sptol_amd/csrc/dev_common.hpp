@@ -36,12 +36,42 @@ hipStream_t stream_of(void *s);
 int verbose();
 
 constexpr int kMaxModes = 8;
+constexpr uint32_t kNoRow = 0xFFFFFFFFu;
+
+// Compute units of the current device (cached).
+int num_cus();
+
+// Nonzeros sorted by one mode and packed for the segmented MTTKRP kernel
+// (mttkrp_seg.hip), for 3- and 4-mode tensors.
+struct SegView {
+    uint32_t nmodes = 0, mode = 0, nrows = 0;
+    uint64_t nnz = 0;
+    uint4 *rec = nullptr;          // N=3: {i0, i1, i2, bits(v)}; N=4: {i0, i1, i2, i3}
+    float *vals = nullptr;         // N=4: values (owned when own_vals)
+    bool own_vals = false;
+    uint32_t *touched = nullptr;   // bitmap: row r of the mode holds a nonzero
+    float *carry = nullptr;        // per wave: partial row carried into an earlier wave's row
+    uint32_t *carry_row = nullptr; // per wave: that row, or kNoRow
+    uint64_t carry_waves = 0;      // capacity in waves (64 floats each)
+    bool ready = false;
+    void release();
+};
+
+// Pack a mode-sorted SoA copy (inds[0..N-1], vals) into v (allocates).
+int seg_build(SegView &v, uint32_t nmodes, uint32_t mode, uint32_t nrows, uint64_t nnz,
+              uint32_t *const *inds, const float *vals, bool copy_vals, hipStream_t s);
+// Whether the segmented kernel handles this shape.
+bool seg_ok(uint32_t nmodes, uint32_t stride);
+// out[0 .. nrows*stride) = MTTKRP of the view (no fill needed beforehand).
+int seg_mttkrp(SegView &v, const void *const *mats, const uint32_t *order, uint32_t R, uint32_t stride,
+               int dtype, float *out, hipStream_t s);
 
 // Device COO tensor: structure-of-arrays copy of the host tensor, plus
 // optional copies sorted by one mode.
 struct SortedCopy {
     uint32_t *inds[kMaxModes] = {};
     float *vals = nullptr;
+    SegView seg;
     bool ready = false;
 };
 
@@ -140,7 +170,7 @@ int batch_launch(std::vector<BatchDesc> &descs, int dtype, double tile_scale, in
 
 // HiCOO per-mode views (hicoo_view.hip).
 void free_hicoo_views(sptolDevHicoo h);
-int launch_hicoo_tiled(sptolDevHicoo X, const MttkrpArgs &a, int dtype, hipStream_t s, int *used);
+int launch_hicoo_view(sptolDevHicoo X, const MttkrpArgs &a, int dtype, hipStream_t s, int *used);
 
 int launch_mttkrp_coo(const MttkrpArgs &a, int dtype, hipStream_t s);
 int launch_mttkrp_hicoo(const MttkrpArgs &a, int dtype, hipStream_t s);

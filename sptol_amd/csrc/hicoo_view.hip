@@ -34,6 +34,8 @@ struct HicooItem {
 
 // One mode-n view: blocks sorted by binds[n], entries moved with them.
 struct HicooView {
+    SegView seg;         // segmented kind (when use_seg)
+    bool use_seg = false;
     uint64_t *bptr = nullptr;
     uint32_t *binds[kMaxModes] = {};
     uint8_t *einds[kMaxModes] = {};
@@ -49,6 +51,8 @@ struct HicooView {
         dfree(vals); vals = nullptr;
         dfree(items); items = nullptr;
         nitems = 0;
+        seg.release();
+        use_seg = false;
         ready = false;
     }
 };
@@ -395,9 +399,95 @@ int build_hicoo_view(sptolDevHicoo X, uint32_t mode, HicooView &v, hipStream_t s
 
 }  // namespace sptol
 
+namespace sptol {
+
+// Segmented view: one thread per block rebuilds its entries' coordinates.
+__global__ void hv_decompress(const uint64_t *__restrict__ bptr, uint64_t nb, uint32_t nmodes, uint32_t sb,
+                              const MttkrpArgs src, uint32_t **coords)
+{
+    for (uint64_t b = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; b < nb; b += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t z0 = bptr[b], z1 = bptr[b + 1];
+        for (uint32_t m = 0; m < nmodes; ++m) {
+            const uint32_t base = src.binds[m][b] << sb;
+            for (uint64_t z = z0; z < z1; ++z) coords[m][z] = base + (uint32_t)src.einds[m][z];
+        }
+    }
+}
+
+template <typename T>
+__global__ void hv_gather(T *__restrict__ dst, const T *__restrict__ src, const uint32_t *__restrict__ perm, uint64_t n)
+{
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        dst[i] = src[perm[i]];
+}
+
+int build_hicoo_seg(sptolDevHicoo X, uint32_t mode, HicooView &v, hipStream_t s)
+{
+    const uint64_t nb = X->nb, nnz = X->nnz;
+    const uint32_t N = X->nmodes;
+    if (nnz >= 0xFFFFFFFFull) SPTOL_CHECK(SPTERR_VALUE_ERROR, "HiCOO view", "nnz >= 2^32");
+    uint32_t *coords[kMaxModes] = {}, *sorted[kMaxModes] = {};
+    uint32_t *perm = nullptr, *perm2 = nullptr, *key2 = nullptr, **d_coords = nullptr;
+    float *svals = nullptr;
+    void *tmp = nullptr;
+    auto cleanup = [&]() {
+        for (uint32_t m = 0; m < kMaxModes; ++m) { dfree(coords[m]); dfree(sorted[m]); }
+        dfree(perm); dfree(perm2); dfree(key2); dfree(d_coords); dfree(svals); dfree(tmp);
+    };
+    int r = 0;
+#define HS_TRY(x) do { r = (x); if (r) { cleanup(); v.release(); return r; } } while (0)
+#define HS_HIP(x) do { hipError_t e_ = (x); if (e_ != hipSuccess) { cleanup(); v.release(); SPTOL_HIP(e_, "HiCOO view"); } } while (0)
+    for (uint32_t m = 0; m < N; ++m) HS_TRY(dmalloc((void **)&coords[m], nnz * 4, "HiCOO view"));
+    HS_TRY(dmalloc((void **)&d_coords, kMaxModes * sizeof(void *), "HiCOO view"));
+    HS_HIP(hipMemcpyAsync(d_coords, coords, kMaxModes * sizeof(void *), hipMemcpyHostToDevice, s));
+    MttkrpArgs src;
+    std::memset(&src, 0, sizeof(src));
+    for (uint32_t m = 0; m < N; ++m) { src.binds[m] = X->binds[m]; src.einds[m] = X->einds[m]; }
+    hipLaunchKernelGGL(hv_decompress, dim3(hv_grid(nb)), dim3(256), 0, s, X->bptr, nb, N, X->sb_bits, src, d_coords);
+    HS_TRY(dmalloc((void **)&perm, nnz * 4, "HiCOO view"));
+    HS_TRY(dmalloc((void **)&perm2, nnz * 4, "HiCOO view"));
+    HS_TRY(dmalloc((void **)&key2, nnz * 4, "HiCOO view"));
+    hipLaunchKernelGGL(hv_iota, dim3(hv_grid(nnz)), dim3(256), 0, s, perm, nnz);
+    int endbit = 1;
+    while (endbit < 32 && (1ull << endbit) < X->ndims[mode]) ++endbit;
+    size_t tb = 0;
+    HS_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, coords[mode], key2, perm, perm2, (int)nnz, 0, endbit, s));
+    HS_TRY(dmalloc(&tmp, tb ? tb : 4, "HiCOO view"));
+    HS_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, tb, coords[mode], key2, perm, perm2, (int)nnz, 0, endbit, s));
+    dfree(tmp); tmp = nullptr;
+    dfree(key2); key2 = nullptr;
+    dfree(perm); perm = nullptr;
+    for (uint32_t m = 0; m < N; ++m) {
+        HS_TRY(dmalloc((void **)&sorted[m], nnz * 4, "HiCOO view"));
+        hipLaunchKernelGGL(hv_gather<uint32_t>, dim3(hv_grid(nnz)), dim3(256), 0, s, sorted[m], coords[m], perm2, nnz);
+    }
+    HS_TRY(dmalloc((void **)&svals, nnz * 4, "HiCOO view"));
+    hipLaunchKernelGGL(hv_gather<float>, dim3(hv_grid(nnz)), dim3(256), 0, s, svals, X->vals, perm2, nnz);
+    HS_HIP(hipGetLastError());
+    HS_TRY(seg_build(v.seg, N, mode, X->ndims[mode], nnz, sorted, svals, true, s));
+    cleanup();
+    v.use_seg = true;
+    v.ready = true;
+    return 0;
+#undef HS_TRY
+#undef HS_HIP
+}
+
+static int g_view_kind = SPTOL_HICOO_VIEW_AUTO;
+
+}  // namespace sptol
+
 using namespace sptol;
 
 extern "C" {
+
+int sptolSetHicooViewKind(int kind)
+{
+    if (kind < SPTOL_HICOO_VIEW_AUTO || kind > SPTOL_HICOO_VIEW_SEG)
+        SPTOL_CHECK(SPTERR_VALUE_ERROR, "HiCOO view", "unknown view kind");
+    g_view_kind = kind;
+    return 0;
+}
 
 int sptolDevHicooPrepareMode(sptolDevHicoo h, sptIndex mode)
 {
@@ -405,6 +495,11 @@ int sptolDevHicooPrepareMode(sptolDevHicoo h, sptIndex mode)
     if (!h->views) h->views = new sptolHicooViews();
     HicooView &v = h->views->v[mode];
     if (v.ready || h->nnz == 0) return 0;
+    const bool seg_shape = h->nmodes == 3 || h->nmodes == 4;
+    bool use_seg = false;
+    if (g_view_kind == SPTOL_HICOO_VIEW_SEG) use_seg = seg_shape;
+    else if (g_view_kind == SPTOL_HICOO_VIEW_AUTO) use_seg = seg_shape && h->nnz < 8 * h->nb;
+    if (use_seg) return build_hicoo_seg(h, mode, v, nullptr);
     return build_hicoo_view(h, mode, v, nullptr);
 }
 
@@ -420,14 +515,21 @@ void free_hicoo_views(sptolDevHicoo h)
     h->views = nullptr;
 }
 
-// Run the tiled kernel on the mode's view; returns 1 when no view is ready
-// (caller falls back to the plain kernel).
-int launch_hicoo_tiled(sptolDevHicoo X, const MttkrpArgs &in, int dtype, hipStream_t s, int *used)
+// Run the mode's view (segmented, or zero fill + tiled kernel); *used = 0
+// when no view applies (the caller runs the block-order kernel).
+int launch_hicoo_view(sptolDevHicoo X, const MttkrpArgs &in, int dtype, hipStream_t s, int *used)
 {
     *used = 0;
-    if (!X->views || !X->views->v[in.mode].ready || !hicoo_tiled_ok(in.sb_bits, in.stride)) return 0;
-    const HicooView &v = X->views->v[in.mode];
+    if (!X->views || !X->views->v[in.mode].ready) return 0;
+    HicooView &v = X->views->v[in.mode];
+    if (v.use_seg) {
+        if (!seg_ok(in.nmodes, in.stride)) return 0;
+        *used = 1;
+        return seg_mttkrp(v.seg, in.mats, in.order, in.R, in.stride, dtype, in.out, s);
+    }
+    if (!hicoo_tiled_ok(in.sb_bits, in.stride)) return 0;
     *used = 1;
+    SPTOL_TRY(launch_fill_zero(in.out, (uint64_t)X->ndims[in.mode] * in.stride, s));
     if (v.nitems == 0) return 0;
     MttkrpArgs a = in;
     a.bptr = v.bptr;

@@ -276,7 +276,7 @@ __global__ __launch_bounds__(256) void k_mttkrp_hicoo(const MttkrpArgs a)
 // ---------------------------------------------------------------------------
 static int g_num_cus = 0;
 
-static int num_cus()
+int num_cus()
 {
     if (g_num_cus == 0) {
         int dev = 0;
@@ -489,9 +489,12 @@ int coo_prepare(sptolDevCoo h, uint32_t mode, hipStream_t s)
     if (!r) hipLaunchKernelGGL(k_gather<float>, dim3(grid_for(n)), dim3(256), 0, s, sc.vals, h->vals, perm2, n);
     hipError_t e2 = hipStreamSynchronize(s);
     cleanup();
+    if (!r && e2 == hipSuccess && (h->nmodes == 3 || h->nmodes == 4))
+        r = seg_build(sc.seg, h->nmodes, mode, h->ndims[mode], n, sc.inds, sc.vals, false, s);
     if (r) {
         for (uint32_t m = 0; m < h->nmodes; ++m) { dfree(sc.inds[m]); sc.inds[m] = nullptr; }
         dfree(sc.vals); sc.vals = nullptr;
+        sc.seg.release();
         return r;
     }
     SPTOL_HIP(e2, "DevCoo Prepare");
@@ -514,6 +517,7 @@ void sptolDevCooFree(sptolDevCoo h)
         dfree(h->inds[m]);
         for (uint32_t q = 0; q < kMaxModes; ++q) dfree(h->sorted[m].inds[q]);
         dfree(h->sorted[m].vals);
+        h->sorted[m].seg.release();
     }
     dfree(h->vals);
     delete h;
@@ -549,6 +553,12 @@ static int check_dev_mats(uint32_t nmodes, const uint32_t *ndims, const sptolDev
 int sptol::coo_mttkrp(sptolDevCoo X, const sptolDevMatrix *mats, const uint32_t *mats_order, uint32_t mode,
                       sptolDevMatrix *out, hipStream_t s)
 {
+    SortedCopy &sc = X->sorted[mode];
+    if (sc.ready && sc.seg.ready && seg_ok(X->nmodes, out->stride)) {
+        const void *mp[kMaxModes];
+        for (uint32_t m = 0; m < X->nmodes; ++m) mp[m] = mats[m].values;
+        return seg_mttkrp(sc.seg, mp, mats_order, out->ncols, out->stride, mats[0].dtype, (float *)out->values, s);
+    }
     MttkrpArgs a;
     std::memset(&a, 0, sizeof(a));
     a.nnz = X->nnz;
@@ -556,7 +566,6 @@ int sptol::coo_mttkrp(sptolDevCoo X, const sptolDevMatrix *mats, const uint32_t 
     a.mode = mode;
     a.R = out->ncols;
     a.stride = out->stride;
-    const SortedCopy &sc = X->sorted[mode];
     for (uint32_t m = 0; m < X->nmodes; ++m) {
         a.order[m] = mats_order[m];
         a.inds[m] = sc.ready ? sc.inds[m] : X->inds[m];
@@ -656,10 +665,10 @@ int sptolDevMTTKRPHiCOO(sptolDevHicoo X, const sptolDevMatrix *mats, const sptIn
     a.sb_bits = X->sb_bits;
     a.out = (float *)out->values;
     hipStream_t s = stream_of(stream);
-    SPTOL_TRY(launch_fill_zero(a.out, (uint64_t)X->ndims[mode] * out->stride, s));
     int used = 0;
-    SPTOL_TRY(launch_hicoo_tiled(X, a, mats[0].dtype, s, &used));
+    SPTOL_TRY(launch_hicoo_view(X, a, mats[0].dtype, s, &used));
     if (used) return 0;
+    SPTOL_TRY(launch_fill_zero(a.out, (uint64_t)X->ndims[mode] * out->stride, s));
     return launch_mttkrp_hicoo(a, mats[0].dtype, s);
 }
 

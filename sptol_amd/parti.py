@@ -252,6 +252,9 @@ def _declare(lib):
         "sptolHipStreamSynchronize": (C.c_int, [C.c_void_p]),
         "sptolHipDeviceCount": (C.c_int, [P(C.c_int)]),
         "sptolProfileEnable": (None, [C.c_int]),
+        "sptolSetHicooViewKind": (C.c_int, [C.c_int]),
+        "sptolHipMalloc": (C.c_int, [P(C.c_void_p), C.c_size_t]),
+        "sptolHipFree": (C.c_int, [C.c_void_p]),
         "sptolProfileReset": (None, []),
         "sptolProfileQuery": (C.c_int, [C.c_char_p, P(C.c_double), P(C.c_uint64)]),
         "sptolDevMatrixCreate": (C.c_int, [P(sptolDevMatrix), sptIndex, sptIndex, C.c_int]),

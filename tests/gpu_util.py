@@ -64,11 +64,20 @@ def dev_mttkrp_coo(lib, ndims, inds, vals, factors, order, mode, prepare=False, 
     return res
 
 
+VIEW_KIND = {True: 0, "auto": 0, "tiled": 1, "seg": 2}
+
+
 def dev_mttkrp_hicoo(lib, hicoo_struct, ndims, factors, order, mode, dtype=SPTOL_F32, prepare=False):
+    """prepare: False (block-order kernel), True/"auto", "tiled" or "seg"
+    (per-mode view of that kind)."""
     h = C.c_void_p()
     check(lib, lib.sptolDevHicooCreate(C.byref(h), C.byref(hicoo_struct)), "DevHicooCreate")
     if prepare:
-        check(lib, lib.sptolDevHicooPrepareMode(h, mode), "DevHicooPrepareMode")
+        check(lib, lib.sptolSetHicooViewKind(VIEW_KIND[prepare]), "SetHicooViewKind")
+        try:
+            check(lib, lib.sptolDevHicooPrepareMode(h, mode), "DevHicooPrepareMode")
+        finally:
+            lib.sptolSetHicooViewKind(0)
     F = DevFactors(lib, factors, dtype)
     out = DevOut(lib, int(ndims[mode]), factors[0].shape[1])
     o = np.ascontiguousarray(order, dtype=np.uint32)

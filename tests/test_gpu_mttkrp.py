@@ -177,8 +177,10 @@ def test_dropin_hicoo_vs_reference_golden(gpu_lib, golden_hicoo):
 
 
 @pytest.mark.parametrize("R,dtype,prepare", [(32, 0, False), (16, 0, False), (32, 1, False), (8, 0, False),
-                                            (32, 0, True), (16, 0, True), (32, 1, True), (8, 0, True),
-                                            (5, 0, True), (64, 0, True), (100, 0, True)])
+                                            (32, 0, "tiled"), (16, 0, "tiled"), (32, 1, "tiled"), (8, 0, "tiled"),
+                                            (5, 0, "tiled"), (64, 0, "tiled"), (100, 0, "tiled"),
+                                            (32, 0, "seg"), (16, 0, "seg"), (32, 1, "seg"), (8, 0, "seg"),
+                                            (5, 0, "seg"), (64, 0, "seg"), (100, 0, "seg"), (24, 1, "seg")])
 def test_device_hicoo_zipf(gpu_lib, R, dtype, prepare):
     d, inds, vals = gen.zipf_tensor([120000, 90000, 280000], 300000, 1.1, seed=R + dtype)
     coo = CooTensor(d, [a.copy() for a in inds], vals.copy())
@@ -196,10 +198,12 @@ def test_device_hicoo_zipf(gpu_lib, R, dtype, prepare):
     gpu_lib.sptFreeSparseTensorHiCOO(C.byref(h))
 
 
+@pytest.mark.parametrize("kind", ["tiled", "seg"])
 @pytest.mark.parametrize("sb,sk", [(1, 3), (4, 6), (7, 14), (8, 12)])
-def test_hicoo_tiled_block_sizes_4d(gpu_lib, sb, sk):
-    """LDS-tiled HiCOO path across block sizes (tile rows 2..256), 3-D and 4-D,
-    including a heavy output row shared by many work items."""
+def test_hicoo_tiled_block_sizes_4d(gpu_lib, sb, sk, kind):
+    """HiCOO per-mode views across block sizes (tile rows 2..256), 3-D and 4-D,
+    including a heavy output row shared by many work items (tiled) or
+    spanning many waves (segmented)."""
     for dims, s in (([20000, 9000, 30000], 1.3), ([3000, 2000, 2500, 400], 1.05)):
         d, inds, vals = gen.zipf_tensor(dims, 200000, s, seed=sb * 7 + sk)
         coo = CooTensor(d, [a.copy() for a in inds], vals.copy())
@@ -207,10 +211,34 @@ def test_hicoo_tiled_block_sizes_4d(gpu_lib, sb, sk):
         A = gen.uniform_factors(d, 16, seed=3)
         N = len(d)
         for mode in range(N):
-            _say("tiled", sb, sk, N, mode)
+            _say(kind, sb, sk, N, mode)
             order = [mode] + [(mode + i) % N for i in range(1, N)]
-            got = dev_mttkrp_hicoo(gpu_lib, h, d, A, order, mode, prepare=True)
+            got = dev_mttkrp_hicoo(gpu_lib, h, d, A, order, mode, prepare=kind)
             ref32 = oracle.mttkrp_coo(coo.inds, coo.vals, A, order, mode)
             ref64 = oracle.mttkrp_coo(coo.inds, coo.vals, A, order, mode, dtype=np.float64)
             _check(got, ref32, ref64, 16)
         gpu_lib.sptFreeSparseTensorHiCOO(C.byref(h))
+
+
+def test_segmented_kernel_deterministic_and_rowwise_exact(gpu_lib):
+    """The segmented kernel (sorted copy, 3-D and 4-D) is deterministic: two
+    runs agree bit for bit.  Rows whose nonzeros all fall inside one lane
+    group's range are summed in the reference's order, so with short rows
+    nearly all rows equal the sequential FP32 reference bit for bit."""
+    for dims, nnz in (([50000, 400, 300], 60000), ([30000, 60, 50, 40], 50000)):
+        d, inds, vals = gen.uniform_tensor(dims, nnz, seed=11)
+        A = gen.uniform_factors(d, 32, seed=2)
+        N = len(d)
+        h = dev_coo(gpu_lib, d, inds, vals)
+        order = [0] + list(range(1, N))
+        _say("seg determinism", N)
+        a = dev_mttkrp_coo(gpu_lib, d, inds, vals, A, order, 0, prepare=True, h=h)
+        b = dev_mttkrp_coo(gpu_lib, d, inds, vals, A, order, 0, prepare=True, h=h)
+        gpu_lib.sptolDevCooFree(h)
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+        ref32 = oracle.mttkrp_coo(inds, vals, A, order, 0)
+        exact = np.all(a[:, :32] == ref32[:, :32], axis=1)
+        frac = exact.mean()
+        _say("  rows bit-exact vs sequential reference:", frac)
+        assert frac > 0.95, frac
+        _check(a, ref32, oracle.mttkrp_coo(inds, vals, A, order, 0, dtype=np.float64), 32)

@@ -125,6 +125,14 @@ void sptolDevHicooFree(sptolDevHicoo h);
 int  sptolDevMTTKRPHiCOO(sptolDevHicoo X, const sptolDevMatrix *mats, const sptIndex mats_order[],
                          sptIndex mode, sptolDevMatrix *out, void *stream);
 
+/* COO -> HiCOO conversion on the GPU.  Same contract and the same arrays,
+ * bit for bit, as sptSparseTensorToHiCOO (convert.c:300, ParTI.h): fills
+ * `hitsr` (host arrays), permutes `tsr` in place into HiCOO order, returns
+ * the largest block's nonzero count in *max_nnzb (may be NULL).  Needs
+ * nnz < 2^31; the work runs on the default stream and is synchronous. */
+int  sptolSparseTensorToHiCOODevice(sptSparseTensorHiCOO *hitsr, sptNnzIndex *max_nnzb, sptSparseTensor *tsr,
+                                    sptElementIndex sb_bits, sptElementIndex sk_bits);
+
 /* ---------------------------------------------------------------------- */
 /* Batched MTTKRP: many device tensors in one persistent launch           */
 /* ---------------------------------------------------------------------- */

@@ -3,17 +3,26 @@
 // The reference runs one tensor per call and copies it to the device each
 // time (src/sptensor/mttkrp_cuda_onekernel.cu:100-149).  Here a batch of
 // device-resident tensors (each with its own factors, rank, mode and output)
-// is cut into tiles of consecutive nonzeros; one persistent kernel, sized to
-// the chip, pulls tiles from a global work queue (one atomic per tile), so
-// small and large tensors and different ranks share the launch without idle
-// CUs.  A preceding fill kernel zeroes every output and resets the queue, and
-// the pair can be captured once into a hipGraph and replayed.
+// runs as persistent kernels sized to the chip that pull work from global
+// counters, so small and large tensors and different ranks share the launch
+// without idle CUs, and the launch can be captured once into a hipGraph and
+// replayed.
 //
-// The tile length per rank class (nonzeros per tile) and the number of
-// resident blocks per CU come from an autotuner that times candidate
-// configurations on the batch itself and caches the winner per batch
-// signature.
-#include "dev_common.hpp"
+//  * Tensors with a mode-sorted view (sptolDevCooPrepareMode; 3- or 4-mode,
+//    stride <= 64) run the segmented range body of seg_kernel.hpp: one
+//    persistent kernel per class of (lanes per nonzero, modes) takes ranges
+//    of any tensor of its class from the class counter; one carry kernel over
+//    every tensor then finishes rows that cross ranges and resets the
+//    counters.  Every output row is written exactly once (no zero fill, no
+//    atomics) and results are deterministic.
+//  * Other tensors take the atomic path: a fill kernel zeroes their outputs
+//    and resets its queue, then a persistent kernel pulls tiles of
+//    consecutive nonzeros and flushes row runs with float atomics.
+//
+// The range length (nonzeros per lane group) and the number of resident
+// blocks per CU come from an autotuner that times candidate configurations
+// on the batch itself and caches the winner per batch signature.
+#include "seg_kernel.hpp"
 #include <vector>
 #include <map>
 #include <algorithm>
@@ -148,6 +157,195 @@ __global__ __launch_bounds__(256) void k_batch_mttkrp(const BatchDesc *__restric
     }
 }
 
+// ---------------------------------------------------------------------------
+// Segmented part
+// ---------------------------------------------------------------------------
+constexpr int kBatchWPB = 4;
+
+template <int LPN, int NM, typename FT>
+__global__ __launch_bounds__(kBatchWPB * 64) void k_batch_seg(const SegArgs *__restrict__ items,
+                                                              const uint64_t *__restrict__ prefix, uint32_t n,
+                                                              uint64_t total, uint32_t *__restrict__ counter)
+{
+    __shared__ SegShared<LPN, NM, kBatchWPB> sh;
+    const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;
+    for (;;) {
+        uint32_t wq = 0;
+        if (lane == 0) wq = atomicAdd(counter, 1u);
+        const uint64_t t = __builtin_amdgcn_readfirstlane(wq);
+        if (t >= total) break;
+        uint32_t lo = 0, hi = n;   // prefix[lo] <= t < prefix[hi]
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (prefix[mid] <= t) lo = mid; else hi = mid;
+        }
+        const SegArgs a = items[lo];
+        seg_range<LPN, NM, FT, kBatchWPB>(a, t - prefix[lo], sh, wv, lane);
+    }
+}
+
+// Carry blocks of every segmented item (class order); block 0 also resets
+// the class counters for the next launch.
+__global__ __launch_bounds__(256) void k_batch_seg_carry(const SegArgs *__restrict__ items,
+                                                         const uint64_t *__restrict__ cprefix, uint32_t n,
+                                                         uint32_t *__restrict__ counters)
+{
+    __shared__ uint32_t rows[kSegCarrySlots + 2];
+    __shared__ float4 part[256];
+    if (blockIdx.x == 0 && threadIdx.x < 8) counters[threadIdx.x] = 0u;
+    const uint64_t b = blockIdx.x;
+    uint32_t lo = 0, hi = n;   // cprefix[lo] <= b < cprefix[hi]
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (cprefix[mid] <= b) lo = mid; else hi = mid;
+    }
+    const SegArgs a = items[lo];
+    const uint64_t k0 = (b - cprefix[lo]) * kSegCarrySlots;
+    const uint64_t k1 = min<uint64_t>(k0 + kSegCarrySlots, a.nwaves);
+    seg_carry_slots(a.carry, a.carry_row, a.nwaves, a.out, a.S, a.lpn, k0, k1, rows, part);
+}
+
+SegBatchDev::~SegBatchDev()
+{
+    dfree(arena); dfree(carry); dfree(carry_row); dfree(counters);
+}
+
+static int seg_class(uint32_t lpn, uint32_t nmodes)
+{
+    const int li = lpn <= 2 ? 0 : (lpn == 4 ? 1 : (lpn == 8 ? 2 : 3));
+    return li * 2 + (nmodes == 4 ? 1 : 0);
+}
+
+// Nonzeros per lane group of a batch: 256 scaled by the tuned factor.
+static uint32_t seg_tile_chg(double scale)
+{
+    double c = 256.0 * scale;
+    if (c < 64) c = 64;
+    if (c > 2048) c = 2048;
+    return ((uint32_t)c + 7) / 8 * 8;
+}
+
+// Plan and upload the segmented items (stream-ordered copies from a host
+// staging buffer that the runtime stages before returning).
+static int seg_batch_prepare(const std::vector<const BatchDesc *> &items, double scale, SegBatchDev &dev, hipStream_t s)
+{
+    std::vector<SegArgs> cls[8];
+    const uint32_t chg = seg_tile_chg(scale);
+    uint64_t ranges = 0;
+    for (const BatchDesc *d : items) {
+        SegArgs a;
+        SPTOL_TRY(seg_fill_args(*d->seg, d->mats, d->order, d->R, d->stride, d->out, chg, a));
+        ranges += a.nwaves;
+        cls[seg_class(a.lpn, a.nmodes)].push_back(a);
+    }
+    if (dev.carry_cap < ranges) {
+        dfree(dev.carry); dev.carry = nullptr;
+        dfree(dev.carry_row); dev.carry_row = nullptr;
+        dev.carry_cap = 0;
+        SPTOL_TRY(dmalloc((void **)&dev.carry, ranges * kSegCarryStride * 4, "Batch"));
+        SPTOL_TRY(dmalloc((void **)&dev.carry_row, ranges * 4, "Batch"));
+        dev.carry_cap = ranges;
+    }
+    if (!dev.counters) {
+        SPTOL_TRY(dmalloc((void **)&dev.counters, 8 * 4, "Batch"));
+        SPTOL_HIP(hipMemsetAsync(dev.counters, 0, 8 * 4, s), "Batch");
+    }
+    // carry space, in class order
+    std::vector<SegArgs> all;
+    std::vector<uint64_t> cprefix(1, 0);
+    uint64_t off = 0;
+    for (int c = 0; c < 8; ++c)
+        for (SegArgs &a : cls[c]) {
+            a.carry = dev.carry + off * kSegCarryStride;
+            a.carry_row = dev.carry_row + off;
+            off += a.nwaves;
+            all.push_back(a);
+            cprefix.push_back(cprefix.back() + (a.nwaves + kSegCarrySlots - 1) / kSegCarrySlots);
+        }
+    // staging: per class items + prefix, then all items + carry prefix
+    std::vector<uint8_t> host;
+    auto put = [&](const void *p, size_t bytes) {
+        const size_t at = (host.size() + 255) / 256 * 256;
+        host.resize(at + bytes);
+        if (bytes) std::memcpy(host.data() + at, p, bytes);
+        return at;
+    };
+    for (int c = 0; c < 8; ++c) {
+        std::vector<uint64_t> pre(1, 0);
+        for (const SegArgs &a : cls[c]) pre.push_back(pre.back() + a.nwaves);
+        dev.n[c] = (uint32_t)cls[c].size();
+        dev.total[c] = pre.back();
+        if (dev.total[c] > 0xFFFFFFFFull) SPTOL_CHECK(SPTERR_VALUE_ERROR, "Batch", "too many ranges in one class");
+        dev.off_items[c] = put(cls[c].data(), cls[c].size() * sizeof(SegArgs));
+        dev.off_prefix[c] = put(pre.data(), pre.size() * 8);
+    }
+    dev.n_all = (uint32_t)all.size();
+    dev.carry_blocks = cprefix.back();
+    dev.off_all = put(all.data(), all.size() * sizeof(SegArgs));
+    dev.off_cprefix = put(cprefix.data(), cprefix.size() * 8);
+    if (dev.arena_cap < host.size()) {
+        dfree(dev.arena);
+        dev.arena = nullptr;
+        dev.arena_cap = 0;
+        const size_t cap = host.size() * 2;
+        SPTOL_TRY(dmalloc((void **)&dev.arena, cap, "Batch"));
+        dev.arena_cap = cap;
+    }
+    SPTOL_HIP(hipMemcpyAsync(dev.arena, host.data(), host.size(), hipMemcpyHostToDevice, s), "Batch");
+    return 0;
+}
+
+template <int LPN, int NM, typename FT>
+static void seg_batch_class(const SegBatchDev &dev, int c, int blocks_per_cu, hipStream_t s)
+{
+    const void *fn = reinterpret_cast<const void *>(&k_batch_seg<LPN, NM, FT>);
+    int occ = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fn, kBatchWPB * 64, 0) != hipSuccess || occ < 1) occ = 1;
+    const int bpc = blocks_per_cu < occ ? blocks_per_cu : occ;
+    uint64_t blocks = (uint64_t)num_cus() * (uint64_t)bpc;
+    const uint64_t need = (dev.total[c] + kBatchWPB - 1) / kBatchWPB;
+    if (blocks > need) blocks = need;
+    hipLaunchKernelGGL((k_batch_seg<LPN, NM, FT>), dim3((unsigned)blocks), dim3(kBatchWPB * 64), 0, s,
+                       (const SegArgs *)(dev.arena + dev.off_items[c]), (const uint64_t *)(dev.arena + dev.off_prefix[c]),
+                       dev.n[c], dev.total[c], dev.counters + c);
+}
+
+template <typename FT>
+static void seg_batch_kernels_ft(const SegBatchDev &dev, int bpc, hipStream_t s)
+{
+    for (int c = 0; c < 8; ++c) {
+        if (!dev.n[c]) continue;
+        switch (c) {
+        case 0: seg_batch_class<2, 3, FT>(dev, c, bpc, s); break;
+        case 1: seg_batch_class<2, 4, FT>(dev, c, bpc, s); break;
+        case 2: seg_batch_class<4, 3, FT>(dev, c, bpc, s); break;
+        case 3: seg_batch_class<4, 4, FT>(dev, c, bpc, s); break;
+        case 4: seg_batch_class<8, 3, FT>(dev, c, bpc, s); break;
+        case 5: seg_batch_class<8, 4, FT>(dev, c, bpc, s); break;
+        case 6: seg_batch_class<16, 3, FT>(dev, c, bpc, s); break;
+        default: seg_batch_class<16, 4, FT>(dev, c, bpc, s); break;
+        }
+    }
+}
+
+// Class kernels then the carry kernel (which resets the class counters).
+static int seg_batch_kernels(const SegBatchDev &dev, int dtype, int blocks_per_cu, hipStream_t s)
+{
+    if (!dev.n_all) return 0;
+    int tok = prof_start("batch_seg", s);
+    if (dtype == SPTOL_BF16) seg_batch_kernels_ft<uint16_t>(dev, blocks_per_cu, s);
+    else seg_batch_kernels_ft<float>(dev, blocks_per_cu, s);
+    prof_stop(tok, s);
+    SPTOL_HIP(hipGetLastError(), "Batch launch");
+    tok = prof_start("batch_seg_carry", s);
+    hipLaunchKernelGGL(k_batch_seg_carry, dim3((unsigned)dev.carry_blocks), dim3(256), 0, s,
+                       (const SegArgs *)(dev.arena + dev.off_all), (const uint64_t *)(dev.arena + dev.off_cprefix),
+                       dev.n_all, dev.counters);
+    prof_stop(tok, s);
+    SPTOL_HIP(hipGetLastError(), "Batch carry launch");
+    return 0;
+}
+
 struct TuneCfg {
     double tile_scale = 1.0;     // multiplies the base tile length
     int blocks_per_cu = 8;
@@ -172,10 +370,14 @@ struct sptolBatch_s {
     int dtype = -1;
     TuneCfg cfg;
     bool planned = false;
+    // atomic path (tensors without a usable mode-sorted view)
+    uint32_t nlegacy = 0;
     BatchDesc *d_desc = nullptr;
     uint64_t *d_tile_prefix = nullptr, *d_fill_prefix = nullptr;
     uint32_t *d_counter = nullptr;
     uint64_t ntiles = 0, nchunks = 0;
+    // segmented path
+    SegBatchDev seg;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     hipStream_t graph_stream = nullptr;
@@ -199,51 +401,79 @@ static uint64_t tile_len(uint32_t stride, double scale)
     return t < 64 ? 64 : t;
 }
 
+static void split_descs(const std::vector<BatchDesc> &descs, std::vector<const BatchDesc *> &seg,
+                        std::vector<BatchDesc> &legacy)
+{
+    seg.clear();
+    legacy.clear();
+    for (const BatchDesc &d : descs) {
+        if (d.seg) seg.push_back(&d);
+        else legacy.push_back(d);
+    }
+}
+
 static int build_plan(sptolBatch b)
 {
-    const size_t n = b->descs.size();
-    std::vector<uint64_t> tp(n + 1, 0), fp(n + 1, 0);
-    for (size_t i = 0; i < n; ++i) {
-        BatchDesc &d = b->descs[i];
-        d.tile_nnz = tile_len(d.stride, b->cfg.tile_scale);
-        tp[i + 1] = tp[i] + (d.nnz + d.tile_nnz - 1) / d.tile_nnz;
-        fp[i + 1] = fp[i] + (d.out_len + 65535) / 65536;
-    }
-    if (tp[n] >= 0xFFFFFFFFull) SPTOL_CHECK(SPTERR_VALUE_ERROR, "Batch Plan", "too many tiles");
-    b->ntiles = tp[n];
-    b->nchunks = fp[n];
-    dfree(b->d_desc); dfree(b->d_tile_prefix); dfree(b->d_fill_prefix);
-    b->d_desc = nullptr; b->d_tile_prefix = nullptr; b->d_fill_prefix = nullptr;
-    SPTOL_TRY(sptolHipDuplicateMemory((void **)&b->d_desc, b->descs.data(), n * sizeof(BatchDesc)));
-    SPTOL_TRY(sptolHipDuplicateMemory((void **)&b->d_tile_prefix, tp.data(), (n + 1) * 8));
-    SPTOL_TRY(sptolHipDuplicateMemory((void **)&b->d_fill_prefix, fp.data(), (n + 1) * 8));
-    if (!b->d_counter) SPTOL_TRY(dmalloc((void **)&b->d_counter, 4, "Batch Plan"));
+    std::vector<const BatchDesc *> segs;
+    std::vector<BatchDesc> leg;
+    split_descs(b->descs, segs, leg);
     if (b->exec) { (void)hipGraphExecDestroy(b->exec); b->exec = nullptr; }
     if (b->graph) { (void)hipGraphDestroy(b->graph); b->graph = nullptr; }
+    SPTOL_TRY(seg_batch_prepare(segs, b->cfg.tile_scale, b->seg, nullptr));
+    const size_t n = leg.size();
+    b->nlegacy = (uint32_t)n;
+    dfree(b->d_desc); dfree(b->d_tile_prefix); dfree(b->d_fill_prefix);
+    b->d_desc = nullptr; b->d_tile_prefix = nullptr; b->d_fill_prefix = nullptr;
+    b->ntiles = b->nchunks = 0;
+    if (n) {
+        std::vector<uint64_t> tp(n + 1, 0), fp(n + 1, 0);
+        for (size_t i = 0; i < n; ++i) {
+            BatchDesc &d = leg[i];
+            d.tile_nnz = tile_len(d.stride, b->cfg.tile_scale);
+            tp[i + 1] = tp[i] + (d.nnz + d.tile_nnz - 1) / d.tile_nnz;
+            fp[i + 1] = fp[i] + (d.out_len + 65535) / 65536;
+        }
+        if (tp[n] >= 0xFFFFFFFFull) SPTOL_CHECK(SPTERR_VALUE_ERROR, "Batch Plan", "too many tiles");
+        b->ntiles = tp[n];
+        b->nchunks = fp[n];
+        SPTOL_TRY(sptolHipDuplicateMemory((void **)&b->d_desc, leg.data(), n * sizeof(BatchDesc)));
+        SPTOL_TRY(sptolHipDuplicateMemory((void **)&b->d_tile_prefix, tp.data(), (n + 1) * 8));
+        SPTOL_TRY(sptolHipDuplicateMemory((void **)&b->d_fill_prefix, fp.data(), (n + 1) * 8));
+        if (!b->d_counter) SPTOL_TRY(dmalloc((void **)&b->d_counter, 4, "Batch Plan"));
+    }
+    SPTOL_HIP(hipStreamSynchronize(nullptr), "Batch Plan");
     b->planned = true;
+    return 0;
+}
+
+static int launch_legacy(const BatchDesc *d_desc, const uint64_t *d_tile_prefix, const uint64_t *d_fill_prefix,
+                         uint32_t *d_counter, uint32_t n, uint64_t ntiles, uint64_t nchunks, int dtype,
+                         int blocks_per_cu, hipStream_t s)
+{
+    if (!n) return 0;
+    const int cus = cu_count();
+    unsigned fill_grid = (unsigned)std::min<uint64_t>(std::max<uint64_t>(nchunks, 1), (uint64_t)cus * 8);
+    int tok = prof_start("batch_fill", s);
+    hipLaunchKernelGGL(k_batch_fill, dim3(fill_grid), dim3(256), 0, s, d_desc, d_fill_prefix, n, nchunks, d_counter);
+    prof_stop(tok, s);
+    const unsigned grid = (unsigned)(cus * blocks_per_cu);
+    tok = prof_start("batch_mttkrp", s);
+    if (dtype == SPTOL_BF16)
+        hipLaunchKernelGGL(k_batch_mttkrp<uint16_t>, dim3(grid), dim3(256), 0, s, d_desc, d_tile_prefix, n, ntiles,
+                           d_counter);
+    else
+        hipLaunchKernelGGL(k_batch_mttkrp<float>, dim3(grid), dim3(256), 0, s, d_desc, d_tile_prefix, n, ntiles,
+                           d_counter);
+    prof_stop(tok, s);
+    SPTOL_HIP(hipGetLastError(), "Batch launch");
     return 0;
 }
 
 static int launch_batch(sptolBatch b, hipStream_t s)
 {
-    const uint32_t n = (uint32_t)b->descs.size();
-    const int cus = cu_count();
-    unsigned fill_grid = (unsigned)std::min<uint64_t>(std::max<uint64_t>(b->nchunks, 1), (uint64_t)cus * 8);
-    int tok = prof_start("batch_fill", s);
-    hipLaunchKernelGGL(k_batch_fill, dim3(fill_grid), dim3(256), 0, s, b->d_desc, b->d_fill_prefix, n, b->nchunks,
-                       b->d_counter);
-    prof_stop(tok, s);
-    const unsigned grid = (unsigned)(cus * b->cfg.blocks_per_cu);
-    tok = prof_start("batch_mttkrp", s);
-    if (b->dtype == SPTOL_BF16)
-        hipLaunchKernelGGL(k_batch_mttkrp<uint16_t>, dim3(grid), dim3(256), 0, s, b->d_desc, b->d_tile_prefix, n,
-                           b->ntiles, b->d_counter);
-    else
-        hipLaunchKernelGGL(k_batch_mttkrp<float>, dim3(grid), dim3(256), 0, s, b->d_desc, b->d_tile_prefix, n,
-                           b->ntiles, b->d_counter);
-    prof_stop(tok, s);
-    SPTOL_HIP(hipGetLastError(), "Batch launch");
-    return 0;
+    SPTOL_TRY(launch_legacy(b->d_desc, b->d_tile_prefix, b->d_fill_prefix, b->d_counter, b->nlegacy, b->ntiles,
+                            b->nchunks, b->dtype, b->cfg.blocks_per_cu, s));
+    return seg_batch_kernels(b->seg, b->dtype, b->cfg.blocks_per_cu, s);
 }
 
 void make_desc(sptolDevCoo X, const sptolDevMatrix *mats, const uint32_t *mats_order, uint32_t mode,
@@ -258,6 +488,7 @@ void make_desc(sptolDevCoo X, const sptolDevMatrix *mats, const uint32_t *mats_o
         d->order[m] = mats_order[m];
     }
     d->vals = sc.ready ? sc.vals : X->vals;
+    d->seg = (sc.ready && sc.seg.ready && seg_ok(X->nmodes, out->stride)) ? const_cast<SegView *>(&sc.seg) : nullptr;
     d->out = (float *)out->values;
     d->out_len = (uint64_t)X->ndims[mode] * out->stride;
     d->nmodes = X->nmodes;
@@ -275,11 +506,19 @@ BatchBuffers::~BatchBuffers()
 int batch_launch(std::vector<BatchDesc> &descs, int dtype, double tile_scale, int blocks_per_cu, BatchBuffers &bb,
                  hipStream_t s)
 {
-    const size_t n = descs.size();
+    if (descs.empty()) return 0;
+    std::vector<const BatchDesc *> segs;
+    std::vector<BatchDesc> leg;
+    split_descs(descs, segs, leg);
+    if (!segs.empty()) {
+        SPTOL_TRY(seg_batch_prepare(segs, tile_scale, bb.seg, s));
+        SPTOL_TRY(seg_batch_kernels(bb.seg, dtype, blocks_per_cu, s));
+    }
+    const size_t n = leg.size();
     if (n == 0) return 0;
     std::vector<uint64_t> tp(n + 1, 0), fp(n + 1, 0);
     for (size_t i = 0; i < n; ++i) {
-        BatchDesc &d = descs[i];
+        BatchDesc &d = leg[i];
         d.tile_nnz = tile_len(d.stride, tile_scale);
         tp[i + 1] = tp[i] + (d.nnz + d.tile_nnz - 1) / d.tile_nnz;
         fp[i + 1] = fp[i] + (d.out_len + 65535) / 65536;
@@ -296,24 +535,11 @@ int batch_launch(std::vector<BatchDesc> &descs, int dtype, double tile_scale, in
     if (!bb.counter) SPTOL_TRY(dmalloc((void **)&bb.counter, 4, "Batch"));
     // pageable sources: each copy is staged before the call returns, so the
     // host vectors may be reused immediately; the copies stay stream-ordered
-    SPTOL_HIP(hipMemcpyAsync(bb.desc, descs.data(), n * sizeof(BatchDesc), hipMemcpyHostToDevice, s), "Batch");
+    SPTOL_HIP(hipMemcpyAsync(bb.desc, leg.data(), n * sizeof(BatchDesc), hipMemcpyHostToDevice, s), "Batch");
     SPTOL_HIP(hipMemcpyAsync(bb.tile_prefix, tp.data(), (n + 1) * 8, hipMemcpyHostToDevice, s), "Batch");
     SPTOL_HIP(hipMemcpyAsync(bb.fill_prefix, fp.data(), (n + 1) * 8, hipMemcpyHostToDevice, s), "Batch");
-    const int cus = cu_count();
-    const unsigned fill_grid = (unsigned)std::min<uint64_t>(std::max<uint64_t>(fp[n], 1), (uint64_t)cus * 8);
-    hipLaunchKernelGGL(k_batch_fill, dim3(fill_grid), dim3(256), 0, s, bb.desc, bb.fill_prefix, (uint32_t)n, fp[n],
-                       bb.counter);
-    const unsigned grid = (unsigned)(cus * blocks_per_cu);
-    int tok = prof_start("batch_mttkrp", s);
-    if (dtype == SPTOL_BF16)
-        hipLaunchKernelGGL(k_batch_mttkrp<uint16_t>, dim3(grid), dim3(256), 0, s, bb.desc, bb.tile_prefix, (uint32_t)n,
-                           tp[n], bb.counter);
-    else
-        hipLaunchKernelGGL(k_batch_mttkrp<float>, dim3(grid), dim3(256), 0, s, bb.desc, bb.tile_prefix, (uint32_t)n,
-                           tp[n], bb.counter);
-    prof_stop(tok, s);
-    SPTOL_HIP(hipGetLastError(), "Batch launch");
-    return 0;
+    return launch_legacy(bb.desc, bb.tile_prefix, bb.fill_prefix, bb.counter, (uint32_t)n, tp[n], fp[n], dtype,
+                         blocks_per_cu, s);
 }
 
 static std::string signature(sptolBatch b)

@@ -45,6 +45,7 @@ int num_cus();
 // (mttkrp_seg.hip), for 3- and 4-mode tensors.
 struct SegView {
     uint32_t nmodes = 0, mode = 0, nrows = 0;
+    uint32_t ndims[4] = {};        // rows of every mode's factor
     uint64_t nnz = 0;
     uint4 *rec = nullptr;          // N=3: {i0, i1, i2, bits(v)}; N=4: {i0, i1, i2, i3}
     float *vals = nullptr;         // N=4: values (owned when own_vals)
@@ -53,15 +54,20 @@ struct SegView {
     float *carry = nullptr;        // per wave: partial row carried into an earlier wave's row
     uint32_t *carry_row = nullptr; // per wave: that row, or kNoRow
     uint64_t carry_waves = 0;      // capacity in waves (64 floats each)
+    uint32_t *counter = nullptr;   // work counter of the persistent kernel (0 between launches)
     bool ready = false;
     void release();
 };
 
 // Pack a mode-sorted SoA copy (inds[0..N-1], vals) into v (allocates).
 int seg_build(SegView &v, uint32_t nmodes, uint32_t mode, uint32_t nrows, uint64_t nnz,
-              uint32_t *const *inds, const float *vals, bool copy_vals, hipStream_t s);
+              uint32_t *const *inds, const float *vals, bool copy_vals, hipStream_t s, const uint32_t *ndims);
 // Whether the segmented kernel handles this shape.
 bool seg_ok(uint32_t nmodes, uint32_t stride);
+// Lanes per nonzero (one float4 of columns each) for a row stride.
+uint32_t seg_lpn(uint32_t stride);
+// Nonzeros per lane group for a tensor of nnz nonzeros and G groups per wave.
+uint32_t seg_chg(uint64_t nnz, uint32_t G);
 // out[0 .. nrows*stride) = MTTKRP of the view (no fill needed beforehand).
 int seg_mttkrp(SegView &v, const void *const *mats, const uint32_t *order, uint32_t R, uint32_t stride,
                int dtype, float *out, hipStream_t s);
@@ -150,10 +156,28 @@ struct BatchDesc {
     uint32_t order[4];
     uint32_t lpn;              // lanes per nonzero = stride / 4
     uint32_t pad;
+    SegView *seg;              // host-side: mode-sorted view for the segmented launch, or null
 };
 
 void make_desc(sptolDevCoo X, const sptolDevMatrix *mats, const uint32_t *mats_order, uint32_t mode,
                const sptolDevMatrix *out, BatchDesc *d);
+
+// Device side of the segmented part of a batch (batch.hip): per class of
+// (lanes per nonzero, modes) the items and their range prefix; all items in
+// class order with their carry-block prefix; per-batch carry space; one
+// work counter per class (zero between launches).
+struct SegBatchDev {
+    uint8_t *arena = nullptr;
+    size_t arena_cap = 0;
+    float *carry = nullptr;
+    uint32_t *carry_row = nullptr;
+    uint64_t carry_cap = 0;              // ranges
+    uint32_t *counters = nullptr;        // 8
+    size_t off_items[8] = {}, off_prefix[8] = {}, off_all = 0, off_cprefix = 0;
+    uint32_t n[8] = {}, n_all = 0;
+    uint64_t total[8] = {}, carry_blocks = 0;
+    ~SegBatchDev();
+};
 
 // Reusable device buffers of a per-step batched launch.
 struct BatchBuffers {
@@ -161,6 +185,7 @@ struct BatchBuffers {
     uint64_t *tile_prefix = nullptr, *fill_prefix = nullptr;
     uint32_t *counter = nullptr;
     size_t cap = 0;
+    SegBatchDev seg;
     ~BatchBuffers();
 };
 

@@ -464,7 +464,7 @@ int build_hicoo_seg(sptolDevHicoo X, uint32_t mode, HicooView &v, hipStream_t s)
     HS_TRY(dmalloc((void **)&svals, nnz * 4, "HiCOO view"));
     hipLaunchKernelGGL(hv_gather<float>, dim3(hv_grid(nnz)), dim3(256), 0, s, svals, X->vals, perm2, nnz);
     HS_HIP(hipGetLastError());
-    HS_TRY(seg_build(v.seg, N, mode, X->ndims[mode], nnz, sorted, svals, true, s));
+    HS_TRY(seg_build(v.seg, N, mode, X->ndims[mode], nnz, sorted, svals, true, s, X->ndims));
     cleanup();
     v.use_seg = true;
     v.ready = true;

@@ -490,7 +490,7 @@ int coo_prepare(sptolDevCoo h, uint32_t mode, hipStream_t s)
     hipError_t e2 = hipStreamSynchronize(s);
     cleanup();
     if (!r && e2 == hipSuccess && (h->nmodes == 3 || h->nmodes == 4))
-        r = seg_build(sc.seg, h->nmodes, mode, h->ndims[mode], n, sc.inds, sc.vals, false, s);
+        r = seg_build(sc.seg, h->nmodes, mode, h->ndims[mode], n, sc.inds, sc.vals, false, s, h->ndims);
     if (r) {
         for (uint32_t m = 0; m < h->nmodes; ++m) { dfree(sc.inds[m]); sc.inds[m] = nullptr; }
         dfree(sc.vals); sc.vals = nullptr;

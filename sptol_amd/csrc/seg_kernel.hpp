@@ -1,0 +1,333 @@
+// Device code of the segmented MTTKRP, shared by the single-tensor launch
+// (mttkrp_seg.hip) and the batched launch (batch.hip).
+//
+// Semantics are those of the reference (src/sptensor/mttkrp.c:39-177):
+//   out[i_mode][r] = sum_x v_x * prod_{k=1..N-1} A_{order[k]}[i_{order[k]}][r]
+// with every output row written, untouched rows as zeros and padding
+// columns as zeros.  Each product is formed in the reference's association
+// ((v*A_o1)*A_o2)*A_o3 and, with contraction off, a row whose nonzeros all
+// fall in one lane group's range is summed in the reference's order: the
+// sort is stable, so such rows match the sequential reference bit for bit.
+//
+// Input: one 16-byte record per nonzero ({i0,i1,i2,bits(v)} for 3 modes,
+// {i0,i1,i2,i3} plus a value array for 4), sorted by the output mode, and a
+// bitmap of the output rows that hold at least one nonzero.
+//
+// One "range" (seg_range) is the unit of work of a wave:
+//  * The wave owns G*chg consecutive nonzeros; each of its G lane groups
+//    (LPN = stride/4 lanes, 4 columns per lane) owns chg of them.
+//  * A group loads U records with one coalesced access (lane l loads record
+//    z+l), passes them through LDS and reads them back as broadcasts, then
+//    issues all U*(N-1) factor-row gathers before using any; the next
+//    step's records are loaded behind the gathers.
+//  * Products are summed in registers while the output row repeats.  A row
+//    that starts and ends inside the group is stored with one plain 16-byte
+//    store per lane.  The group's first row, when it began in the previous
+//    group, and its last row stay in registers.
+//  * The wave then merges its groups' boundary rows through LDS in a fixed
+//    order: the group holding a row's first nonzero stores the row.  A row
+//    that began in an earlier range is written as this range's carry.
+//  * Finally it zero-fills the untouched rows of its slice of the output
+//    (rows_per_wave rows), so every output row is written exactly once.
+// seg_carry_slots then adds each run of carries, summed in a fixed order, to
+// the row's stored value.  The result is deterministic.
+#pragma once
+
+#include "dev_common.hpp"
+
+namespace sptol {
+
+struct SegArgs {
+    const uint4 *rec;
+    const float *vals;       // N=4 values
+    uint64_t nnz;
+    const void *A1, *A2, *A3;
+    uint32_t o1, o2, o3, mode;
+    uint32_t R, S, nrows, chg;
+    uint32_t rows_per_wave, lpn;
+    uint64_t nwaves;         // ranges
+    float *out;
+    const uint32_t *touched;
+    float *carry;            // nwaves x kSegCarryStride floats
+    uint32_t *carry_row;     // nwaves
+    uint32_t nmodes, pad;
+};
+
+// SegArgs of one MTTKRP on view v (carry space left unset; mttkrp_seg.hip).
+int seg_fill_args(const SegView &v, const void *const *mats, const uint32_t *order, uint32_t R, uint32_t stride,
+                  float *out, uint32_t chg, SegArgs &a);
+
+constexpr uint32_t kSegCarryStride = 64;
+constexpr int kSegU = 4;                 // records per lane-group step
+constexpr uint32_t kSegCarrySlots = 64;  // ranges per carry block
+
+__device__ __forceinline__ uint32_t seg_comp(const uint4 r, uint32_t k)
+{
+    return k == 0 ? r.x : (k == 1 ? r.y : (k == 2 ? r.z : r.w));
+}
+
+__device__ __forceinline__ float4 seg_ld(const float *p) { return *reinterpret_cast<const float4 *>(p); }
+
+__device__ __forceinline__ float4 seg_ld(const uint16_t *p)
+{
+    const uint2 u = *reinterpret_cast<const uint2 *>(p);
+    return make_float4(__uint_as_float(u.x << 16), __uint_as_float(u.x & 0xFFFF0000u),
+                       __uint_as_float(u.y << 16), __uint_as_float(u.y & 0xFFFF0000u));
+}
+
+__device__ __forceinline__ void seg_st(float *p, const float4 v) { *reinterpret_cast<float4 *>(p) = v; }
+
+__device__ __forceinline__ void seg_add(float4 &a, const float4 b)
+{
+#pragma clang fp contract(off)
+    a.x += b.x; a.y += b.y; a.z += b.z; a.w += b.w;
+}
+
+// LDS hand-off between lanes of one wave: LDS operations of a wave complete
+// in order; the fences keep the compiler from moving accesses across.
+__device__ __forceinline__ void seg_wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Per-wave LDS of the range body, for WPB waves.
+template <int LPN, int NM, int WPB>
+struct SegShared {
+    static constexpr uint32_t G = 64 / LPN;
+    static constexpr uint32_t SL = kSegU + 1;   // staging slots per group (+1 spreads banks)
+    uint4 stage[WPB][G][SL];
+    float stagev[WPB][G][NM == 4 ? SL : 1];
+    float4 piece[WPB][64];
+    uint32_t info[WPB][G][2];
+};
+
+template <int LPN, int NM, typename FT, int WPB>
+__device__ __forceinline__ void seg_range(const SegArgs &a, uint64_t wave, SegShared<LPN, NM, WPB> &sh, uint32_t wv,
+                                          uint32_t lane)
+{
+#pragma clang fp contract(off)
+    constexpr int U = kSegU;
+    constexpr uint32_t KL = (U + LPN - 1) / LPN;   // records each lane stages per step
+    constexpr uint32_t G = 64 / LPN;
+    const uint32_t g = lane / LPN, l = lane % LPN;
+    const uint32_t S = a.S, R = a.R, c0 = l * 4u;
+    const bool col_ok = c0 < S;
+    const uint32_t cc = col_ok ? c0 : 0u;     // idle lanes gather column 0
+    const bool m0 = c0 < R, m1 = c0 + 1 < R, m2 = c0 + 2 < R, m3 = c0 + 3 < R;
+    const FT *__restrict__ A1 = (const FT *)a.A1;
+    const FT *__restrict__ A2 = (const FT *)a.A2;
+    const FT *__restrict__ A3 = (const FT *)a.A3;
+    const uint32_t mode = a.mode, o1 = a.o1, o2 = a.o2, o3 = a.o3;
+    const uint4 *__restrict__ rec = a.rec;
+    float *__restrict__ out = a.out;
+    const float4 zero = make_float4(0.f, 0.f, 0.f, 0.f);
+
+    const uint64_t zs = (wave * G + g) * (uint64_t)a.chg;
+    const uint64_t ze = zs < a.nnz ? min<uint64_t>(zs + a.chg, a.nnz) : zs;
+    const bool nonempty = zs < ze;
+    float4 acc = zero, facc = zero;
+    uint32_t cur = kNoRow, frow = kNoRow;
+    bool first_cont = false, in_first = true;
+    if (nonempty) {
+        frow = seg_comp(rec[zs], mode);
+        first_cont = zs > 0 && seg_comp(rec[zs - 1], mode) == frow;
+    }
+
+    // this lane's share of a step's records; slots past the range repeat
+    // the last nonzero (in bounds, unused)
+    uint4 nx[KL];
+    float nv[KL];
+    auto load_recs = [&](uint64_t zb) {
+#pragma unroll
+        for (uint32_t k = 0; k < KL; ++k) {
+            const uint32_t idx = l + k * LPN;
+            if (idx < U) {
+                const uint64_t zz = min<uint64_t>(zb + idx, ze - 1);
+                nx[k] = rec[zz];
+                if (NM == 4) nv[k] = a.vals[zz];
+            }
+        }
+    };
+    if (nonempty) load_recs(zs);
+
+    for (uint64_t z = zs; z < ze; z += U) {
+#pragma unroll
+        for (uint32_t k = 0; k < KL; ++k) {
+            const uint32_t idx = l + k * LPN;
+            if (idx < U) {
+                sh.stage[wv][g][idx] = nx[k];
+                if (NM == 4) sh.stagev[wv][g][idx] = nv[k];
+            }
+        }
+        seg_wave_sync();
+        uint4 rr[U];
+        float vv[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            rr[u] = sh.stage[wv][g][u];
+            vv[u] = NM == 4 ? sh.stagev[wv][g][u] : __uint_as_float(rr[u].w);
+        }
+        seg_wave_sync();
+        float4 p[U], q[U], t[NM == 4 ? U : 1];
+#pragma unroll
+        for (int u = 0; u < U; ++u) p[u] = seg_ld(A1 + (size_t)seg_comp(rr[u], o1) * S + cc);
+#pragma unroll
+        for (int u = 0; u < U; ++u) q[u] = seg_ld(A2 + (size_t)seg_comp(rr[u], o2) * S + cc);
+        if (NM == 4) {
+#pragma unroll
+            for (int u = 0; u < U; ++u) t[u] = seg_ld(A3 + (size_t)seg_comp(rr[u], o3) * S + cc);
+        }
+        if (z + U < ze) load_recs(z + U);
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            p[u].x = (vv[u] * p[u].x) * q[u].x;
+            p[u].y = (vv[u] * p[u].y) * q[u].y;
+            p[u].z = (vv[u] * p[u].z) * q[u].z;
+            p[u].w = (vv[u] * p[u].w) * q[u].w;
+            if (NM == 4) {
+                p[u].x *= t[u].x; p[u].y *= t[u].y; p[u].z *= t[u].z; p[u].w *= t[u].w;
+            }
+            // padding columns (>= R) hold unspecified factor values: keep them 0
+            p[u].x = m0 ? p[u].x : 0.f;
+            p[u].y = m1 ? p[u].y : 0.f;
+            p[u].z = m2 ? p[u].z : 0.f;
+            p[u].w = m3 ? p[u].w : 0.f;
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            if (z + u < ze) {
+                const uint32_t row = seg_comp(rr[u], mode);
+                if (row != cur) {
+                    if (cur != kNoRow) {
+                        if (in_first && first_cont) facc = acc;
+                        else if (col_ok) seg_st(out + (size_t)cur * S + c0, acc);
+                        in_first = false;
+                    }
+                    cur = row;
+                    acc = p[u];
+                } else {
+                    seg_add(acc, p[u]);
+                }
+            }
+        }
+    }
+
+    // Merge the groups' boundary rows.  flags: 1 nonempty, 2 first row
+    // continues from the previous nonzero, 4 single row.  A group's "piece"
+    // is what it adds to a row begun before it: its whole sum when single,
+    // else its first row's sum.
+    const bool single = in_first;
+    const uint32_t flags = (nonempty ? 1u : 0u) | (first_cont ? 2u : 0u) | (single ? 4u : 0u);
+    const float4 piece = single ? acc : facc;
+    sh.piece[wv][lane] = piece;
+    if (l == 0) {
+        sh.info[wv][g][0] = frow;
+        sh.info[wv][g][1] = flags;
+    }
+    seg_wave_sync();
+    // the chain of pieces that extends a row past group `from - 1`
+    auto chain = [&](float4 tot, uint32_t from) {
+        for (uint32_t k = from; k < G; ++k) {
+            const uint32_t f = sh.info[wv][k][1];
+            if ((f & 3u) != 3u) break;
+            seg_add(tot, sh.piece[wv][k * LPN + l]);
+            if (!(f & 4u)) break;
+        }
+        return tot;
+    };
+    if (nonempty && !(single && first_cont)) {
+        // this group holds the first nonzero of its last row
+        const float4 tot = chain(acc, g + 1);
+        if (col_ok) seg_st(out + (size_t)cur * S + c0, tot);
+    }
+    if (g == 0) {
+        uint32_t crow = kNoRow;
+        if (nonempty && first_cont) {
+            const float4 tot = single ? chain(piece, 1) : piece;
+            crow = frow;
+            if (col_ok) seg_st(a.carry + wave * kSegCarryStride + c0, tot);
+        }
+        if (l == 0) a.carry_row[wave] = crow;
+    }
+    seg_wave_sync();   // piece / info are rewritten by the next range
+
+    // zero the untouched rows of this range's slice
+    const uint64_t r0 = wave * (uint64_t)a.rows_per_wave;
+    const uint64_t r1 = min<uint64_t>(r0 + a.rows_per_wave, a.nrows);
+    for (uint64_t r = r0 + g; r < r1; r += G) {
+        const uint32_t w = a.touched[r >> 5];
+        if (!((w >> (r & 31u)) & 1u) && col_ok) seg_st(out + r * S + c0, zero);
+    }
+}
+
+// Carries of ranges [k0, k1) of one MTTKRP (k1 - k0 <= kSegCarrySlots), by a
+// 256-thread block.  A run is a maximal sequence of consecutive ranges
+// carrying the same row; the block whose window holds a run's first range
+// sums the run in a fixed order and adds it to the row.  Runs that end
+// inside the window are summed in parallel, one lane group each; the one
+// run per window that may continue past it (long runs of heavy rows) is
+// summed by the whole block (groups stride the run, then an ordered sum
+// over groups).  `rows` is block LDS of kSegCarrySlots + 2 entries.
+__device__ __forceinline__ void seg_carry_slots(const float *__restrict__ carry, const uint32_t *__restrict__ carry_row,
+                                                uint64_t nwaves, float *__restrict__ out, uint32_t S, uint32_t lpn,
+                                                uint64_t k0, uint64_t k1, uint32_t *rows, float4 *part)
+{
+#pragma clang fp contract(off)
+    const uint32_t t = threadIdx.x;
+    const uint32_t n = (uint32_t)(k1 - k0);
+    if (t <= n + 1) {
+        // rows[i] = carry_row[k0 + i - 1]: the range before the window and the one after it
+        const uint64_t k = k0 + t;   // = index + 1
+        rows[t] = (t == 0 && k0 == 0) || k - 1 >= nwaves ? kNoRow : carry_row[k - 1];
+    }
+    __syncthreads();
+    const uint32_t G = 256 / lpn, g = t / lpn, l = t % lpn, c0 = l * 4u;
+    // runs that start and end in the window
+    for (uint32_t i = g; i < n; i += G) {
+        const uint32_t r = rows[i + 1];
+        if (r == kNoRow || rows[i] == r) continue;
+        uint32_t j = i + 1;
+        while (j < n && rows[j + 1] == r) ++j;
+        if (j == n && rows[n + 1] == r) continue;   // continues past the window
+        if (c0 < S) {
+            float4 s = seg_ld(carry + (k0 + i) * kSegCarryStride + c0);
+            for (uint32_t q = i + 1; q < j; ++q) seg_add(s, seg_ld(carry + (k0 + q) * kSegCarryStride + c0));
+            float4 o = seg_ld(out + (size_t)r * S + c0);
+            seg_add(o, s);
+            seg_st(out + (size_t)r * S + c0, o);
+        }
+    }
+    // the run that continues past the window, if it starts here
+    if (n == 0 || rows[n + 1] == kNoRow || rows[n + 1] != rows[n]) return;   // block-uniform
+    const uint32_t r = rows[n];
+    uint32_t i = n - 1;
+    while (i > 0 && rows[i] == r) --i;
+    if (i == 0 && rows[0] == r) return;   // began before the window: its head block sums it
+    const uint64_t k = k0 + i;
+    uint64_t e = k1 + 1;                  // ranges k1 - 1 and k1 are in the run
+    for (uint64_t base = e;; base += 256) {
+        const uint64_t jj = base + t;
+        const bool in = jj < nwaves && carry_row[jj] == r;
+        const int cnt = __syncthreads_count(in);
+        e = base + (uint64_t)cnt;
+        if (cnt < 256) break;
+    }
+    float4 tot = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (c0 < S) {
+#pragma unroll 4
+        for (uint64_t j = k + g; j < e; j += G) seg_add(tot, seg_ld(carry + j * kSegCarryStride + c0));
+    }
+    part[t] = tot;
+    __syncthreads();
+    if (g == 0 && c0 < S) {
+        float4 s = part[l];
+        for (uint32_t gg = 1; gg < G; ++gg) seg_add(s, part[gg * lpn + l]);
+        float4 o = seg_ld(out + (size_t)r * S + c0);
+        seg_add(o, s);
+        seg_st(out + (size_t)r * S + c0, o);
+    }
+}
+
+}  // namespace sptol

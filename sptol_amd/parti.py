@@ -222,7 +222,7 @@ def load_library(path: str | None = None) -> C.CDLL:
     global _LIB
     if _LIB is not None and path is None:
         return _LIB
-    p = path or os.path.join(_here(), "libsptol_amd.so")
+    p = path or os.environ.get("SPTOL_LIBRARY") or os.path.join(_here(), "libsptol_amd.so")
     if not os.path.exists(p):
         raise RuntimeError(f"{p} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'` "
                            "or `make -C sptol_amd/csrc`")
@@ -253,6 +253,8 @@ def _declare(lib):
         "sptolHipDeviceCount": (C.c_int, [P(C.c_int)]),
         "sptolProfileEnable": (None, [C.c_int]),
         "sptolSetHicooViewKind": (C.c_int, [C.c_int]),
+        "sptolSparseTensorToHiCOODevice": (C.c_int, [P(sptSparseTensorHiCOO), P(sptNnzIndex), P(sptSparseTensor),
+                                                     C.c_uint8, C.c_uint8]),
         "sptolHipMalloc": (C.c_int, [P(C.c_void_p), C.c_size_t]),
         "sptolHipFree": (C.c_int, [C.c_void_p]),
         "sptolProfileReset": (None, []),
@@ -319,6 +321,16 @@ def check(lib, rc: int, what: str = "call"):
     if rc != 0:
         code, msg = last_error(lib)
         raise RuntimeError(f"{what} failed: rc=0x{rc:x} ({msg})")
+
+
+def to_hicoo_device(lib, coo: CooTensor, sb_bits: int, sk_bits: int):
+    """Convert on the GPU through lib.sptolSparseTensorToHiCOODevice (same
+    arrays as to_hicoo; permutes coo in place)."""
+    h = sptSparseTensorHiCOO()
+    mx = sptNnzIndex(0)
+    rc = lib.sptolSparseTensorToHiCOODevice(C.byref(h), C.byref(mx), coo.ref(), sb_bits, sk_bits)
+    check(lib, rc, "sptolSparseTensorToHiCOODevice")
+    return h, int(mx.value)
 
 
 def to_hicoo(lib, coo: CooTensor, sb_bits: int, sk_bits: int, tk: int = 0):

@@ -1,0 +1,98 @@
+"""GPU COO -> HiCOO conversion (sptolSparseTensorToHiCOODevice).
+
+Bar: bit-exact.  Every array (kptr, cptr, bptr, binds, einds, values,
+kschr, nkiters, max_nnzb) and the caller's permuted COO tensor equal the
+reference's golden conversions (tests/golden/hicoo.npz, written by the
+compiled reference) and the host converter, itself pinned to those
+goldens, on 2-D to 5-D tensors, Morton keys wider than 64 bits, sk_bits
+above the 3-D key's 24 bits, and dense blocks that open extra chunks."""
+import ctypes as C
+import sys
+
+import numpy as np
+import pytest
+
+from sptol_amd import gen
+from sptol_amd.parti import CooTensor, hicoo_to_numpy, to_hicoo, to_hicoo_device
+from test_host import _check_against_golden
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _same(a, b, coo_a, coo_b, N):
+    for f in ("kptr", "cptr", "bptr", "nkiters"):
+        assert np.array_equal(a[f], b[f]), f
+    assert np.array_equal(a["values"].view(np.uint32), b["values"].view(np.uint32))
+    for m in range(N):
+        assert np.array_equal(a["binds"][m], b["binds"][m]), m
+        assert np.array_equal(a["einds"][m], b["einds"][m]), m
+        assert len(a["kschr"][m]) == len(b["kschr"][m])
+        for x, y in zip(a["kschr"][m], b["kschr"][m]):
+            assert np.array_equal(x, y)
+        assert np.array_equal(coo_a.inds[m], coo_b.inds[m]), m
+    assert np.array_equal(coo_a.vals.view(np.uint32), coo_b.vals.view(np.uint32))
+
+
+def test_device_converter_matches_reference_golden(gpu_lib, golden_hicoo):
+    g = golden_hicoo
+    for key in g.cases():
+        _say("golden", key)
+        d, inds, vals = g.tensor(key, prefix="in_ind", vals="in_vals")
+        sb, sk = (int(x) for x in g.get(key, "sb_sk"))
+        coo = CooTensor(d, [a.copy() for a in inds], vals.copy())
+        h, mx = to_hicoo_device(gpu_lib, coo, sb, sk)
+        _check_against_golden(hicoo_to_numpy(h), coo, g, key, len(d))
+        assert mx == int(g.get(key, "max_nnzb"))
+        gpu_lib.sptFreeSparseTensorHiCOO(C.byref(h))
+
+
+@pytest.mark.parametrize("dims,nnz,sb,sk,s", [
+    ([12000, 9000, 28000], 200000, 7, 14, 1.1),      # the headline layout, Zipf
+    ([5000, 5000, 5000, 500], 150000, 4, 8, 1.05),   # 4-D
+    ([3000, 2000, 2500, 400], 100000, 7, 31, 1.05),  # 4-D, 128-bit Morton key
+    ([1 << 25, 20000000, 5000], 80000, 5, 28, 1.2),   # 3-D, sk above the 24-bit key
+    ([70000, 3, 900000], 60000, 2, 9, 1.2),
+    ([40000, 30000], 50000, 3, 6, 1.1),              # 2-D: no Morton order
+    ([300, 200, 100, 50, 20], 50000, 2, 4, 1.1),     # 5-D: no Morton order
+])
+def test_device_converter_matches_host(gpu_lib, dims, nnz, sb, sk, s):
+    d, inds, vals = gen.zipf_tensor(dims, nnz, s, seed=sb * 13 + sk)
+    _say("host vs device", dims, sb, sk)
+    ca = CooTensor(d, [a.copy() for a in inds], vals.copy())
+    cb = CooTensor(d, [a.copy() for a in inds], vals.copy())
+    ha, ma = to_hicoo(gpu_lib, ca, sb, sk)
+    hb, mb = to_hicoo_device(gpu_lib, cb, sb, sk)
+    _same(hicoo_to_numpy(ha), hicoo_to_numpy(hb), ca, cb, len(d))
+    assert ma == mb
+    gpu_lib.sptFreeSparseTensorHiCOO(C.byref(ha))
+    gpu_lib.sptFreeSparseTensorHiCOO(C.byref(hb))
+
+
+def test_device_converter_dense_blocks_open_chunks(gpu_lib):
+    """Blocks of more than 2^14 nonzeros: chunk pointers open at block starts
+    and, with the previous block's count carried, twice at a superblock's
+    first block (convert.c:386-456)."""
+    d, inds, vals = gen.uniform_tensor([512, 512, 64], 3_000_000, seed=7)
+    ca = CooTensor(d, [a.copy() for a in inds], vals.copy())
+    cb = CooTensor(d, [a.copy() for a in inds], vals.copy())
+    ha, _ = to_hicoo(gpu_lib, ca, 7, 8)
+    hb, _ = to_hicoo_device(gpu_lib, cb, 7, 8)
+    a, b = hicoo_to_numpy(ha), hicoo_to_numpy(hb)
+    assert len(a["cptr"]) > len(a["kptr"])        # extra chunks were opened
+    _same(a, b, ca, cb, 3)
+    gpu_lib.sptFreeSparseTensorHiCOO(C.byref(ha))
+    gpu_lib.sptFreeSparseTensorHiCOO(C.byref(hb))
+
+
+def test_device_converter_rejects_bad_input(gpu_lib):
+    d = np.array([10, 10, 10], np.uint32)
+    inds = [np.array([1, 2], np.uint32), np.array([3, 12], np.uint32), np.array([0, 0], np.uint32)]
+    with pytest.raises(RuntimeError):
+        to_hicoo_device(gpu_lib, CooTensor(d, inds, np.array([1, 2], np.float32)), 1, 2)   # 12 >= ndims[1]
+    with pytest.raises(RuntimeError):
+        to_hicoo_device(gpu_lib, CooTensor(d, [a % 10 for a in inds], np.array([1, 2], np.float32)), 3, 2)

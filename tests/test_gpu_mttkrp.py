@@ -242,3 +242,25 @@ def test_segmented_kernel_deterministic_and_rowwise_exact(gpu_lib):
         _say("  rows bit-exact vs sequential reference:", frac)
         assert frac > 0.95, frac
         _check(a, ref32, oracle.mttkrp_coo(inds, vals, A, order, 0, dtype=np.float64), 32)
+
+
+@pytest.mark.parametrize("R", [16, 32])
+def test_segmented_hot_rows_short_modes(gpu_lib, R):
+    """Segmented kernel with hot factor rows in LDS (>= 65536 nonzeros) on
+    4-D and 3-D Zipf tensors with modes shorter than the 256 hot slots, every
+    output mode, FP32 and BF16 factors."""
+    for dims in ([500, 400, 300, 60], [3000, 40, 2000]):
+        d, inds, vals = gen.zipf_tensor(dims, 150000, 1.05, seed=R + len(dims))
+        A = gen.uniform_factors(d, R, seed=4)
+        N = len(d)
+        h = dev_coo(gpu_lib, d, inds, vals)
+        for mode in range(N):
+            order = [mode] + [(mode + i) % N for i in range(1, N)]
+            for dtype in (0, SPTOL_BF16):
+                _say("seg hot", dims, mode, dtype)
+                got = dev_mttkrp_coo(gpu_lib, d, inds, vals, A, order, mode, prepare=True, dtype=dtype, h=h)
+                Au = [bf16_round(a) for a in A] if dtype else A
+                ref32 = oracle.mttkrp_coo(inds, vals, Au, order, mode)
+                ref64 = oracle.mttkrp_coo(inds, vals, Au, order, mode, dtype=np.float64)
+                _check(got, ref32, ref64, R)
+        gpu_lib.sptolDevCooFree(h)

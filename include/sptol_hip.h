@@ -41,6 +41,11 @@ int    sptolHipStreamSynchronize(void *stream);
  * SPTERR_CUDA_ERROR + hipErrorOutOfMemory before reaching the runtime. */
 int    sptolHipSetMemoryBudget(size_t bytes);               /* 0 = unlimited */
 size_t sptolHipMemoryInUse(void);
+/* Device blocks held for reuse by the library's block cache (not in use), and
+ * a call that returns them all to HIP (it waits for the device to drain).
+ * SPTOL_CACHE_BYTES caps the cache (default 64 GiB; 0 disables it). */
+size_t sptolHipMemoryCached(void);
+int    sptolHipReleaseCache(void);
 /* Fault injection: the n-th sptolHipMalloc from now fails (n >= 1); 0 disables. */
 int    sptolHipInjectMallocFailure(int n);
 /* Seed of the library's generator (sptRandomizeMatrix, sptCpdAls init). */
@@ -168,9 +173,12 @@ int  sptolHipStreamDestroy(void *stream);
 /* ---------------------------------------------------------------------- */
 typedef struct {
     int      factor_dtype;   /* in: SPTOL_F32 or SPTOL_BF16 storage for MTTKRP gathers       */
-    int      exact_fit;      /* in: 1 = <X,K> by an FP64 pass over the nonzeros; 0 = from the
-                                last mode's FP32 MTTKRP, as the reference (ktensor.c:173-257),
-                                which loses digits to cancellation when the fit nears 1     */
+    int      exact_fit;      /* in: 1 = the returned fit takes <X,K> from an FP64 pass over the
+                                nonzeros, once, after the last iteration (3- and 4-mode
+                                tensors); the per-iteration fit that drives tol always comes
+                                from the last mode's FP32 MTTKRP, as the reference
+                                (ktensor.c:173-257), which loses digits to cancellation when
+                                the fit nears 1                                             */
     int      iterations;     /* out: iterations executed                                    */
     int      reserved;
     double   fit;            /* out: last fit (FP64, correct ||K||^2)                       */
@@ -196,6 +204,10 @@ typedef struct {
     int    exact_fit;       /* 1 = FP64 pass for <X,K> (see sptolCpdStats.exact_fit)         */
     int    blocks_per_cu;   /* persistent-kernel residency (0 = default 8)                  */
     double tile_scale;      /* batched tile length multiplier (0 = default 1.0)             */
+    int    admit_threads;   /* parallel admission workers, each with its own stream (0 = 4) */
+    int    borrow_inputs;   /* 1 = Submit keeps pointers to the caller's index and value
+                               arrays, which must stay valid and unchanged until the
+                               request has been waited for; 0 = Submit copies them     */
 } sptolServerConfig;
 
 #define SPTOL_REQ_QUEUED    0
@@ -216,8 +228,8 @@ typedef struct {
 
 /* Start the admission and compute threads on the current device. */
 int  sptolServerCreate(sptolServer *s, const sptolServerConfig *cfg);
-/* Queue a CPD-ALS request (copies the tensor; the caller may free it on
- * return).  Factors start U[0,1) from `seed`; the solve stops after niters
+/* Queue a CPD-ALS request (copies the tensor, so the caller may free it on
+ * return, unless the server borrows its inputs).  Factors start U[0,1) from `seed`; the solve stops after niters
  * iterations or when the fit changes by less than tol (tol <= 0: never). */
 int  sptolServerSubmit(sptolServer s, const sptSparseTensor *X, sptIndex rank, sptIndex niters, double tol,
                        uint64_t seed, int factor_dtype, uint64_t *request_id);

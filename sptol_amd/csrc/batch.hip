@@ -208,6 +208,11 @@ __global__ __launch_bounds__(256) void k_batch_seg_carry(const SegArgs *__restri
 SegBatchDev::~SegBatchDev()
 {
     dfree(arena); dfree(carry); dfree(carry_row); dfree(counters);
+    for (int k = 0; k < 2; ++k) {
+        if (stage_used[k]) (void)hipEventSynchronize(stage_ev[k]);
+        if (stage[k]) (void)hipHostFree(stage[k]);
+        if (stage_ev[k]) (void)hipEventDestroy(stage_ev[k]);
+    }
 }
 
 static int seg_class(uint32_t lpn, uint32_t nmodes)
@@ -225,8 +230,8 @@ static uint32_t seg_tile_chg(double scale)
     return ((uint32_t)c + 7) / 8 * 8;
 }
 
-// Plan and upload the segmented items (stream-ordered copies from a host
-// staging buffer that the runtime stages before returning).
+// Plan and upload the segmented items (a stream-ordered copy from pinned
+// staging).
 static int seg_batch_prepare(const std::vector<const BatchDesc *> &items, double scale, SegBatchDev &dev, hipStream_t s)
 {
     std::vector<SegArgs> cls[8];
@@ -291,7 +296,23 @@ static int seg_batch_prepare(const std::vector<const BatchDesc *> &items, double
         SPTOL_TRY(dmalloc((void **)&dev.arena, cap, "Batch"));
         dev.arena_cap = cap;
     }
-    SPTOL_HIP(hipMemcpyAsync(dev.arena, host.data(), host.size(), hipMemcpyHostToDevice, s), "Batch");
+    const int k = dev.stage_cur;
+    dev.stage_cur ^= 1;
+    if (dev.stage_used[k]) SPTOL_HIP(hipEventSynchronize(dev.stage_ev[k]), "Batch");
+    dev.stage_used[k] = false;
+    if (dev.stage_cap[k] < host.size()) {
+        if (dev.stage[k]) (void)hipHostFree(dev.stage[k]);
+        dev.stage[k] = nullptr;
+        dev.stage_cap[k] = 0;
+        const size_t cap = host.size() * 2;
+        SPTOL_HIP(hipHostMalloc((void **)&dev.stage[k], cap, hipHostMallocDefault), "Batch");
+        dev.stage_cap[k] = cap;
+    }
+    if (!dev.stage_ev[k]) SPTOL_HIP(hipEventCreateWithFlags(&dev.stage_ev[k], hipEventDisableTiming), "Batch");
+    std::memcpy(dev.stage[k], host.data(), host.size());
+    SPTOL_HIP(hipMemcpyAsync(dev.arena, dev.stage[k], host.size(), hipMemcpyHostToDevice, s), "Batch");
+    SPTOL_HIP(hipEventRecord(dev.stage_ev[k], s), "Batch");
+    dev.stage_used[k] = true;
     return 0;
 }
 

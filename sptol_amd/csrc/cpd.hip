@@ -17,6 +17,7 @@
 //   * Reductions are two-stage with a fixed block count, so the solve, norm
 //     and fit steps are bitwise reproducible for a given MTTKRP result.
 #include "cpd_job.hpp"
+#include <algorithm>
 #include <vector>
 #include <cmath>
 #include <cstring>
@@ -68,16 +69,37 @@ __global__ __launch_bounds__(256) void k_gram_partial(const float *__restrict__ 
     }
 }
 
-// out[e] = sum_blk partial[blk][e], summed in block order.
-__global__ void k_sum_partials(const double *__restrict__ partial, uint32_t nblk, uint32_t n, double *__restrict__ out)
+// Wave-wide reduction of one value per lane in a fixed pattern.
+__device__ __forceinline__ double wave_sum(double v)
 {
-    const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= n) return;
-    double s = 0.0;
-    for (uint32_t b = 0; b < nblk; ++b) s += partial[(size_t)b * n + e];
-    out[e] = s;
+#pragma clang fp contract(off)
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
 }
 
+__device__ __forceinline__ double wave_max(double v)
+{
+    for (int o = 32; o > 0; o >>= 1) {
+        const double y = __shfl_xor(v, o);
+        v = y > v ? y : v;
+    }
+    return v;
+}
+
+// out[e] = sum over blk of partial[blk][e]: one wave per output, lanes
+// stride the blocks, then a fixed butterfly.  Four outputs per block.
+__global__ __launch_bounds__(256) void k_sum_partials(const double *__restrict__ partial, uint32_t nblk, uint32_t n,
+                                                      double *__restrict__ out)
+{
+    const uint32_t e = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63u;
+    if (e >= n) return;   // whole wave
+    double s = 0.0;
+    for (uint32_t b = lane; b < nblk; b += 64) s += partial[(size_t)b * n + e];
+    s = wave_sum(s);
+    if (lane == 0) out[e] = s;
+}
+
+static unsigned sum_grid(uint32_t n) { return (n + 3) / 4; }
 // Column reduction partials: kind 0 = sum of squares, 1 = max (from 0).
 __global__ __launch_bounds__(256) void k_col_partial(const float *__restrict__ A, uint32_t I, uint32_t R, uint32_t S,
                                                      int kind, double *__restrict__ partial)
@@ -113,21 +135,25 @@ __global__ __launch_bounds__(256) void k_col_partial(const float *__restrict__ A
 
 // lambda[c] from the partials: kind 0 -> sqrt(sum), 1 -> max(max, 1).
 // mult != 0 multiplies the existing lambda by the new norm instead
-// (GetFinalLambda); the norms are also written to norms[].
-__global__ void k_col_final(const double *__restrict__ partial, uint32_t nblk, uint32_t R, int kind, int mult,
-                            double *__restrict__ lambda, double *__restrict__ norms)
+// (GetFinalLambda); the norms are also written to norms[].  One wave per
+// column, four columns per block.
+__global__ __launch_bounds__(256) void k_col_final(const double *__restrict__ partial, uint32_t nblk, uint32_t R, int kind,
+                                                   int mult, double *__restrict__ lambda, double *__restrict__ norms)
 {
-    const uint32_t c = threadIdx.x;
-    if (c >= R) return;
+    const uint32_t c = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63u;
+    if (c >= R) return;   // whole wave
     double s = 0.0;
-    for (uint32_t b = 0; b < nblk; ++b) {
+    for (uint32_t b = lane; b < nblk; b += 64) {
         const double y = partial[(size_t)b * R + c];
         if (kind == 0) s += y;
         else s = y > s ? y : s;
     }
-    double nrm = kind == 0 ? sqrt(s) : (s < 1.0 ? 1.0 : s);
-    norms[c] = nrm;
-    lambda[c] = mult ? lambda[c] * nrm : nrm;
+    s = kind == 0 ? wave_sum(s) : wave_max(s);
+    if (lane == 0) {
+        const double nrm = kind == 0 ? sqrt(s) : (s < 1.0 ? 1.0 : s);
+        norms[c] = nrm;
+        lambda[c] = mult ? lambda[c] * nrm : nrm;
+    }
 }
 
 // A[i][c] /= norms[c] (FP64 divide, FP32 store); optional BF16 copy.
@@ -156,75 +182,106 @@ __global__ void k_scale(float *__restrict__ A, uint16_t *__restrict__ Abf, uint3
 // factorisation V = L L^T.  If V is not positive definite, an LU
 // factorisation with partial pivoting of V replaces it (the gesv fallback
 // of matrix_solver.c:66-79).  Output: fac (R*R), piv (R), *kind = 0 (L) / 1 (LU).
-__global__ __launch_bounds__(256) void k_factor(const double *__restrict__ ata, uint32_t N, uint32_t m, uint32_t R,
-                                                double *__restrict__ fac, int *__restrict__ piv, int *__restrict__ kind)
+//
+// One wave (64 threads, R <= RM <= 64).  The Cholesky is right-looking with
+// lane i holding row i in registers: at step j lane j publishes its updated
+// diagonal, every lane takes its square root, lanes i > j scale their
+// column-j entry and publish it, then each lane subtracts l_ij * l_kj from
+// its row for k in (j, i].  Every entry sees the same operations in the same
+// order as the textbook left-looking loops, and the wave needs no barriers
+// (LDS operations of a wave complete in order).
+__device__ __forceinline__ void chol_wave_sync()
 {
-    __shared__ double V[kMaxRank * kMaxRank];
-    __shared__ double W[kMaxRank * kMaxRank];
-    __shared__ int spd;
-    const uint32_t RR = R * R;
-    for (uint32_t e = threadIdx.x; e < RR; e += blockDim.x) {
-        double v = 1.0;
-        for (uint32_t k = 1; k < N; ++k) v *= ata[(size_t)((m + k) % N) * RR + e];
-        V[e] = v;
-        W[e] = v;
-    }
-    if (threadIdx.x == 0) spd = 1;
-    __syncthreads();
-    // Cholesky, column by column (lower triangle of W)
-    for (uint32_t j = 0; j < R; ++j) {
-        if (threadIdx.x == 0) {
-            double d = W[j * R + j];
-            for (uint32_t k = 0; k < j; ++k) d -= W[j * R + k] * W[j * R + k];
-            if (!(d > 0.0)) spd = 0;
-            W[j * R + j] = d > 0.0 ? sqrt(d) : 1.0;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <int RM>
+__global__ __launch_bounds__(64) void k_factor(const double *__restrict__ ata, uint32_t N, uint32_t m, uint32_t R,
+                                               double *__restrict__ fac, int *__restrict__ piv, int *__restrict__ kind)
+{
+#pragma clang fp contract(off)
+    __shared__ double col[RM];
+    __shared__ double V[RM * (RM + 1)];   // LU fallback, rows padded against bank conflicts
+    const uint32_t RR = R * R, t = threadIdx.x;
+    const bool mine = t < R;
+    double row[RM];
+#pragma unroll
+    for (int k = 0; k < RM; ++k) {
+        double v = 0.0;
+        if (mine && k < (int)R) {
+            v = 1.0;
+            for (uint32_t q = 1; q < N; ++q) v *= ata[(size_t)((m + q) % N) * RR + t * R + k];
         }
-        __syncthreads();
-        if (!spd) break;
-        for (uint32_t i = j + 1 + threadIdx.x; i < R; i += blockDim.x) {
-            double s = W[i * R + j];
-            for (uint32_t k = 0; k < j; ++k) s -= W[i * R + k] * W[j * R + k];
-            W[i * R + j] = s / W[j * R + j];
-        }
-        __syncthreads();
+        row[k] = v;
     }
-    __syncthreads();
+    bool spd = true;                                    // wave-uniform
+#pragma unroll
+    for (int j = 0; j < RM; ++j) {
+        if (j < (int)R && spd) {
+            if (t == (uint32_t)j) col[j] = row[j];      // a_jj - sum_{k<j} l_jk^2
+            chol_wave_sync();
+            const double d = col[j];
+            spd = d > 0.0;
+            if (spd) {
+                const double dj = sqrt(d);
+                if (t == (uint32_t)j) row[j] = dj;
+                if (mine && t > (uint32_t)j) {
+                    row[j] = row[j] / dj;
+                    col[t] = row[j];                    // l_tj
+                }
+                chol_wave_sync();
+                if (mine && t > (uint32_t)j) {
+                    const double ltj = row[j];
+#pragma unroll
+                    for (int k = j + 1; k < RM; ++k)
+                        if (k <= (int)t) row[k] = row[k] - ltj * col[k];
+                }
+                chol_wave_sync();                        // col is rewritten next step
+            }
+        }
+    }
     if (spd) {
-        for (uint32_t e = threadIdx.x; e < RR; e += blockDim.x) {
-            const uint32_t i = e / R, k = e % R;
-            fac[e] = k <= i ? W[e] : 0.0;
+        if (mine) {
+#pragma unroll
+            for (int k = 0; k < RM; ++k)
+                if (k < (int)R) fac[t * R + k] = k <= (int)t ? row[k] : 0.0;
         }
-        if (threadIdx.x == 0) *kind = 0;
+        if (t == 0) *kind = 0;
         return;
     }
-    // LU with partial pivoting on V (in place): at step j swap row j with the
-    // pivot row, then each thread eliminates one row below j.
-    __shared__ uint32_t p;
-    for (uint32_t j = 0; j < R; ++j) {
-        if (threadIdx.x == 0) {
-            uint32_t best = j;
-            for (uint32_t i = j + 1; i < R; ++i)
-                if (fabs(V[i * R + j]) > fabs(V[best * R + j])) best = i;
-            p = best;
-            piv[j] = (int)best;
-        }
-        __syncthreads();
-        if (p != j)
-            for (uint32_t k = threadIdx.x; k < R; k += blockDim.x) {
-                const double t = V[j * R + k];
-                V[j * R + k] = V[p * R + k];
-                V[p * R + k] = t;
-            }
-        __syncthreads();
-        for (uint32_t i = j + 1 + threadIdx.x; i < R; i += blockDim.x) {
-            const double l = V[i * R + j] / V[j * R + j];
-            V[i * R + j] = l;
-            for (uint32_t k = j + 1; k < R; ++k) V[i * R + k] -= l * V[j * R + k];
-        }
-        __syncthreads();
+    // LU with partial pivoting of V (recomputed; in place): at step j swap row
+    // j with the pivot row, then the lanes eliminate the rows below j.
+    const uint32_t LD = R + 1;
+    for (uint32_t e = t; e < RR; e += 64) {
+        const uint32_t i = e / R, k = e % R;
+        double v = 1.0;
+        for (uint32_t q = 1; q < N; ++q) v *= ata[(size_t)((m + q) % N) * RR + e];
+        V[i * LD + k] = v;
     }
-    for (uint32_t e = threadIdx.x; e < RR; e += blockDim.x) fac[e] = V[e];
-    if (threadIdx.x == 0) *kind = 1;
+    chol_wave_sync();
+    for (uint32_t j = 0; j < R; ++j) {
+        uint32_t best = j;
+        for (uint32_t i = j + 1; i < R; ++i)
+            if (fabs(V[i * LD + j]) > fabs(V[best * LD + j])) best = i;
+        if (t == 0) piv[j] = (int)best;
+        if (best != j)
+            for (uint32_t k = t; k < R; k += 64) {
+                const double tmp = V[j * LD + k];
+                V[j * LD + k] = V[best * LD + k];
+                V[best * LD + k] = tmp;
+            }
+        chol_wave_sync();
+        for (uint32_t i = j + 1 + t; i < R; i += 64) {
+            const double l = V[i * LD + j] / V[j * LD + j];
+            V[i * LD + j] = l;
+            for (uint32_t k = j + 1; k < R; ++k) V[i * LD + k] -= l * V[j * LD + k];
+        }
+        chol_wave_sync();
+    }
+    for (uint32_t e = t; e < RR; e += 64) fac[e] = V[(e / R) * LD + e % R];
+    if (t == 0) *kind = 1;
 }
 
 // A[i] = M[i] V^{-1} for every row, one thread per row, FP64 arithmetic.
@@ -331,34 +388,65 @@ __global__ __launch_bounds__(256) void k_inner_partial(const float *__restrict__
 // <X, K> = sum_nnz x * sum_r lambda_r prod_m A_m[i_m][r], every product and
 // sum in FP64 from the FP32 factors: no cancellation-prone reuse of an FP32
 // MTTKRP.  One thread per nonzero; per-block partial sums (fixed order).
-struct InnerArgs {
-    const uint32_t *inds[kMaxModes];
+struct InnerArgs {   // 3- and 4-mode tensors
+    const uint32_t *inds[4];
     const float *vals;
-    const float *mats[kMaxModes];
+    const float *mats[4];
     uint64_t nnz;
     uint32_t N, R, S;
 };
 
+// Lane groups of LPN lanes take nonzeros kInnerU at a time (interleaved
+// over the block's contiguous range, all gathers of a step issued before
+// use); lane l holds columns 4l..4l+3 and gathers them with one 16-byte load
+// per factor row.  Per lane: sum over its nonzeros of
+// x * sum_{its columns} lambda_c * prod_m A_m[i_m][c], all in FP64; then a
+// fixed-order block reduction into partial[block].
+constexpr int kInnerU = 4;
+constexpr uint32_t kInnerBlocks = 4096;
+
+template <int LPN>
 __global__ __launch_bounds__(256) void k_inner_exact_partial(const InnerArgs a, const double *__restrict__ lambda,
                                                              double *__restrict__ partial)
 {
+#pragma clang fp contract(off)
+    constexpr uint32_t G = 256 / LPN;
+    constexpr int U = kInnerU;
     __shared__ double red[256];
-    __shared__ double lam[kMaxRank];
-    if (threadIdx.x < a.R) lam[threadIdx.x] = lambda[threadIdx.x];
-    __syncthreads();
+    const uint32_t g = threadIdx.x / LPN, l = threadIdx.x % LPN, c0 = 4 * l;
+    double lam[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) lam[q] = c0 + q < a.R ? lambda[c0 + q] : 0.0;
+    const uint32_t cc = c0 < a.S ? c0 : 0;
     const uint64_t per = (a.nnz + gridDim.x - 1) / gridDim.x;
     const uint64_t z0 = (uint64_t)blockIdx.x * per, z1 = min<uint64_t>(z0 + per, a.nnz);
     double s = 0.0;
-    for (uint64_t z = z0 + threadIdx.x; z < z1; z += 256) {
-        const float *row[kMaxModes];
-        for (uint32_t m = 0; m < a.N; ++m) row[m] = a.mats[m] + (size_t)a.inds[m][z] * a.S;
-        double k = 0.0;
-        for (uint32_t r = 0; r < a.R; ++r) {
-            double p = lam[r];
-            for (uint32_t m = 0; m < a.N; ++m) p *= (double)row[m][r];
-            k += p;
+    for (uint64_t zb = z0 + (uint64_t)g * U; zb < z1; zb += (uint64_t)G * U) {
+        float4 f[U][4];
+        float v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const uint64_t z = zb + u < z1 ? zb + u : zb;   // zb < z1: in bounds
+            v[u] = zb + u < z1 ? a.vals[z] : 0.0f;
+#pragma unroll
+            for (int m = 0; m < 4; ++m)
+                if (m < (int)a.N) f[u][m] = *reinterpret_cast<const float4 *>(a.mats[m] + (size_t)a.inds[m][z] * a.S + cc);
         }
-        s += (double)a.vals[z] * k;
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            double p[4] = {lam[0], lam[1], lam[2], lam[3]};
+#pragma unroll
+            for (int m = 0; m < 4; ++m) {
+                if (m < (int)a.N) {
+                    p[0] *= (double)f[u][m].x; p[1] *= (double)f[u][m].y;
+                    p[2] *= (double)f[u][m].z; p[3] *= (double)f[u][m].w;
+                }
+            }
+            // padding columns (>= R) may hold unspecified values: drop them
+#pragma unroll
+            for (int q = 0; q < 4; ++q) p[q] = c0 + q < a.R ? p[q] : 0.0;
+            s += (double)v[u] * (((p[0] + p[1]) + p[2]) + p[3]);
+        }
     }
     red[threadIdx.x] = s;
     __syncthreads();
@@ -408,18 +496,27 @@ __global__ __launch_bounds__(256) void k_fit(const double *__restrict__ ata, uin
         if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
         __syncthreads();
     }
-    if (threadIdx.x == 0) {
-        const double normk = fabs(red[0]);
-        double inner = 0.0;
-        if (inner_scalar) {
-            for (uint32_t b = 0; b < nblk; ++b) inner += inner_partial[b];
-        } else {
-            for (uint32_t r = 0; r < R; ++r) {
-                double acc = 0.0;
-                for (uint32_t b = 0; b < nblk; ++b) acc += inner_partial[(size_t)b * R + r];
-                inner += acc * lambda[r];
-            }
+    const double normk = fabs(red[0]);
+    __syncthreads();
+    // inner product: per column r (or the scalar), its partials in block order
+    __shared__ double col[kMaxRank];
+    if (inner_scalar) {
+        if (threadIdx.x == 0) {
+            double acc = 0.0;
+            for (uint32_t b = 0; b < nblk; ++b) acc += inner_partial[b];
+            col[0] = acc;
         }
+    } else if (threadIdx.x < R) {
+        double acc = 0.0;
+        for (uint32_t b = 0; b < nblk; ++b) acc += inner_partial[(size_t)b * R + threadIdx.x];
+        col[threadIdx.x] = acc * lambda[threadIdx.x];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double inner = 0.0;
+        if (inner_scalar) inner = col[0];
+        else
+            for (uint32_t r = 0; r < R; ++r) inner += col[r];
         double residual = *normx + normk - 2.0 * inner;
         if (residual > 0.0) residual = sqrt(residual);
         *fit = 1.0 - residual / sqrt(*normx);
@@ -430,6 +527,16 @@ __global__ void k_set(double *p, uint32_t n, double v)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
+}
+
+// Blocks of the two-stage row reductions: one per `per` rows, at most
+// kRedBlocks (a function of the row count only, so results are reproducible).
+static uint32_t red_blocks(uint64_t rows, uint32_t per)
+{
+    uint64_t b = (rows + per - 1) / per;
+    if (b < 1) b = 1;
+    if (b > (uint64_t)kRedBlocks) b = kRedBlocks;
+    return (uint32_t)b;
 }
 
 static unsigned grid_rows(uint64_t n, uint32_t per_block)
@@ -454,9 +561,9 @@ CpdWork::~CpdWork()
 static int gram(const float *A, uint32_t I, uint32_t R, uint32_t S, double *out, double *partial, hipStream_t s)
 {
     int tok = prof_start("cpd_gram", s);
-    hipLaunchKernelGGL(k_gram_partial, dim3(kRedBlocks), dim3(256), 0, s, A, I, R, S, partial);
-    hipLaunchKernelGGL(k_sum_partials, dim3((R * R + 255) / 256), dim3(256), 0, s, partial, (uint32_t)kRedBlocks,
-                       R * R, out);
+    const uint32_t nb = red_blocks(I, 32);
+    hipLaunchKernelGGL(k_gram_partial, dim3(nb), dim3(256), 0, s, A, I, R, S, partial);
+    hipLaunchKernelGGL(k_sum_partials, dim3(sum_grid(R * R)), dim3(256), 0, s, partial, nb, R * R, out);
     prof_stop(tok, s);
     SPTOL_HIP(hipGetLastError(), "CPD gram");
     return 0;
@@ -466,9 +573,10 @@ static int colnorm(float *A, uint16_t *Abf, uint32_t I, uint32_t R, uint32_t S, 
                    hipStream_t s)
 {
     int tok = prof_start("cpd_norm", s);
-    hipLaunchKernelGGL(k_col_partial, dim3(kRedBlocks), dim3(256), 0, s, A, I, R, S, kind, w.partial);
-    hipLaunchKernelGGL(k_col_final, dim3(1), dim3(kMaxRank), 0, s, w.partial, (uint32_t)kRedBlocks, R, kind, mult,
-                       w.lambda, w.norms);
+    const uint32_t nb = red_blocks(I, 64);
+    hipLaunchKernelGGL(k_col_partial, dim3(nb), dim3(256), 0, s, A, I, R, S, kind, w.partial);
+    hipLaunchKernelGGL(k_col_final, dim3(sum_grid(R)), dim3(256), 0, s, w.partial, nb, R, kind, mult, w.lambda,
+                       w.norms);
     hipLaunchKernelGGL(k_scale, dim3(grid_rows((uint64_t)I * S, 256)), dim3(256), 0, s, A, Abf, I, R, S, w.norms);
     prof_stop(tok, s);
     SPTOL_HIP(hipGetLastError(), "CPD norm");
@@ -479,7 +587,10 @@ static int solve(const float *M, float *A, uint32_t I, uint32_t R, uint32_t S, C
                  hipStream_t s)
 {
     int tok = prof_start("cpd_solve", s);
-    hipLaunchKernelGGL(k_factor, dim3(1), dim3(256), 0, s, w.ata, N, m, R, w.fac, w.piv, w.kind);
+    if (R <= 8) hipLaunchKernelGGL(k_factor<8>, dim3(1), dim3(64), 0, s, w.ata, N, m, R, w.fac, w.piv, w.kind);
+    else if (R <= 16) hipLaunchKernelGGL(k_factor<16>, dim3(1), dim3(64), 0, s, w.ata, N, m, R, w.fac, w.piv, w.kind);
+    else if (R <= 32) hipLaunchKernelGGL(k_factor<32>, dim3(1), dim3(64), 0, s, w.ata, N, m, R, w.fac, w.piv, w.kind);
+    else hipLaunchKernelGGL(k_factor<64>, dim3(1), dim3(64), 0, s, w.ata, N, m, R, w.fac, w.piv, w.kind);
     const unsigned g = grid_rows(I, 256);
     if (R <= 8) hipLaunchKernelGGL(k_solve_rows<8>, dim3(g), dim3(256), 0, s, M, A, I, R, S, w.fac, w.piv, w.kind);
     else if (R <= 16) hipLaunchKernelGGL(k_solve_rows<16>, dim3(g), dim3(256), 0, s, M, A, I, R, S, w.fac, w.piv, w.kind);
@@ -492,7 +603,7 @@ static int solve(const float *M, float *A, uint32_t I, uint32_t R, uint32_t S, C
 
 CpdJob::~CpdJob()
 {
-    if (h_fit) (void)hipHostFree(h_fit);
+    pinned_slot_free(h_fit);
 }
 
 size_t CpdJob::workspace_bytes(const uint32_t *ndims, uint32_t N, uint32_t R, int fdt)
@@ -500,7 +611,8 @@ size_t CpdJob::workspace_bytes(const uint32_t *ndims, uint32_t N, uint32_t R, in
     const uint32_t S = R ? ((R - 1) / 8 + 1) * 8 : 8;
     uint64_t maxI = 1, sumI = 0;
     for (uint32_t m = 0; m < N; ++m) { maxI = ndims[m] > maxI ? ndims[m] : maxI; sumI += ndims[m]; }
-    size_t b = (size_t)maxI * S * 4 + (size_t)N * R * R * 8 + (size_t)kRedBlocks * R * R * 8 + (size_t)R * R * 8 + 4096;
+    size_t b = (size_t)maxI * S * 4 + (size_t)N * R * R * 8 +
+               std::max<size_t>((size_t)kRedBlocks * R * R, kInnerBlocks + 1) * 8 + (size_t)R * R * 8 + 4096;
     if (fdt == SPTOL_BF16) b += (size_t)sumI * S * 2;
     return b;
 }
@@ -521,7 +633,7 @@ int CpdJob::setup(sptolDevCoo X_, uint32_t R_, uint32_t niters_, double tol_, sp
     S = mats[0].stride;
     SPTOL_TRY(dmalloc((void **)&w.M, (size_t)(maxI ? maxI : 1) * S * 4, "CPD"));
     SPTOL_TRY(dmalloc((void **)&w.ata, (size_t)N * R * R * 8, "CPD"));
-    SPTOL_TRY(dmalloc((void **)&w.partial, (size_t)kRedBlocks * R * R * 8, "CPD"));
+    SPTOL_TRY(dmalloc((void **)&w.partial, std::max<size_t>((size_t)kRedBlocks * R * R, kInnerBlocks + 1) * 8, "CPD"));
     SPTOL_TRY(dmalloc((void **)&w.fac, (size_t)R * R * 8, "CPD"));
     SPTOL_TRY(dmalloc((void **)&w.piv, (size_t)R * 4, "CPD"));
     SPTOL_TRY(dmalloc((void **)&w.kind, 4, "CPD"));
@@ -537,10 +649,12 @@ int CpdJob::setup(sptolDevCoo X_, uint32_t R_, uint32_t niters_, double tol_, sp
         if (fdt == SPTOL_BF16) { mt[m].dtype = SPTOL_BF16; mt[m].values = w.bf[m]; }
     }
     Mout.nrows = maxI; Mout.ncols = R; Mout.stride = S; Mout.dtype = SPTOL_F32; Mout.values = w.M;
-    SPTOL_HIP(hipHostMalloc((void **)&h_fit, sizeof(double), hipHostMallocDefault), "CPD");
+    static_assert((kMaxRank + 1) * sizeof(double) <= kPinnedSlotBytes, "fit and weights fit one pinned slot");
+    SPTOL_TRY(pinned_slot((void **)&h_fit));
     // ||X||^2 and the initial Grams (cpd.c:65-70)
     hipLaunchKernelGGL(k_sqnorm_partial, dim3(kRedBlocks), dim3(256), 0, s, X->vals, X->nnz, w.partial);
     hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, s, w.partial, (uint32_t)kRedBlocks, 1u, w.normx);
+    // (k_sqnorm_partial runs kRedBlocks blocks over the nonzeros)
     hipLaunchKernelGGL(k_set, dim3(1), dim3(kMaxRank), 0, s, w.lambda, R, 1.0);
     SPTOL_HIP(hipGetLastError(), "CPD init");
     for (uint32_t m = 0; m < N; ++m) {
@@ -582,20 +696,10 @@ int CpdJob::after_mttkrp(hipStream_t s)
     if (++m < N) return 0;
     // end of an iteration: fit from the last mode's MTTKRP, still in w.M
     const uint32_t last = N - 1;
-    if (exact_fit) {
-        InnerArgs ia;
-        std::memset(&ia, 0, sizeof(ia));
-        for (uint32_t q = 0; q < N; ++q) { ia.inds[q] = X->inds[q]; ia.mats[q] = (const float *)mats[q].values; }
-        ia.vals = X->vals;
-        ia.nnz = X->nnz;
-        ia.N = N; ia.R = R; ia.S = S;
-        hipLaunchKernelGGL(k_inner_exact_partial, dim3(kRedBlocks), dim3(256), 0, s, ia, w.lambda, w.partial);
-    } else {
-        hipLaunchKernelGGL(k_inner_partial, dim3(kRedBlocks), dim3(256), 0, s, (const float *)mats[last].values, w.M,
-                           X->ndims[last], R, S, w.partial);
-    }
-    hipLaunchKernelGGL(k_fit, dim3(1), dim3(256), 0, s, w.ata, N, R, w.lambda, w.partial, (uint32_t)kRedBlocks,
-                       exact_fit ? 1 : 0, w.normx, w.fit);
+    const uint32_t nfit = red_blocks(X->ndims[last], 256);
+    hipLaunchKernelGGL(k_inner_partial, dim3(nfit), dim3(256), 0, s, (const float *)mats[last].values, w.M,
+                       X->ndims[last], R, S, w.partial);
+    hipLaunchKernelGGL(k_fit, dim3(1), dim3(256), 0, s, w.ata, N, R, w.lambda, w.partial, nfit, 0, w.normx, w.fit);
     SPTOL_HIP(hipGetLastError(), "CPD fit");
     SPTOL_HIP(hipMemcpyAsync(h_fit, w.fit, sizeof(double), hipMemcpyDeviceToHost, s), "CPD fit");
     fit_pending = true;
@@ -621,15 +725,50 @@ void CpdJob::fit_ready()
     }
 }
 
-int CpdJob::finish(hipStream_t s, sptValue *lambda_out)
+// Queue the final results: with exact_fit (3- and 4-mode tensors) the fit
+// of the current model with <X,K> by an FP64 pass over the nonzeros (before
+// the renormalisation below, which leaves the model unchanged), then the
+// final weights (GetFinalLambda, matrix.c:557-573); both are copied to the
+// pinned slot.
+int CpdJob::finish_enqueue(hipStream_t s)
 {
-    // GetFinalLambda (matrix.c:557-573)
+    if (exact_fit && N <= 4) {
+        InnerArgs ia;
+        std::memset(&ia, 0, sizeof(ia));
+        for (uint32_t q = 0; q < N; ++q) { ia.inds[q] = X->inds[q]; ia.mats[q] = (const float *)mats[q].values; }
+        ia.vals = X->vals;
+        ia.nnz = X->nnz;
+        ia.N = N; ia.R = R; ia.S = S;
+        const dim3 g(kInnerBlocks);
+        if (S <= 8) hipLaunchKernelGGL(k_inner_exact_partial<2>, g, dim3(256), 0, s, ia, w.lambda, w.partial);
+        else if (S <= 16) hipLaunchKernelGGL(k_inner_exact_partial<4>, g, dim3(256), 0, s, ia, w.lambda, w.partial);
+        else if (S <= 32) hipLaunchKernelGGL(k_inner_exact_partial<8>, g, dim3(256), 0, s, ia, w.lambda, w.partial);
+        else hipLaunchKernelGGL(k_inner_exact_partial<16>, g, dim3(256), 0, s, ia, w.lambda, w.partial);
+        // block partials -> one scalar (kept after the partials)
+        hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, s, w.partial, kInnerBlocks, 1u,
+                           w.partial + kInnerBlocks);
+        hipLaunchKernelGGL(k_fit, dim3(1), dim3(256), 0, s, w.ata, N, R, w.lambda, w.partial + kInnerBlocks, 1u, 1,
+                           w.normx, w.fit);
+        SPTOL_HIP(hipGetLastError(), "CPD exact fit");
+        SPTOL_HIP(hipMemcpyAsync(h_fit, w.fit, sizeof(double), hipMemcpyDeviceToHost, s), "CPD exact fit");
+    }
     for (uint32_t q = 0; q < N; ++q)
         SPTOL_TRY(colnorm((float *)mats[q].values, nullptr, X->ndims[q], R, S, 0, 1, w, s));
-    std::vector<double> lam(R);
-    SPTOL_HIP(hipMemcpyAsync(lam.data(), w.lambda, R * 8, hipMemcpyDeviceToHost, s), "CPD");
+    SPTOL_HIP(hipMemcpyAsync(h_fit + 1, w.lambda, (size_t)R * 8, hipMemcpyDeviceToHost, s), "CPD");
+    return 0;
+}
+
+void CpdJob::finish_read(sptValue *lambda_out)
+{
+    if (exact_fit && N <= 4) fit = h_fit[0];
+    for (uint32_t r = 0; r < R; ++r) lambda_out[r] = (sptValue)h_fit[1 + r];
+}
+
+int CpdJob::finish(hipStream_t s, sptValue *lambda_out)
+{
+    SPTOL_TRY(finish_enqueue(s));
     SPTOL_HIP(hipStreamSynchronize(s), "CPD");
-    for (uint32_t r = 0; r < R; ++r) lambda_out[r] = (sptValue)lam[r];
+    finish_read(lambda_out);
     return 0;
 }
 
@@ -643,6 +782,8 @@ int sptolCpdAlsDevice(sptolDevCoo X, sptIndex rank, sptIndex niters, double tol,
                       sptValue *lambda_out, sptolCpdStats *stats, void *stream)
 {
     hipStream_t s = stream_of(stream);
+    // the workspace below is freed after the final synchronisation
+    StreamScope scope(s);
     auto t0 = std::chrono::steady_clock::now();
     for (uint32_t m = 0; m < X->nmodes; ++m) SPTOL_TRY(coo_prepare(X, m, s));
     CpdJob job;

@@ -38,8 +38,8 @@ struct CpdJob {
     double fit = 0.0, oldfit = 0.0;
     int iters = 0;
     bool done = false, fit_pending = false;
-    bool exact_fit = false;               // <X,K> by an FP64 pass over the nonzeros
-    double *h_fit = nullptr;              // pinned
+    bool exact_fit = false;               // final fit with <X,K> by an FP64 pass over the nonzeros
+    double *h_fit = nullptr;              // pinned slot: [0] fit, [1 .. R] final lambda
     std::chrono::steady_clock::time_point t_iter;
 
     ~CpdJob();
@@ -55,7 +55,12 @@ struct CpdJob {
     // fit_pending (call fit_ready() after the stream synchronises).
     int after_mttkrp(hipStream_t s);
     void fit_ready();
-    int finish(hipStream_t s, sptValue *lambda_out);   // final lambda
+    // Final weights (and the exact fit when exact_fit): finish_enqueue
+    // queues the kernels and the copies to the pinned slot; finish_read
+    // takes the results once the stream has passed them.  finish does both.
+    int finish_enqueue(hipStream_t s);
+    void finish_read(sptValue *lambda_out);
+    int finish(hipStream_t s, sptValue *lambda_out);
 };
 
 }  // namespace sptol

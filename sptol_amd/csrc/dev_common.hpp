@@ -30,8 +30,31 @@
 namespace sptol {
 
 // Budget-tracked device allocation (zero-size requests return nullptr).
+// dmalloc may hand out a cached block (see StreamScope); dfree returns the
+// block to HIP, or, inside a StreamScope, to the cache.
 int dmalloc(void **ptr, size_t bytes, const char *module);
 void dfree(void *ptr);
+
+// While a StreamScope(s) is alive on a thread, the thread's dfree calls keep
+// their blocks for reuse instead of returning them to HIP.  The caller
+// promises that every use of a block freed in the scope is ordered on s
+// before the dfree call (work enqueued on s, or on streams s was made to
+// wait for, or already complete).  dmalloc calls in the scope may reuse such
+// a block before that work has run, as their own work is queued behind it.
+// Page-locked host slot of kPinnedSlotBytes (process-lifetime slabs).
+constexpr size_t kPinnedSlotBytes = 1024;
+int pinned_slot(void **p);
+void pinned_slot_free(void *p);
+
+struct StreamScope {
+    explicit StreamScope(hipStream_t s);
+    ~StreamScope();
+    StreamScope(const StreamScope &) = delete;
+    StreamScope &operator=(const StreamScope &) = delete;
+private:
+    hipStream_t prev_;
+    int prev_on_;
+};
 hipStream_t stream_of(void *s);
 int verbose();
 
@@ -139,6 +162,8 @@ int dup_on(void **dst, const void *src, size_t bytes, hipStream_t s, const char 
 // Stream-aware forms of sptolDevCooCreate / sptolDevCooPrepareMode.
 int coo_create(sptolDevCoo *h, const sptSparseTensor *X, hipStream_t s);
 int coo_prepare(sptolDevCoo h, uint32_t mode, hipStream_t s);
+// *finite = no NaN or infinity among vals[0..n) (synchronises s).
+int check_finite(const float *vals, uint64_t n, hipStream_t s, bool *finite);
 // Device MTTKRP on stream s (sptolDevMTTKRP without the argument checks).
 int coo_mttkrp(sptolDevCoo X, const sptolDevMatrix *mats, const uint32_t *order, uint32_t mode,
                sptolDevMatrix *out, hipStream_t s);
@@ -176,6 +201,14 @@ struct SegBatchDev {
     size_t off_items[8] = {}, off_prefix[8] = {}, off_all = 0, off_cprefix = 0;
     uint32_t n[8] = {}, n_all = 0;
     uint64_t total[8] = {}, carry_blocks = 0;
+    // pinned staging of the tables: two buffers used in turn, each reused
+    // once the copy that read it has run, so the host never waits for the
+    // device to drain before queuing the next launch
+    uint8_t *stage[2] = {};
+    size_t stage_cap[2] = {};
+    hipEvent_t stage_ev[2] = {};
+    bool stage_used[2] = {};
+    int stage_cur = 0;
     ~SegBatchDev();
 };
 

@@ -11,6 +11,8 @@
 #include <atomic>
 #include <mutex>
 #include <unordered_map>
+#include <map>
+#include <cstdlib>
 #include <vector>
 #include <cstring>
 #include <cstdio>
@@ -19,7 +21,8 @@
 namespace sptol {
 
 static std::mutex g_mem_mu;
-static std::unordered_map<void *, size_t> g_live;   // device pointer -> bytes
+struct LiveBlock { size_t bytes, rounded; };
+static std::unordered_map<void *, LiveBlock> g_live;  // device pointer -> sizes
 static size_t g_in_use = 0;
 static size_t g_budget = 0;                          // 0 = unlimited
 static int g_fail_countdown = 0;
@@ -27,10 +30,135 @@ static std::atomic<int> g_verbose{0};
 
 int verbose() { return g_verbose.load(); }
 
+// ---------------------------------------------------------------------------
+// Block cache.  hipFree waits for every stream on the device, so a serving
+// loop that frees a finished request's arrays while other requests run
+// would stall them all.  Blocks freed inside a StreamScope are kept instead,
+// tagged with an event recorded on the scope's stream: a later allocation
+// on that same stream may take the block at once (stream order covers it),
+// an allocation elsewhere once the event has completed.  Blocks freed
+// outside a scope go back to HIP as before.  Sizes are rounded up to one of
+// eight steps per power of two so that blocks of similar size are shared.
+// ---------------------------------------------------------------------------
+struct CachedBlock {
+    void *p;
+    hipStream_t s;
+    hipEvent_t ev;   // nullptr: no pending work
+};
+static std::multimap<size_t, CachedBlock> g_cache;   // rounded size -> block
+static size_t g_cached = 0;
+static size_t g_cache_cap = (size_t)64 << 30;
+static bool g_cache_cap_set = false;
+static std::vector<hipEvent_t> g_spare_events;
+static thread_local hipStream_t tl_stream = nullptr;
+static thread_local int tl_scoped = 0;
+
+StreamScope::StreamScope(hipStream_t s) : prev_(tl_stream), prev_on_(tl_scoped)
+{
+    tl_stream = s;
+    tl_scoped = 1;
+}
+
+StreamScope::~StreamScope()
+{
+    tl_stream = prev_;
+    tl_scoped = prev_on_;
+}
+
+static size_t round_block(size_t n)
+{
+    if (n <= 512) return 512;
+    int lg = 63 - __builtin_clzll((unsigned long long)n);
+    const size_t step = (size_t)1 << (lg - 3);
+    return (n + step - 1) / step * step;
+}
+
+// Take a cached block of `rounded` bytes usable on the caller's stream
+// (caller holds g_mem_mu).
+static void *cache_take(size_t rounded)
+{
+    auto range = g_cache.equal_range(rounded);
+    int scanned = 0;
+    for (auto it = range.first; it != range.second && scanned < 16; ++it, ++scanned) {
+        CachedBlock &b = it->second;
+        bool ok = b.ev == nullptr || (tl_scoped && b.s == tl_stream);
+        if (!ok && hipEventQuery(b.ev) == hipSuccess) ok = true;
+        if (!ok) continue;
+        void *p = b.p;
+        if (b.ev) g_spare_events.push_back(b.ev);
+        g_cache.erase(it);
+        g_cached -= rounded;
+        return p;
+    }
+    return nullptr;
+}
+
+// Return every cached block to HIP after the device has drained.
+static void cache_release_all()
+{
+    std::vector<void *> ps;
+    std::vector<hipEvent_t> evs;
+    {
+        std::lock_guard<std::mutex> lk(g_mem_mu);
+        for (auto &kv : g_cache) {
+            ps.push_back(kv.second.p);
+            if (kv.second.ev) evs.push_back(kv.second.ev);
+        }
+        g_cache.clear();
+        g_cached = 0;
+        for (auto e : g_spare_events) evs.push_back(e);
+        g_spare_events.clear();
+    }
+    if (ps.empty() && evs.empty()) return;
+    (void)hipDeviceSynchronize();
+    for (void *p : ps) (void)hipFree(p);
+    for (auto e : evs) (void)hipEventDestroy(e);
+}
+
+static size_t cache_cap()
+{
+    if (!g_cache_cap_set) {
+        g_cache_cap_set = true;
+        if (const char *e = getenv("SPTOL_CACHE_BYTES")) g_cache_cap = (size_t)strtoull(e, nullptr, 10);
+    }
+    return g_cache_cap;
+}
+
+// ---------------------------------------------------------------------------
+// Pinned host slots: small page-locked buffers (fits, weights) that async
+// device-to-host copies land in.  Carved from 1 MiB pinned slabs that live
+// until the process ends, so taking and returning a slot never calls the
+// runtime's pinned allocator (whose free waits for the device).
+// ---------------------------------------------------------------------------
+static std::mutex g_pin_mu;
+static std::vector<void *> g_pin_free;
+
+int pinned_slot(void **p)
+{
+    std::lock_guard<std::mutex> lk(g_pin_mu);
+    if (g_pin_free.empty()) {
+        void *slab = nullptr;
+        SPTOL_HIP(hipHostMalloc(&slab, (size_t)1 << 20, hipHostMallocDefault), "HipWrap pinned slot");
+        for (size_t k = 0; k < ((size_t)1 << 20) / kPinnedSlotBytes; ++k)
+            g_pin_free.push_back((char *)slab + k * kPinnedSlotBytes);
+    }
+    *p = g_pin_free.back();
+    g_pin_free.pop_back();
+    return 0;
+}
+
+void pinned_slot_free(void *p)
+{
+    if (!p) return;
+    std::lock_guard<std::mutex> lk(g_pin_mu);
+    g_pin_free.push_back(p);
+}
+
 int dmalloc(void **ptr, size_t bytes, const char *module)
 {
     *ptr = nullptr;
     if (bytes == 0) return 0;
+    const size_t rounded = round_block(bytes);
     {
         std::lock_guard<std::mutex> lk(g_mem_mu);
         if (g_fail_countdown > 0 && --g_fail_countdown == 0) {
@@ -44,8 +172,18 @@ int dmalloc(void **ptr, size_t bytes, const char *module)
             return SPTERR_CUDA_ERROR + (int)hipErrorOutOfMemory;
         }
         g_in_use += bytes;
+        if (void *p = cache_take(rounded)) {
+            g_live[p] = LiveBlock{bytes, rounded};
+            *ptr = p;
+            return 0;
+        }
     }
-    hipError_t e = hipMalloc(ptr, bytes);
+    hipError_t e = hipMalloc(ptr, rounded);
+    if (e == hipErrorOutOfMemory) {
+        (void)hipGetLastError();
+        cache_release_all();
+        e = hipMalloc(ptr, rounded);
+    }
     if (e != hipSuccess) {
         std::lock_guard<std::mutex> lk(g_mem_mu);
         g_in_use -= bytes;
@@ -54,20 +192,38 @@ int dmalloc(void **ptr, size_t bytes, const char *module)
         return SPTERR_CUDA_ERROR + (int)e;
     }
     std::lock_guard<std::mutex> lk(g_mem_mu);
-    g_live[*ptr] = bytes;
+    g_live[*ptr] = LiveBlock{bytes, rounded};
     return 0;
 }
 
 void dfree(void *ptr)
 {
     if (!ptr) return;
+    size_t rounded = 0;
+    hipEvent_t ev = nullptr;
     {
         std::lock_guard<std::mutex> lk(g_mem_mu);
         auto it = g_live.find(ptr);
         if (it != g_live.end()) {
-            g_in_use -= it->second;
+            g_in_use -= it->second.bytes;
+            rounded = it->second.rounded;
             g_live.erase(it);
         }
+        if (rounded && tl_scoped && g_cached + rounded <= cache_cap()) {
+            if (!g_spare_events.empty()) { ev = g_spare_events.back(); g_spare_events.pop_back(); }
+        } else {
+            rounded = 0;
+        }
+    }
+    if (rounded) {
+        if (!ev && hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) ev = nullptr;
+        if (ev && hipEventRecord(ev, tl_stream) == hipSuccess) {
+            std::lock_guard<std::mutex> lk(g_mem_mu);
+            g_cache.emplace(rounded, CachedBlock{ptr, tl_stream, ev});
+            g_cached += rounded;
+            return;
+        }
+        if (ev) (void)hipEventDestroy(ev);
     }
     (void)hipFree(ptr);
 }
@@ -217,6 +373,18 @@ size_t sptolHipMemoryInUse(void)
 {
     std::lock_guard<std::mutex> lk(g_mem_mu);
     return g_in_use;
+}
+
+size_t sptolHipMemoryCached(void)
+{
+    std::lock_guard<std::mutex> lk(g_mem_mu);
+    return g_cached;
+}
+
+int sptolHipReleaseCache(void)
+{
+    cache_release_all();
+    return 0;
 }
 
 int sptolHipInjectMallocFailure(int n)

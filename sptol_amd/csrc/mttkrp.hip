@@ -428,20 +428,51 @@ int dup_on(void **dst, const void *src, size_t bytes, hipStream_t s, const char 
     return 0;
 }
 
+// bad[0] |= 1 when some index of mode m is >= ndims[m]
+__global__ void k_check_inds(const uint32_t *__restrict__ idx, uint64_t n, uint32_t lim, uint32_t *__restrict__ bad)
+{
+    uint32_t b = 0;
+    for (uint64_t z = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; z < n; z += (uint64_t)gridDim.x * blockDim.x)
+        b |= idx[z] >= lim ? 1u : 0u;
+    if (b) atomicOr(bad, 1u);
+}
+
+// bad[0] |= 1 when some value is NaN or infinite
+__global__ void k_check_finite(const float *__restrict__ v, uint64_t n, uint32_t *__restrict__ bad)
+{
+    uint32_t b = 0;
+    for (uint64_t z = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; z < n; z += (uint64_t)gridDim.x * blockDim.x)
+        b |= (__float_as_uint(v[z]) & 0x7F800000u) == 0x7F800000u ? 1u : 0u;
+    if (b) atomicOr(bad, 1u);
+}
+
+int check_finite(const float *vals, uint64_t n, hipStream_t s, bool *finite)
+{
+    *finite = true;
+    if (!n) return 0;
+    StreamScope scope(s);
+    uint32_t *bad = nullptr;
+    SPTOL_TRY(dmalloc((void **)&bad, 4, "Check finite"));
+    uint32_t hb = 0;
+    hipError_t e = hipMemsetAsync(bad, 0, 4, s);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_check_finite, dim3(grid_for(n)), dim3(256), 0, s, vals, n, bad);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(&hb, bad, 4, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    dfree(bad);
+    SPTOL_HIP(e, "Check finite");
+    *finite = hb == 0;
+    return 0;
+}
+
 int coo_create(sptolDevCoo *h, const sptSparseTensor *X, hipStream_t s)
 {
     *h = nullptr;
+    StreamScope scope(s);   // everything freed here was used on s only
     if (X->nmodes < 2 || X->nmodes > kMaxModes)
         SPTOL_CHECK(SPTERR_VALUE_ERROR, "DevCoo New", "nmodes must be in [2, 8]");
-    // The kernels index factor rows with these values, so reject any index
-    // >= ndims on the host before anything reaches the device (the reference
-    // trusts its input; here an out-of-range index would be a device fault).
-    for (uint32_t m = 0; m < X->nmodes; ++m) {
-        const uint32_t *a = X->inds[m].data, lim = X->ndims[m];
-        uint32_t mx = 0;
-        for (uint64_t z = 0; z < X->nnz; ++z) mx = a[z] > mx ? a[z] : mx;
-        if (X->nnz && mx >= lim) SPTOL_CHECK(SPTERR_VALUE_ERROR, "DevCoo New", "index >= ndims");
-    }
     sptolDevCoo t = new sptolDevCoo_s();
     t->nmodes = X->nmodes;
     t->nnz = X->nnz;
@@ -451,6 +482,24 @@ int coo_create(sptolDevCoo *h, const sptSparseTensor *X, hipStream_t s)
         r = dup_on((void **)&t->inds[m], X->inds[m].data, X->nnz * sizeof(uint32_t), s, "DevCoo New");
     if (!r) r = dup_on((void **)&t->vals, X->values.data, X->nnz * sizeof(float), s, "DevCoo New");
     if (r) { sptolDevCooFree(t); return r; }
+    // The kernels index factor rows with these values, so reject any index
+    // >= ndims before any kernel uses them (the reference trusts its input;
+    // here an out-of-range index would be a device fault).
+    if (X->nnz) {
+        uint32_t *bad = nullptr;
+        if ((r = dmalloc((void **)&bad, 4, "DevCoo New"))) { sptolDevCooFree(t); return r; }
+        uint32_t hb = 0;
+        hipError_t e = hipMemsetAsync(bad, 0, 4, s);
+        for (uint32_t m = 0; m < t->nmodes && e == hipSuccess; ++m) {
+            hipLaunchKernelGGL(k_check_inds, dim3(grid_for(X->nnz)), dim3(256), 0, s, t->inds[m], X->nnz, X->ndims[m], bad);
+            e = hipGetLastError();
+        }
+        if (e == hipSuccess) e = hipMemcpyAsync(&hb, bad, 4, hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        dfree(bad);
+        if (e != hipSuccess) { sptolDevCooFree(t); SPTOL_HIP(e, "DevCoo New"); }
+        if (hb) { sptolDevCooFree(t); SPTOL_CHECK(SPTERR_VALUE_ERROR, "DevCoo New", "index >= ndims"); }
+    }
     *h = t;
     return 0;
 }
@@ -460,6 +509,7 @@ int coo_prepare(sptolDevCoo h, uint32_t mode, hipStream_t s)
     if (mode >= h->nmodes) SPTOL_CHECK(SPTERR_VALUE_ERROR, "DevCoo Prepare", "mode out of range");
     SortedCopy &sc = h->sorted[mode];
     if (sc.ready || h->nnz == 0) { sc.ready = true; return 0; }
+    StreamScope scope(s);   // everything freed here was used on s only
     if (h->nnz >= 0xFFFFFFFFull) SPTOL_CHECK(SPTERR_VALUE_ERROR, "DevCoo Prepare", "nnz >= 2^32");
     const uint64_t n = h->nnz;
     uint32_t *perm = nullptr, *perm2 = nullptr, *keys2 = nullptr;

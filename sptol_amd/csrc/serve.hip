@@ -3,19 +3,23 @@
 // The reference solves one tensor at a time on the host (src/sptensor/cpd.c).
 // Here requests (tensor, rank, iteration cap, tolerance, seed) arrive at any
 // time and are served concurrently on one GPU:
-//   * an admission thread takes queued requests in arrival order, reserves
-//     their device footprint against an HBM budget (waiting while admitted
-//     work holds the budget or all slots are full, rejecting requests that
-//     could never fit), uploads and prepares them on its own stream, and
-//     hands them to the compute thread -- so uploads overlap with compute;
+//   * admission threads (admit_threads, each with its own stream) take
+//     queued requests in arrival order, reserve their device footprint
+//     against an HBM budget (waiting while admitted work holds the budget or
+//     all slots are full, rejecting requests that could never fit), upload
+//     and prepare them, and hand them to the compute thread -- so uploads
+//     and preparation overlap with each other and with compute;
 //   * a compute thread advances every active request by one ALS mode-step
 //     per tick: the MTTKRPs of all of them run as one persistent batched
 //     launch, followed by each request's FP64 solve / normalise / Gram
 //     kernels; at the end of an iteration the request's fit is copied back
 //     behind an event, and the request sits out ticks until the event has
 //     completed, so the host never blocks the device to read a fit;
-//   * finished requests copy their factors to host memory, release their
-//     device memory and budget, and wake the admission thread and waiters.
+//   * a finished request's final kernels are queued behind an event and the
+//     request goes to a completion thread, which copies its factors to host
+//     memory, returns its device memory to the block cache (no device-wide
+//     hipFree wait while other requests run), releases its budget and wakes
+//     the admission threads and waiters.
 // A request that fails (malformed input, allocation failure, budget) is
 // reported with its error code and never affects the others: each request
 // owns its tensor, factors and workspace, and the batched kernel writes only
@@ -41,12 +45,15 @@ enum { REQ_QUEUED = 0, REQ_PREPARING = 1, REQ_RUNNING = 2, REQ_DONE = 3, REQ_FAI
 
 struct Request {
     uint64_t id = 0;
-    // input (host copy, released after upload)
+    // input (host copy, released after upload; or the caller's arrays when
+    // the server borrows its inputs)
     uint32_t nmodes = 0;
     uint32_t ndims[kMaxModes] = {};
     uint64_t nnz = 0;
     std::vector<uint32_t> inds[kMaxModes];
     std::vector<float> vals;
+    const uint32_t *src_inds[kMaxModes] = {};
+    const float *src_vals = nullptr;
     uint32_t R = 0, niters = 0;
     double tol = 0.0;
     uint64_t seed = 0;
@@ -68,6 +75,7 @@ struct Request {
     double fit = 0.0;
     int iterations = 0;
 
+    // Callers hold a StreamScope of a stream ordered after every use.
     void free_device()
     {
         delete job;
@@ -108,16 +116,20 @@ using namespace sptol;
 struct sptolServer_s {
     sptolServerConfig cfg;
     std::mutex mu;
-    std::condition_variable cv_admit, cv_compute, cv_done;
+    std::condition_variable cv_admit, cv_compute, cv_done, cv_release;
     std::deque<Request *> queued;
     std::vector<Request *> ready;
+    std::deque<Request *> completing;   // final kernels queued behind fit_event
     std::unordered_map<uint64_t, Request *> all;
     size_t reserved = 0;
     int admitted = 0;           // preparing + ready + active
     uint64_t next_id = 1;
     bool stop = false;
-    std::thread admit_thr, compute_thr;
-    hipStream_t admit_stream = nullptr, compute_stream = nullptr;
+    std::vector<std::thread> admit_thr;
+    std::thread compute_thr, release_thr;
+    bool stop_release = false;
+    std::vector<hipStream_t> admit_stream;
+    hipStream_t compute_stream = nullptr, release_stream = nullptr;
     int device = 0;
     std::atomic<uint64_t> ticks{0};
 };
@@ -137,15 +149,14 @@ static void finish_request(sptolServer srv, Request *r, int err)
     srv->cv_done.notify_all();
 }
 
-static int prepare_request(sptolServer srv, Request *r)
+static int prepare_request(sptolServer srv, Request *r, hipStream_t s)
 {
-    hipStream_t s = srv->admit_stream;
     // host view of the copied tensor
     sptIndexVector iv[kMaxModes];
     sptIndex nd[kMaxModes], so[kMaxModes];
     for (uint32_t m = 0; m < r->nmodes; ++m) {
         iv[m].len = iv[m].cap = r->nnz;
-        iv[m].data = r->inds[m].data();
+        iv[m].data = const_cast<uint32_t *>(r->src_inds[m]);
         nd[m] = r->ndims[m];
         so[m] = m;
     }
@@ -156,8 +167,11 @@ static int prepare_request(sptolServer srv, Request *r)
     X.nnz = r->nnz;
     X.inds = iv;
     X.values.len = X.values.cap = r->nnz;
-    X.values.data = r->vals.data();
+    X.values.data = const_cast<float *>(r->src_vals);
     SPTOL_TRY(coo_create(&r->X, &X, s));
+    bool finite = true;
+    SPTOL_TRY(check_finite(r->X->vals, r->nnz, s, &finite));
+    if (!finite) SPTOL_CHECK(SPTERR_VALUE_ERROR, "Serve", "non-finite value");
     for (uint32_t m = 0; m < r->nmodes; ++m) SPTOL_TRY(coo_prepare(r->X, m, s));
     uint64_t st = r->seed;
     for (uint32_t m = 0; m < r->nmodes; ++m) {
@@ -176,43 +190,53 @@ static int prepare_request(sptolServer srv, Request *r)
     return 0;
 }
 
-static void admit_loop(sptolServer srv)
+static void admit_loop(sptolServer srv, int worker)
 {
     (void)hipSetDevice(srv->device);
+    hipStream_t s = srv->admit_stream[worker];
     for (;;) {
         Request *r = nullptr;
         {
             std::unique_lock<std::mutex> lk(srv->mu);
-            srv->cv_admit.wait(lk, [&] { return srv->stop || !srv->queued.empty(); });
-            if (srv->stop) return;
-            r = srv->queued.front();
-            const size_t need = estimate_bytes(*r);
-            if (need > srv->cfg.hbm_budget) {
-                srv->queued.pop_front();
-                srv->admitted += 1;   // balanced by finish_request
-                lk.unlock();
-                sptol_complain("Serve", SPTERR_CUDA_ERROR + (int)hipErrorOutOfMemory, __FILE__, __LINE__,
-                               "request exceeds the HBM budget");
-                finish_request(srv, r, SPTERR_CUDA_ERROR + (int)hipErrorOutOfMemory);
-                continue;
+            // the first queued request, once it fits the budget and a slot
+            // (every worker waits on the same front, so arrival order holds)
+            for (;;) {
+                srv->cv_admit.wait(lk, [&] { return srv->stop || !srv->queued.empty(); });
+                if (srv->stop) return;
+                Request *f = srv->queued.front();
+                const size_t need = estimate_bytes(*f);
+                if (need > srv->cfg.hbm_budget) {
+                    srv->queued.pop_front();
+                    srv->admitted += 1;   // balanced by finish_request
+                    lk.unlock();
+                    sptol_complain("Serve", SPTERR_CUDA_ERROR + (int)hipErrorOutOfMemory, __FILE__, __LINE__,
+                                   "request exceeds the HBM budget");
+                    finish_request(srv, f, SPTERR_CUDA_ERROR + (int)hipErrorOutOfMemory);
+                    lk.lock();
+                    continue;
+                }
+                if (srv->reserved + need <= srv->cfg.hbm_budget && srv->admitted < srv->cfg.max_active) {
+                    srv->queued.pop_front();
+                    srv->reserved += need;
+                    f->reserved = need;
+                    srv->admitted += 1;
+                    f->status = REQ_PREPARING;
+                    f->admitted_once = true;
+                    f->t_admit = clk::now();
+                    r = f;
+                    break;
+                }
+                srv->cv_admit.wait(lk);   // budget or slots free up, or the queue changes
             }
-            srv->cv_admit.wait(lk, [&] {
-                return srv->stop || (srv->reserved + need <= srv->cfg.hbm_budget && srv->admitted < srv->cfg.max_active);
-            });
-            if (srv->stop) return;
-            srv->queued.pop_front();
-            srv->reserved += need;
-            r->reserved = need;
-            srv->admitted += 1;
-            r->status = REQ_PREPARING;
-            r->admitted_once = true;
-            r->t_admit = clk::now();
         }
-        int err = prepare_request(srv, r);
+        StreamScope scope(s);   // the request's arrays are used on s until it is handed over
+        int err = prepare_request(srv, r, s);
         for (auto &v : r->inds) std::vector<uint32_t>().swap(v);
         std::vector<float>().swap(r->vals);
+        for (auto &p : r->src_inds) p = nullptr;
+        r->src_vals = nullptr;
         if (err) {
-            (void)hipStreamSynchronize(srv->admit_stream);
+            (void)hipStreamSynchronize(s);
             r->free_device();
             finish_request(srv, r, err);
             continue;
@@ -221,28 +245,72 @@ static void admit_loop(sptolServer srv)
         r->status = REQ_RUNNING;
         srv->ready.push_back(r);
         srv->cv_compute.notify_all();
+        srv->cv_admit.notify_all();
     }
 }
 
+// Queue a converged request's final kernels on the compute stream and hand
+// it to the completion thread.
 static void complete(sptolServer srv, Request *r)
 {
     hipStream_t s = srv->compute_stream;
     r->lambda.resize(r->R);
-    int err = r->job->finish(s, r->lambda.data());
+    int err = r->job->finish_enqueue(s);
     if (!err) {
-        r->fit = r->job->fit;
-        r->iterations = r->job->iters;
-        r->factors.resize(r->nmodes);
-        for (uint32_t m = 0; m < r->nmodes && !err; ++m) {
-            r->factors[m].resize((size_t)r->ndims[m] * r->mats[m].stride);
-            hipError_t e = hipMemcpyAsync(r->factors[m].data(), r->mats[m].values, r->factors[m].size() * 4,
-                                          hipMemcpyDeviceToHost, s);
-            if (e == hipSuccess) e = hipStreamSynchronize(s);
-            if (e != hipSuccess) err = SPTERR_CUDA_ERROR + (int)e;
-        }
+        hipError_t e = hipEventRecord(r->fit_event, s);
+        if (e != hipSuccess) err = SPTERR_CUDA_ERROR + (int)e;
     }
-    r->free_device();
-    finish_request(srv, r, err);
+    if (err) {
+        (void)hipStreamSynchronize(s);
+        StreamScope scope(s);
+        r->free_device();
+        finish_request(srv, r, err);
+        return;
+    }
+    std::lock_guard<std::mutex> lk(srv->mu);
+    srv->completing.push_back(r);
+    srv->cv_release.notify_one();
+}
+
+// Completion thread: wait for a request's final kernels, copy its results
+// to host memory and release its device memory.
+static void release_loop(sptolServer srv)
+{
+    (void)hipSetDevice(srv->device);
+    hipStream_t s = srv->release_stream;
+    for (;;) {
+        Request *r = nullptr;
+        {
+            std::unique_lock<std::mutex> lk(srv->mu);
+            srv->cv_release.wait(lk, [&] { return srv->stop_release || !srv->completing.empty(); });
+            if (srv->completing.empty()) return;
+            r = srv->completing.front();
+            srv->completing.pop_front();
+        }
+        int err = 0;
+        hipError_t e = hipEventSynchronize(r->fit_event);
+        if (e != hipSuccess) err = SPTERR_CUDA_ERROR + (int)e;
+        if (!err) {
+            r->job->finish_read(r->lambda.data());
+            r->fit = r->job->fit;
+            r->iterations = r->job->iters;
+            r->factors.resize(r->nmodes);
+            for (uint32_t m = 0; m < r->nmodes && !err; ++m) {
+                r->factors[m].resize((size_t)r->ndims[m] * r->mats[m].stride);
+                e = hipMemcpyAsync(r->factors[m].data(), r->mats[m].values, r->factors[m].size() * 4,
+                                   hipMemcpyDeviceToHost, s);
+                if (e == hipSuccess) e = hipStreamSynchronize(s);
+                if (e != hipSuccess) err = SPTERR_CUDA_ERROR + (int)e;
+            }
+        }
+        {
+            // every use of the request's memory has completed (event and
+            // copies waited for above)
+            StreamScope scope(s);
+            r->free_device();
+        }
+        finish_request(srv, r, err);
+    }
 }
 
 static void compute_loop(sptolServer srv)
@@ -305,6 +373,7 @@ static void compute_loop(sptolServer srv)
         }
         int err = 0;
         if (srv->cfg.batch_mttkrp) {
+            StreamScope scope(s);   // batch buffers that grow were used on s only
             for (int k = 0; k < 2 && !err; ++k)
                 err = batch_launch(descs[k], k ? SPTOL_BF16 : SPTOL_F32, srv->cfg.tile_scale, srv->cfg.blocks_per_cu, bb, s);
         }
@@ -319,6 +388,7 @@ static void compute_loop(sptolServer srv)
             if (e) {
                 (void)hipStreamSynchronize(s);
                 r->job->done = true;
+                StreamScope scope(s);
                 r->free_device();
                 finish_request(srv, r, e);
                 active.erase(std::find(active.begin(), active.end(), r));
@@ -341,15 +411,26 @@ int sptolServerCreate(sptolServer *out, const sptolServerConfig *cfg)
     s->cfg = *cfg;
     if (!(s->cfg.tile_scale > 0)) s->cfg.tile_scale = 1.0;
     if (s->cfg.blocks_per_cu < 1) s->cfg.blocks_per_cu = 8;
+    if (s->cfg.admit_threads < 1) s->cfg.admit_threads = 4;
+    if (s->cfg.admit_threads > 16) s->cfg.admit_threads = 16;
     (void)hipGetDevice(&s->device);
-    hipError_t e = hipStreamCreateWithFlags(&s->admit_stream, hipStreamNonBlocking);
-    if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->compute_stream, hipStreamNonBlocking);
+    hipError_t e = hipStreamCreateWithFlags(&s->compute_stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->release_stream, hipStreamNonBlocking);
+    for (int w = 0; w < s->cfg.admit_threads && e == hipSuccess; ++w) {
+        hipStream_t as = nullptr;
+        e = hipStreamCreateWithFlags(&as, hipStreamNonBlocking);
+        if (e == hipSuccess) s->admit_stream.push_back(as);
+    }
     if (e != hipSuccess) {
+        for (auto as : s->admit_stream) (void)hipStreamDestroy(as);
+        if (s->compute_stream) (void)hipStreamDestroy(s->compute_stream);
+        if (s->release_stream) (void)hipStreamDestroy(s->release_stream);
         delete s;
         SPTOL_HIP(e, "Serve");
     }
-    s->admit_thr = std::thread(admit_loop, s);
+    for (int w = 0; w < s->cfg.admit_threads; ++w) s->admit_thr.emplace_back(admit_loop, s, w);
     s->compute_thr = std::thread(compute_loop, s);
+    s->release_thr = std::thread(release_loop, s);
     *out = s;
     return 0;
 }
@@ -360,16 +441,29 @@ int sptolServerSubmit(sptolServer srv, const sptSparseTensor *X, sptIndex rank, 
     if (X->nmodes < 2 || X->nmodes > (uint32_t)kMaxModes) SPTOL_CHECK(SPTERR_VALUE_ERROR, "Serve", "nmodes must be in [2, 8]");
     if (rank < 1 || rank > (uint32_t)kMaxRank) SPTOL_CHECK(SPTERR_VALUE_ERROR, "Serve", "rank must be in [1, 64]");
     if (factor_dtype != SPTOL_F32 && factor_dtype != SPTOL_BF16) SPTOL_CHECK(SPTERR_VALUE_ERROR, "Serve", "bad dtype");
-    for (uint64_t z = 0; z < X->nnz; ++z)
-        if (!std::isfinite(X->values.data[z])) SPTOL_CHECK(SPTERR_VALUE_ERROR, "Serve", "non-finite value");
     Request *r = new Request();
     r->nmodes = X->nmodes;
     r->nnz = X->nnz;
     for (uint32_t m = 0; m < X->nmodes; ++m) {
         r->ndims[m] = X->ndims[m];
-        r->inds[m].assign(X->inds[m].data, X->inds[m].data + X->nnz);
+        if (srv->cfg.borrow_inputs) {
+            r->src_inds[m] = X->inds[m].data;
+        } else {
+            r->inds[m].assign(X->inds[m].data, X->inds[m].data + X->nnz);
+            r->src_inds[m] = r->inds[m].data();
+        }
     }
-    r->vals.assign(X->values.data, X->values.data + X->nnz);
+    if (srv->cfg.borrow_inputs) {
+        r->src_vals = X->values.data;   // non-finite values fail the request at admission
+    } else {
+        for (uint64_t z = 0; z < X->nnz; ++z)
+            if (!std::isfinite(X->values.data[z])) {
+                delete r;
+                SPTOL_CHECK(SPTERR_VALUE_ERROR, "Serve", "non-finite value");
+            }
+        r->vals.assign(X->values.data, X->values.data + X->nnz);
+        r->src_vals = r->vals.data();
+    }
     r->R = rank;
     r->niters = niters;
     r->tol = tol;
@@ -463,14 +557,25 @@ void sptolServerDestroy(sptolServer srv)
         srv->cv_admit.notify_all();
         srv->cv_compute.notify_all();
     }
-    srv->admit_thr.join();
+    for (auto &t : srv->admit_thr) t.join();
     srv->compute_thr.join();
+    {
+        std::lock_guard<std::mutex> lk(srv->mu);
+        srv->stop_release = true;
+        srv->cv_release.notify_all();
+    }
+    srv->release_thr.join();
+    (void)hipDeviceSynchronize();
     for (auto &kv : srv->all) {
-        kv.second->free_device();
+        {
+            StreamScope scope(srv->release_stream);   // the device has drained
+            kv.second->free_device();
+        }
         delete kv.second;
     }
-    (void)hipStreamDestroy(srv->admit_stream);
+    for (auto as : srv->admit_stream) (void)hipStreamDestroy(as);
     (void)hipStreamDestroy(srv->compute_stream);
+    (void)hipStreamDestroy(srv->release_stream);
     delete srv;
 }
 

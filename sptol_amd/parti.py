@@ -85,7 +85,8 @@ class sptolCpdStats(C.Structure):
 
 class sptolServerConfig(C.Structure):
     _fields_ = [("hbm_budget", C.c_size_t), ("max_active", C.c_int), ("batch_mttkrp", C.c_int),
-                ("exact_fit", C.c_int), ("blocks_per_cu", C.c_int), ("tile_scale", C.c_double)]
+                ("exact_fit", C.c_int), ("blocks_per_cu", C.c_int), ("tile_scale", C.c_double),
+                ("admit_threads", C.c_int), ("borrow_inputs", C.c_int)]
 
 
 class sptolRequestInfo(C.Structure):
@@ -248,6 +249,8 @@ def _declare(lib):
         "sptolSetVerbose": (None, [C.c_int]),
         "sptolHipSetMemoryBudget": (C.c_int, [C.c_size_t]),
         "sptolHipMemoryInUse": (C.c_size_t, []),
+        "sptolHipMemoryCached": (C.c_size_t, []),
+        "sptolHipReleaseCache": (C.c_int, []),
         "sptolHipInjectMallocFailure": (C.c_int, [C.c_int]),
         "sptolHipStreamSynchronize": (C.c_int, [C.c_void_p]),
         "sptolHipDeviceCount": (C.c_int, [P(C.c_int)]),

@@ -149,3 +149,30 @@ def test_cpd_rejects_bad_rank(gpu_lib):
     init = gen.uniform_factors(d, 65, seed=2)
     with pytest.raises(RuntimeError):
         _device_cpd(gpu_lib, d, inds, vals, init, 2, -1.0, SPTOL_F32)
+
+
+def test_block_cache_reuse(gpu_lib):
+    """The workspace of a device CPD goes to the block cache, not back to HIP:
+    memory in use returns to where it was, a second identical solve reuses
+    the cached blocks and gives bit-identical results, and releasing the
+    cache empties it."""
+    lib = gpu_lib
+    lib.sptolHipReleaseCache()
+    assert lib.sptolHipMemoryCached() == 0
+    base = lib.sptolHipMemoryInUse()
+    d, inds, vals, _ = gen.planted_tensor([300, 250, 200, 40], 60000, 16, 1.05, seed=21)
+    init = gen.uniform_factors(d, 16, seed=22)
+    A1, lam1, st1 = _device_cpd(lib, d, inds, vals, init, 6, -1.0, SPTOL_F32)
+    assert lib.sptolHipMemoryInUse() == base
+    cached = lib.sptolHipMemoryCached()
+    _say("cached after one solve", cached)
+    assert cached > 0
+    A2, lam2, st2 = _device_cpd(lib, d, inds, vals, init, 6, -1.0, SPTOL_F32)
+    assert lib.sptolHipMemoryInUse() == base
+    assert lib.sptolHipMemoryCached() <= cached + (1 << 20)
+    assert st1.fit == st2.fit
+    assert np.array_equal(lam1.view(np.uint32), lam2.view(np.uint32))
+    for x, y in zip(A1, A2):
+        assert np.array_equal(x.view(np.uint32), y.view(np.uint32))
+    lib.sptolHipReleaseCache()
+    assert lib.sptolHipMemoryCached() == 0

@@ -17,7 +17,9 @@
 * Failure isolation: a request over the HBM budget, one with an index >= ndims
   and one hit by an injected allocation failure each fail with their own
   error code while the others complete correctly; a non-finite value is
-  rejected at submission."""
+  rejected at submission, or, when the server borrows the caller's arrays
+  instead of copying them, fails its request at admission.
+* A borrowing server returns the same results as a copying one."""
 import ctypes as C
 
 import numpy as np
@@ -37,12 +39,13 @@ def _say(*a):
     print(*a, flush=True)
 
 
-def _server(lib, budget=64 << 30, slots=16, batch=1):
+def _server(lib, budget=64 << 30, slots=16, batch=1, borrow=0):
     cfg = sptolServerConfig()
     cfg.hbm_budget = budget
     cfg.max_active = slots
     cfg.batch_mttkrp = batch
     cfg.exact_fit = 1
+    cfg.borrow_inputs = borrow
     s = C.c_void_p()
     check(lib, lib.sptolServerCreate(C.byref(s), C.byref(cfg)), "ServerCreate")
     return s
@@ -85,10 +88,12 @@ def _requests():
     return reqs
 
 
-def _submit(lib, srv, r):
+def _submit(lib, srv, r, keep=None):
     coo = CooTensor(r["d"], r["inds"], r["vals"])
     rid = C.c_uint64()
     rc = lib.sptolServerSubmit(srv, coo.ref(), r["R"], r["iters"], r["tol"], r["seed"], r["dtype"], C.byref(rid))
+    if keep is not None:
+        keep.append(coo)   # a borrowing server reads these arrays until the request is waited for
     return rc, rid.value
 
 
@@ -224,3 +229,35 @@ def test_failure_isolation(gpu_lib):
     assert r_after["info"].status == SPTOL_REQ_DONE
     _close(r_ok, r_after, False)
     lib.sptolServerDestroy(srv)
+
+
+def test_borrowed_inputs(gpu_lib):
+    lib = gpu_lib
+    reqs = _requests()[:4] + _well_posed()[:2]
+    out = {}
+    for borrow in (0, 1):
+        _say("borrow", borrow)
+        srv = _server(lib, slots=4, borrow=borrow)
+        keep = []
+        ids = []
+        for r in reqs:
+            rc, rid = _submit(lib, srv, r, keep)
+            check(lib, rc, "Submit")
+            ids.append(rid)
+        nan = dict(reqs[0])
+        nv = nan["vals"].copy()
+        nv[7] = np.inf
+        nan["vals"] = nv
+        rc, rid_nan = _submit(lib, srv, nan, keep)
+        if borrow:
+            check(lib, rc, "Submit")
+            r_nan = _result(lib, srv, rid_nan, nan)
+            assert r_nan["info"].status == SPTOL_REQ_FAILED and r_nan["info"].error == SPTERR_VALUE_ERROR
+        else:
+            assert rc == SPTERR_VALUE_ERROR
+        out[borrow] = [_result(lib, srv, rid, r) for rid, r in zip(ids, reqs)]
+        lib.sptolServerDestroy(srv)
+        del keep
+    for a, b, r in zip(out[0], out[1], reqs):
+        assert a["info"].status == SPTOL_REQ_DONE and b["info"].status == SPTOL_REQ_DONE
+        _close(a, b, r["well_posed"])

@@ -208,11 +208,6 @@ __global__ __launch_bounds__(256) void k_batch_seg_carry(const SegArgs *__restri
 SegBatchDev::~SegBatchDev()
 {
     dfree(arena); dfree(carry); dfree(carry_row); dfree(counters);
-    for (int k = 0; k < 2; ++k) {
-        if (stage_used[k]) (void)hipEventSynchronize(stage_ev[k]);
-        if (stage[k]) (void)hipHostFree(stage[k]);
-        if (stage_ev[k]) (void)hipEventDestroy(stage_ev[k]);
-    }
 }
 
 static int seg_class(uint32_t lpn, uint32_t nmodes)
@@ -296,23 +291,7 @@ static int seg_batch_prepare(const std::vector<const BatchDesc *> &items, double
         SPTOL_TRY(dmalloc((void **)&dev.arena, cap, "Batch"));
         dev.arena_cap = cap;
     }
-    const int k = dev.stage_cur;
-    dev.stage_cur ^= 1;
-    if (dev.stage_used[k]) SPTOL_HIP(hipEventSynchronize(dev.stage_ev[k]), "Batch");
-    dev.stage_used[k] = false;
-    if (dev.stage_cap[k] < host.size()) {
-        if (dev.stage[k]) (void)hipHostFree(dev.stage[k]);
-        dev.stage[k] = nullptr;
-        dev.stage_cap[k] = 0;
-        const size_t cap = host.size() * 2;
-        SPTOL_HIP(hipHostMalloc((void **)&dev.stage[k], cap, hipHostMallocDefault), "Batch");
-        dev.stage_cap[k] = cap;
-    }
-    if (!dev.stage_ev[k]) SPTOL_HIP(hipEventCreateWithFlags(&dev.stage_ev[k], hipEventDisableTiming), "Batch");
-    std::memcpy(dev.stage[k], host.data(), host.size());
-    SPTOL_HIP(hipMemcpyAsync(dev.arena, dev.stage[k], host.size(), hipMemcpyHostToDevice, s), "Batch");
-    SPTOL_HIP(hipEventRecord(dev.stage_ev[k], s), "Batch");
-    dev.stage_used[k] = true;
+    SPTOL_TRY(dev.stage.upload(dev.arena, host.data(), host.size(), s));
     return 0;
 }
 

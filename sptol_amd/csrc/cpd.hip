@@ -31,17 +31,22 @@ namespace sptol {
 // Kernels
 // ---------------------------------------------------------------------------
 
+// The kernels below are written as bodies taking the block's index and the
+// block count of its request (blk, nblk), so that the single-request
+// launches and the batched launches of the server (one grid over many
+// requests) run the same arithmetic in the same order.
+
 // partial[blk][a*R+b] = sum over this block's rows of A[i][a]*A[i][b] (FP64).
-__global__ __launch_bounds__(256) void k_gram_partial(const float *__restrict__ A, uint32_t I, uint32_t R,
-                                                      uint32_t S, double *__restrict__ partial)
+__device__ __forceinline__ void gram_partial_body(const float *__restrict__ A, uint32_t I, uint32_t R, uint32_t S,
+                                                  double *__restrict__ partial, uint32_t blk, uint32_t nblk)
 {
     __shared__ float tile[16][kMaxRank];
     const uint32_t RR = R * R;
     double acc[16];
 #pragma unroll
     for (int q = 0; q < 16; ++q) acc[q] = 0.0;
-    const uint64_t rows_per = ((uint64_t)I + gridDim.x - 1) / gridDim.x;
-    const uint64_t r0 = (uint64_t)blockIdx.x * rows_per;
+    const uint64_t rows_per = ((uint64_t)I + nblk - 1) / nblk;
+    const uint64_t r0 = (uint64_t)blk * rows_per;
     const uint64_t r1 = min<uint64_t>(r0 + rows_per, I);
     for (uint64_t base = r0; base < r1; base += 16) {
         for (uint32_t t = threadIdx.x; t < 16 * R; t += 256) {
@@ -65,8 +70,14 @@ __global__ __launch_bounds__(256) void k_gram_partial(const float *__restrict__ 
 #pragma unroll
     for (int q = 0; q < 16; ++q) {
         const uint32_t e = threadIdx.x + 256u * q;
-        if (e < RR) partial[(size_t)blockIdx.x * RR + e] = acc[q];
+        if (e < RR) partial[(size_t)blk * RR + e] = acc[q];
     }
+}
+
+__global__ __launch_bounds__(256) void k_gram_partial(const float *__restrict__ A, uint32_t I, uint32_t R,
+                                                      uint32_t S, double *__restrict__ partial)
+{
+    gram_partial_body(A, I, R, S, partial, blockIdx.x, gridDim.x);
 }
 
 // Wave-wide reduction of one value per lane in a fixed pattern.
@@ -88,10 +99,10 @@ __device__ __forceinline__ double wave_max(double v)
 
 // out[e] = sum over blk of partial[blk][e]: one wave per output, lanes
 // stride the blocks, then a fixed butterfly.  Four outputs per block.
-__global__ __launch_bounds__(256) void k_sum_partials(const double *__restrict__ partial, uint32_t nblk, uint32_t n,
-                                                      double *__restrict__ out)
+__device__ __forceinline__ void sum_partials_body(const double *__restrict__ partial, uint32_t nblk, uint32_t n,
+                                                  double *__restrict__ out, uint32_t blk)
 {
-    const uint32_t e = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63u;
+    const uint32_t e = blk * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63u;
     if (e >= n) return;   // whole wave
     double s = 0.0;
     for (uint32_t b = lane; b < nblk; b += 64) s += partial[(size_t)b * n + e];
@@ -99,18 +110,24 @@ __global__ __launch_bounds__(256) void k_sum_partials(const double *__restrict__
     if (lane == 0) out[e] = s;
 }
 
+__global__ __launch_bounds__(256) void k_sum_partials(const double *__restrict__ partial, uint32_t nblk, uint32_t n,
+                                                      double *__restrict__ out)
+{
+    sum_partials_body(partial, nblk, n, out, blockIdx.x);
+}
+
 static unsigned sum_grid(uint32_t n) { return (n + 3) / 4; }
 // Column reduction partials: kind 0 = sum of squares, 1 = max (from 0).
-__global__ __launch_bounds__(256) void k_col_partial(const float *__restrict__ A, uint32_t I, uint32_t R, uint32_t S,
-                                                     int kind, double *__restrict__ partial)
+__device__ __forceinline__ void col_partial_body(const float *__restrict__ A, uint32_t I, uint32_t R, uint32_t S,
+                                                 int kind, double *__restrict__ partial, uint32_t blk, uint32_t nblk)
 {
     __shared__ double red[256];
     const uint32_t lanes_per_col = 256 / R;          // R <= 64 so >= 4
     const uint32_t c = threadIdx.x % R;
     const uint32_t sub = threadIdx.x / R;
     const bool active = sub < lanes_per_col;
-    const uint64_t rows_per = ((uint64_t)I + gridDim.x - 1) / gridDim.x;
-    const uint64_t r0 = (uint64_t)blockIdx.x * rows_per;
+    const uint64_t rows_per = ((uint64_t)I + nblk - 1) / nblk;
+    const uint64_t r0 = (uint64_t)blk * rows_per;
     const uint64_t r1 = min<uint64_t>(r0 + rows_per, I);
     double v = 0.0;
     if (active) {
@@ -129,18 +146,25 @@ __global__ __launch_bounds__(256) void k_col_partial(const float *__restrict__ A
             if (kind == 0) s += y;
             else s = y > s ? y : s;
         }
-        partial[(size_t)blockIdx.x * R + threadIdx.x] = s;
+        partial[(size_t)blk * R + threadIdx.x] = s;
     }
+}
+
+__global__ __launch_bounds__(256) void k_col_partial(const float *__restrict__ A, uint32_t I, uint32_t R, uint32_t S,
+                                                     int kind, double *__restrict__ partial)
+{
+    col_partial_body(A, I, R, S, kind, partial, blockIdx.x, gridDim.x);
 }
 
 // lambda[c] from the partials: kind 0 -> sqrt(sum), 1 -> max(max, 1).
 // mult != 0 multiplies the existing lambda by the new norm instead
 // (GetFinalLambda); the norms are also written to norms[].  One wave per
 // column, four columns per block.
-__global__ __launch_bounds__(256) void k_col_final(const double *__restrict__ partial, uint32_t nblk, uint32_t R, int kind,
-                                                   int mult, double *__restrict__ lambda, double *__restrict__ norms)
+__device__ __forceinline__ void col_final_body(const double *__restrict__ partial, uint32_t nblk, uint32_t R, int kind,
+                                               int mult, double *__restrict__ lambda, double *__restrict__ norms,
+                                               uint32_t blk)
 {
-    const uint32_t c = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63u;
+    const uint32_t c = blk * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63u;
     if (c >= R) return;   // whole wave
     double s = 0.0;
     for (uint32_t b = lane; b < nblk; b += 64) {
@@ -156,13 +180,19 @@ __global__ __launch_bounds__(256) void k_col_final(const double *__restrict__ pa
     }
 }
 
+__global__ __launch_bounds__(256) void k_col_final(const double *__restrict__ partial, uint32_t nblk, uint32_t R, int kind,
+                                                   int mult, double *__restrict__ lambda, double *__restrict__ norms)
+{
+    col_final_body(partial, nblk, R, kind, mult, lambda, norms, blockIdx.x);
+}
+
 // A[i][c] /= norms[c] (FP64 divide, FP32 store); optional BF16 copy.
-__global__ void k_scale(float *__restrict__ A, uint16_t *__restrict__ Abf, uint32_t I, uint32_t R, uint32_t S,
-                        const double *__restrict__ norms)
+__device__ __forceinline__ void scale_body(float *__restrict__ A, uint16_t *__restrict__ Abf, uint32_t I, uint32_t R,
+                                           uint32_t S, const double *__restrict__ norms, uint32_t blk, uint32_t nblk)
 {
     const uint64_t n = (uint64_t)I * S;
-    uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const uint64_t step = (uint64_t)gridDim.x * blockDim.x;
+    uint64_t e = (uint64_t)blk * blockDim.x + threadIdx.x;
+    const uint64_t step = (uint64_t)nblk * blockDim.x;
     for (; e < n; e += step) {
         const uint32_t c = (uint32_t)(e % S);
         float x = A[e];
@@ -176,6 +206,12 @@ __global__ void k_scale(float *__restrict__ A, uint16_t *__restrict__ Abf, uint3
             Abf[e] = (uint16_t)(u >> 16);
         }
     }
+}
+
+__global__ void k_scale(float *__restrict__ A, uint16_t *__restrict__ Abf, uint32_t I, uint32_t R, uint32_t S,
+                        const double *__restrict__ norms)
+{
+    scale_body(A, Abf, I, R, S, norms, blockIdx.x, gridDim.x);
 }
 
 // V = Hadamard of ata[p] for p = (m+k) % N, k = 1..N-1; then a Cholesky
@@ -198,8 +234,8 @@ __device__ __forceinline__ void chol_wave_sync()
 }
 
 template <int RM>
-__global__ __launch_bounds__(64) void k_factor(const double *__restrict__ ata, uint32_t N, uint32_t m, uint32_t R,
-                                               double *__restrict__ fac, int *__restrict__ piv, int *__restrict__ kind)
+__device__ __forceinline__ void factor_body(const double *__restrict__ ata, uint32_t N, uint32_t m, uint32_t R,
+                                            double *__restrict__ fac, int *__restrict__ piv, int *__restrict__ kind)
 {
 #pragma clang fp contract(off)
     __shared__ double col[RM];
@@ -284,11 +320,18 @@ __global__ __launch_bounds__(64) void k_factor(const double *__restrict__ ata, u
     if (t == 0) *kind = 1;
 }
 
+template <int RM>
+__global__ __launch_bounds__(64) void k_factor(const double *__restrict__ ata, uint32_t N, uint32_t m, uint32_t R,
+                                               double *__restrict__ fac, int *__restrict__ piv, int *__restrict__ kind)
+{
+    factor_body<RM>(ata, N, m, R, fac, piv, kind);
+}
+
 // A[i] = M[i] V^{-1} for every row, one thread per row, FP64 arithmetic.
 template <int RM>
-__global__ __launch_bounds__(256) void k_solve_rows(const float *__restrict__ M, float *__restrict__ A, uint32_t I,
-                                                    uint32_t R, uint32_t S, const double *__restrict__ fac,
-                                                    const int *__restrict__ piv, const int *__restrict__ kind)
+__device__ __forceinline__ void solve_rows_body(const float *__restrict__ M, float *__restrict__ A, uint32_t I,
+                                                uint32_t R, uint32_t S, const double *__restrict__ fac,
+                                                const int *__restrict__ piv, const int *__restrict__ kind, uint32_t blk)
 {
     __shared__ double F[RM * RM];
     __shared__ int pv[RM];
@@ -296,7 +339,7 @@ __global__ __launch_bounds__(256) void k_solve_rows(const float *__restrict__ M,
     for (uint32_t e = threadIdx.x; e < R; e += blockDim.x) pv[e] = piv[e];
     __syncthreads();
     const int lu = *kind;
-    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t i = (uint64_t)blk * blockDim.x + threadIdx.x;
     if (i >= I) return;
     double x[RM];
 #pragma unroll
@@ -362,16 +405,25 @@ __global__ __launch_bounds__(256) void k_solve_rows(const float *__restrict__ M,
         if (a < (int)R) A[i * S + a] = (float)x[a];
 }
 
+template <int RM>
+__global__ __launch_bounds__(256) void k_solve_rows(const float *__restrict__ M, float *__restrict__ A, uint32_t I,
+                                                    uint32_t R, uint32_t S, const double *__restrict__ fac,
+                                                    const int *__restrict__ piv, const int *__restrict__ kind)
+{
+    solve_rows_body<RM>(M, A, I, R, S, fac, piv, kind, blockIdx.x);
+}
+
 // partial[blk][r] = sum over this block's rows of A[i][r] * M[i][r].
-__global__ __launch_bounds__(256) void k_inner_partial(const float *__restrict__ A, const float *__restrict__ M, uint32_t I,
-                                                       uint32_t R, uint32_t S, double *__restrict__ partial)
+__device__ __forceinline__ void inner_partial_body(const float *__restrict__ A, const float *__restrict__ M, uint32_t I,
+                                                   uint32_t R, uint32_t S, double *__restrict__ partial, uint32_t blk,
+                                                   uint32_t nblk)
 {
     __shared__ double red[256];
     const uint32_t lanes_per_col = 256 / R;
     const uint32_t c = threadIdx.x % R;
     const uint32_t sub = threadIdx.x / R;
-    const uint64_t rows_per = ((uint64_t)I + gridDim.x - 1) / gridDim.x;
-    const uint64_t r0 = (uint64_t)blockIdx.x * rows_per;
+    const uint64_t rows_per = ((uint64_t)I + nblk - 1) / nblk;
+    const uint64_t r0 = (uint64_t)blk * rows_per;
     const uint64_t r1 = min<uint64_t>(r0 + rows_per, I);
     double v = 0.0;
     if (sub < lanes_per_col)
@@ -381,8 +433,14 @@ __global__ __launch_bounds__(256) void k_inner_partial(const float *__restrict__
     if (threadIdx.x < R) {
         double s = red[threadIdx.x];
         for (uint32_t k = 1; k < lanes_per_col; ++k) s += red[threadIdx.x + k * R];
-        partial[(size_t)blockIdx.x * R + threadIdx.x] = s;
+        partial[(size_t)blk * R + threadIdx.x] = s;
     }
+}
+
+__global__ __launch_bounds__(256) void k_inner_partial(const float *__restrict__ A, const float *__restrict__ M, uint32_t I,
+                                                       uint32_t R, uint32_t S, double *__restrict__ partial)
+{
+    inner_partial_body(A, M, I, R, S, partial, blockIdx.x, gridDim.x);
 }
 
 // <X, K> = sum_nnz x * sum_r lambda_r prod_m A_m[i_m][r], every product and
@@ -477,10 +535,10 @@ __global__ __launch_bounds__(256) void k_sqnorm_partial(const float *__restrict_
 // fit from ||X||^2, the Grams, lambda and the inner-product partials.
 // inner_scalar = 1: inner_partial holds nblk scalar partials of <X,K>
 // (exact pass); 0: nblk x R partials of sum_i A_last[i][r] M[i][r].
-__global__ __launch_bounds__(256) void k_fit(const double *__restrict__ ata, uint32_t N, uint32_t R,
-                                             const double *__restrict__ lambda, const double *__restrict__ inner_partial,
-                                             uint32_t nblk, int inner_scalar, const double *__restrict__ normx,
-                                             double *__restrict__ fit)
+__device__ __forceinline__ void fit_body(const double *__restrict__ ata, uint32_t N, uint32_t R,
+                                         const double *__restrict__ lambda, const double *__restrict__ inner_partial,
+                                         uint32_t nblk, int inner_scalar, const double *__restrict__ normx,
+                                         double *__restrict__ fit)
 {
     __shared__ double red[256];
     const uint32_t RR = R * R;
@@ -523,6 +581,14 @@ __global__ __launch_bounds__(256) void k_fit(const double *__restrict__ ata, uin
     }
 }
 
+__global__ __launch_bounds__(256) void k_fit(const double *__restrict__ ata, uint32_t N, uint32_t R,
+                                             const double *__restrict__ lambda, const double *__restrict__ inner_partial,
+                                             uint32_t nblk, int inner_scalar, const double *__restrict__ normx,
+                                             double *__restrict__ fit)
+{
+    fit_body(ata, N, R, lambda, inner_partial, nblk, inner_scalar, normx, fit);
+}
+
 __global__ void k_set(double *p, uint32_t n, double v)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -545,6 +611,102 @@ static unsigned grid_rows(uint64_t n, uint32_t per_block)
     if (b < 1) b = 1;
     if (b > 65535) b = 65535;
     return (unsigned)b;
+}
+
+// Row solves: one thread per row, 256 rows per block.
+static unsigned solve_blocks(uint32_t I) { return (unsigned)(((uint64_t)I + 255) / 256); }
+
+// ---------------------------------------------------------------------------
+// Batched post-MTTKRP step: one grid per phase over many requests.  A block
+// finds its request by binary search over the requests' first blocks of the
+// phase and runs the same body as the single-request kernel with its block
+// index within the request.
+// ---------------------------------------------------------------------------
+enum { kPhSolve = 0, kPhColPartial, kPhColFinal, kPhScale, kPhGramPartial, kPhGramSum, kPhInner, kPhases };
+
+struct AlsStepDesc {
+    const float *M;            // MTTKRP output, I x S
+    float *A;                  // factor of the mode, I x S
+    uint16_t *Abf;             // its BF16 copy, or null
+    double *ata;               // N x R x R Grams
+    double *partial;
+    double *fac;
+    int *piv, *kind;
+    double *lambda, *norms;
+    const double *normx;
+    double *fit;
+    uint32_t I, R, S, N, m, norm_kind, end_iter, pad;
+    uint32_t nb_col, nb_scale, nb_gram, nb_inner;
+    uint32_t blk[kPhases];     // first block in each phase's grid (solve: within the rank class)
+    uint32_t pad2;
+};
+
+__device__ __forceinline__ const AlsStepDesc &als_find(const AlsStepDesc *__restrict__ d, uint32_t n, int ph,
+                                                       uint32_t b)
+{
+    uint32_t lo = 0, hi = n;   // last desc with blk[ph] <= b
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (d[mid].blk[ph] <= b) lo = mid; else hi = mid;
+    }
+    return d[lo];
+}
+
+template <int RM>
+__global__ __launch_bounds__(64) void k_factor_b(const AlsStepDesc *__restrict__ d)
+{
+    const AlsStepDesc &q = d[blockIdx.x];
+    factor_body<RM>(q.ata, q.N, q.m, q.R, q.fac, q.piv, q.kind);
+}
+
+template <int RM>
+__global__ __launch_bounds__(256) void k_solve_b(const AlsStepDesc *__restrict__ d, uint32_t n)
+{
+    const AlsStepDesc &q = als_find(d, n, kPhSolve, blockIdx.x);
+    solve_rows_body<RM>(q.M, q.A, q.I, q.R, q.S, q.fac, q.piv, q.kind, blockIdx.x - q.blk[kPhSolve]);
+}
+
+__global__ __launch_bounds__(256) void k_col_partial_b(const AlsStepDesc *__restrict__ d, uint32_t n)
+{
+    const AlsStepDesc &q = als_find(d, n, kPhColPartial, blockIdx.x);
+    col_partial_body(q.A, q.I, q.R, q.S, (int)q.norm_kind, q.partial, blockIdx.x - q.blk[kPhColPartial], q.nb_col);
+}
+
+__global__ __launch_bounds__(256) void k_col_final_b(const AlsStepDesc *__restrict__ d, uint32_t n)
+{
+    const AlsStepDesc &q = als_find(d, n, kPhColFinal, blockIdx.x);
+    col_final_body(q.partial, q.nb_col, q.R, (int)q.norm_kind, 0, q.lambda, q.norms, blockIdx.x - q.blk[kPhColFinal]);
+}
+
+__global__ __launch_bounds__(256) void k_scale_b(const AlsStepDesc *__restrict__ d, uint32_t n)
+{
+    const AlsStepDesc &q = als_find(d, n, kPhScale, blockIdx.x);
+    scale_body(q.A, q.Abf, q.I, q.R, q.S, q.norms, blockIdx.x - q.blk[kPhScale], q.nb_scale);
+}
+
+__global__ __launch_bounds__(256) void k_gram_partial_b(const AlsStepDesc *__restrict__ d, uint32_t n)
+{
+    const AlsStepDesc &q = als_find(d, n, kPhGramPartial, blockIdx.x);
+    gram_partial_body(q.A, q.I, q.R, q.S, q.partial, blockIdx.x - q.blk[kPhGramPartial], q.nb_gram);
+}
+
+__global__ __launch_bounds__(256) void k_gram_sum_b(const AlsStepDesc *__restrict__ d, uint32_t n)
+{
+    const AlsStepDesc &q = als_find(d, n, kPhGramSum, blockIdx.x);
+    sum_partials_body(q.partial, q.nb_gram, q.R * q.R, q.ata + (size_t)q.m * q.R * q.R, blockIdx.x - q.blk[kPhGramSum]);
+}
+
+__global__ __launch_bounds__(256) void k_inner_b(const AlsStepDesc *__restrict__ d, uint32_t n)
+{
+    const AlsStepDesc &q = als_find(d, n, kPhInner, blockIdx.x);
+    inner_partial_body(q.A, q.M, q.I, q.R, q.S, q.partial, blockIdx.x - q.blk[kPhInner], q.nb_inner);
+}
+
+__global__ __launch_bounds__(256) void k_fit_b(const AlsStepDesc *__restrict__ d)
+{
+    const AlsStepDesc &q = d[blockIdx.x];
+    if (!q.end_iter) return;   // whole block
+    fit_body(q.ata, q.N, q.R, q.lambda, q.partial, q.nb_inner, 0, q.normx, q.fit);
 }
 
 // ---------------------------------------------------------------------------
@@ -591,13 +753,113 @@ static int solve(const float *M, float *A, uint32_t I, uint32_t R, uint32_t S, C
     else if (R <= 16) hipLaunchKernelGGL(k_factor<16>, dim3(1), dim3(64), 0, s, w.ata, N, m, R, w.fac, w.piv, w.kind);
     else if (R <= 32) hipLaunchKernelGGL(k_factor<32>, dim3(1), dim3(64), 0, s, w.ata, N, m, R, w.fac, w.piv, w.kind);
     else hipLaunchKernelGGL(k_factor<64>, dim3(1), dim3(64), 0, s, w.ata, N, m, R, w.fac, w.piv, w.kind);
-    const unsigned g = grid_rows(I, 256);
+    const unsigned g = solve_blocks(I);
     if (R <= 8) hipLaunchKernelGGL(k_solve_rows<8>, dim3(g), dim3(256), 0, s, M, A, I, R, S, w.fac, w.piv, w.kind);
     else if (R <= 16) hipLaunchKernelGGL(k_solve_rows<16>, dim3(g), dim3(256), 0, s, M, A, I, R, S, w.fac, w.piv, w.kind);
     else if (R <= 32) hipLaunchKernelGGL(k_solve_rows<32>, dim3(g), dim3(256), 0, s, M, A, I, R, S, w.fac, w.piv, w.kind);
     else hipLaunchKernelGGL(k_solve_rows<64>, dim3(g), dim3(256), 0, s, M, A, I, R, S, w.fac, w.piv, w.kind);
     prof_stop(tok, s);
     SPTOL_HIP(hipGetLastError(), "CPD solve");
+    return 0;
+}
+
+AlsBatchBuffers::~AlsBatchBuffers() { dfree(dev); }
+
+static int rank_class(uint32_t R) { return R <= 8 ? 0 : (R <= 16 ? 1 : (R <= 32 ? 2 : 3)); }
+
+int als_step_batch(const std::vector<CpdJob *> &jobs, AlsBatchBuffers &bb, hipStream_t s)
+{
+    if (jobs.empty()) return 0;
+    StreamScope scope(s);   // the table buffer is used on s only
+    std::vector<AlsStepDesc> d;
+    d.reserve(jobs.size());
+    uint32_t c0[5] = {};
+    for (int c = 0; c < 4; ++c) {
+        c0[c] = (uint32_t)d.size();
+        uint32_t sb = 0;
+        for (CpdJob *j : jobs) {
+            if (rank_class(j->R) != c) continue;
+            AlsStepDesc q;
+            std::memset(&q, 0, sizeof(q));
+            const uint32_t m = j->m, I = j->X->ndims[m];
+            q.M = j->w.M;
+            q.A = (float *)j->mats[m].values;
+            q.Abf = j->fdt == SPTOL_BF16 ? j->w.bf[m] : nullptr;
+            q.ata = j->w.ata;
+            q.partial = j->w.partial;
+            q.fac = j->w.fac; q.piv = j->w.piv; q.kind = j->w.kind;
+            q.lambda = j->w.lambda; q.norms = j->w.norms;
+            q.normx = j->w.normx; q.fit = j->w.fit;
+            q.I = I; q.R = j->R; q.S = j->S; q.N = j->N; q.m = m;
+            q.norm_kind = j->it == 0 ? 0 : 1;
+            q.end_iter = m + 1 == j->N ? 1 : 0;
+            q.nb_col = red_blocks(I, 64);
+            q.nb_scale = grid_rows((uint64_t)I * j->S, 256);
+            q.nb_gram = red_blocks(I, 32);
+            q.nb_inner = q.end_iter ? red_blocks(I, 256) : 0;
+            q.blk[kPhSolve] = sb;
+            sb += solve_blocks(I);
+            d.push_back(q);
+        }
+    }
+    c0[4] = (uint32_t)d.size();
+    const uint32_t n = (uint32_t)d.size();
+    uint64_t tot[kPhases] = {};
+    for (AlsStepDesc &q : d) {
+        const uint32_t nb[kPhases] = {0, q.nb_col, sum_grid(q.R), q.nb_scale, q.nb_gram, sum_grid(q.R * q.R),
+                                      q.nb_inner};
+        for (int p = kPhColPartial; p < kPhases; ++p) {
+            q.blk[p] = (uint32_t)tot[p];
+            tot[p] += nb[p];
+        }
+    }
+    for (int p = 0; p < kPhases; ++p)
+        if (tot[p] > 0x7FFFFFFFull) SPTOL_CHECK(SPTERR_VALUE_ERROR, "CPD batch", "too many blocks in one step");
+    if (bb.cap < n) {
+        dfree(bb.dev);
+        bb.dev = nullptr;
+        bb.cap = 0;
+        SPTOL_TRY(dmalloc((void **)&bb.dev, (size_t)n * 2 * sizeof(AlsStepDesc), "CPD batch"));
+        bb.cap = (size_t)n * 2;
+    }
+    SPTOL_TRY(bb.stage.upload(bb.dev, d.data(), (size_t)n * sizeof(AlsStepDesc), s));
+    int tok = prof_start("cpd_batch_step", s);
+    for (int c = 0; c < 4; ++c) {
+        const uint32_t k = c0[c], nc = c0[c + 1] - c0[c];
+        if (!nc) continue;
+        const AlsStepDesc *dc = bb.dev + k;
+        switch (c) {
+        case 0: hipLaunchKernelGGL(k_factor_b<8>, dim3(nc), dim3(64), 0, s, dc); break;
+        case 1: hipLaunchKernelGGL(k_factor_b<16>, dim3(nc), dim3(64), 0, s, dc); break;
+        case 2: hipLaunchKernelGGL(k_factor_b<32>, dim3(nc), dim3(64), 0, s, dc); break;
+        default: hipLaunchKernelGGL(k_factor_b<64>, dim3(nc), dim3(64), 0, s, dc); break;
+        }
+    }
+    for (int c = 0; c < 4; ++c) {
+        const uint32_t k = c0[c], nc = c0[c + 1] - c0[c];
+        if (!nc) continue;
+        const AlsStepDesc &last = d[c0[c + 1] - 1];
+        const unsigned g = last.blk[kPhSolve] + solve_blocks(last.I);
+        const AlsStepDesc *dc = bb.dev + k;
+        switch (c) {
+        case 0: hipLaunchKernelGGL(k_solve_b<8>, dim3(g), dim3(256), 0, s, dc, nc); break;
+        case 1: hipLaunchKernelGGL(k_solve_b<16>, dim3(g), dim3(256), 0, s, dc, nc); break;
+        case 2: hipLaunchKernelGGL(k_solve_b<32>, dim3(g), dim3(256), 0, s, dc, nc); break;
+        default: hipLaunchKernelGGL(k_solve_b<64>, dim3(g), dim3(256), 0, s, dc, nc); break;
+        }
+    }
+    hipLaunchKernelGGL(k_col_partial_b, dim3((unsigned)tot[kPhColPartial]), dim3(256), 0, s, bb.dev, n);
+    hipLaunchKernelGGL(k_col_final_b, dim3((unsigned)tot[kPhColFinal]), dim3(256), 0, s, bb.dev, n);
+    hipLaunchKernelGGL(k_scale_b, dim3((unsigned)tot[kPhScale]), dim3(256), 0, s, bb.dev, n);
+    hipLaunchKernelGGL(k_gram_partial_b, dim3((unsigned)tot[kPhGramPartial]), dim3(256), 0, s, bb.dev, n);
+    hipLaunchKernelGGL(k_gram_sum_b, dim3((unsigned)tot[kPhGramSum]), dim3(256), 0, s, bb.dev, n);
+    if (tot[kPhInner]) {
+        hipLaunchKernelGGL(k_inner_b, dim3((unsigned)tot[kPhInner]), dim3(256), 0, s, bb.dev, n);
+        hipLaunchKernelGGL(k_fit_b, dim3(n), dim3(256), 0, s, bb.dev);
+    }
+    prof_stop(tok, s);
+    SPTOL_HIP(hipGetLastError(), "CPD batch");
+    for (CpdJob *j : jobs) SPTOL_TRY(j->advance(s));
     return 0;
 }
 
@@ -701,6 +963,14 @@ int CpdJob::after_mttkrp(hipStream_t s)
                        X->ndims[last], R, S, w.partial);
     hipLaunchKernelGGL(k_fit, dim3(1), dim3(256), 0, s, w.ata, N, R, w.lambda, w.partial, nfit, 0, w.normx, w.fit);
     SPTOL_HIP(hipGetLastError(), "CPD fit");
+    SPTOL_HIP(hipMemcpyAsync(h_fit, w.fit, sizeof(double), hipMemcpyDeviceToHost, s), "CPD fit");
+    fit_pending = true;
+    return 0;
+}
+
+int CpdJob::advance(hipStream_t s)
+{
+    if (++m < N) return 0;
     SPTOL_HIP(hipMemcpyAsync(h_fit, w.fit, sizeof(double), hipMemcpyDeviceToHost, s), "CPD fit");
     fit_pending = true;
     return 0;

@@ -3,6 +3,7 @@
 #pragma once
 #include "dev_common.hpp"
 #include <chrono>
+#include <vector>
 
 namespace sptol {
 
@@ -22,6 +23,16 @@ struct CpdWork {
     double *normx = nullptr;       // ||X||^2
     double *fit = nullptr;         // last fit
     ~CpdWork();
+};
+
+struct AlsStepDesc;
+
+// Device table of a batched post-MTTKRP step (reused from step to step).
+struct AlsBatchBuffers {
+    AlsStepDesc *dev = nullptr;
+    size_t cap = 0;               // descriptors
+    PinnedStage stage;
+    ~AlsBatchBuffers();
 };
 
 struct CpdJob {
@@ -54,6 +65,10 @@ struct CpdJob {
     // end of an iteration queue the fit and its copy to the host and set
     // fit_pending (call fit_ready() after the stream synchronises).
     int after_mttkrp(hipStream_t s);
+    // The tail of after_mttkrp once the step's kernels are queued (by
+    // als_step_batch): advance the mode and, at the end of an iteration,
+    // queue the fit's copy to the host and set fit_pending.
+    int advance(hipStream_t s);
     void fit_ready();
     // Final weights (and the exact fit when exact_fit): finish_enqueue
     // queues the kernels and the copies to the pinned slot; finish_read
@@ -62,5 +77,11 @@ struct CpdJob {
     void finish_read(sptValue *lambda_out);
     int finish(hipStream_t s, sptValue *lambda_out);
 };
+
+// The post-MTTKRP step of every job (solve, normalise, Gram and, for jobs
+// at the end of an iteration, the fit) as one set of launches over all of
+// them, then advance() of each.  Each job's arithmetic, block partition and
+// summation order equal its after_mttkrp, so results are bitwise the same.
+int als_step_batch(const std::vector<CpdJob *> &jobs, AlsBatchBuffers &bb, hipStream_t s);
 
 }  // namespace sptol

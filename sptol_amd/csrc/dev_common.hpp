@@ -46,6 +46,23 @@ constexpr size_t kPinnedSlotBytes = 1024;
 int pinned_slot(void **p);
 void pinned_slot_free(void *p);
 
+// Uploads of small per-launch tables through two pinned host buffers used
+// in turn: each is reused once the copy that read it has run, so the host
+// can queue the next launch without waiting for the device to drain.
+struct PinnedStage {
+    uint8_t *buf[2] = {};
+    size_t cap[2] = {};
+    hipEvent_t ev[2] = {};
+    bool used[2] = {};
+    int cur = 0;
+    // dev[0..n) = host[0..n), stream-ordered on s
+    int upload(void *dev, const void *host, size_t n, hipStream_t s);
+    PinnedStage() = default;
+    PinnedStage(const PinnedStage &) = delete;
+    PinnedStage &operator=(const PinnedStage &) = delete;
+    ~PinnedStage();
+};
+
 struct StreamScope {
     explicit StreamScope(hipStream_t s);
     ~StreamScope();
@@ -201,14 +218,7 @@ struct SegBatchDev {
     size_t off_items[8] = {}, off_prefix[8] = {}, off_all = 0, off_cprefix = 0;
     uint32_t n[8] = {}, n_all = 0;
     uint64_t total[8] = {}, carry_blocks = 0;
-    // pinned staging of the tables: two buffers used in turn, each reused
-    // once the copy that read it has run, so the host never waits for the
-    // device to drain before queuing the next launch
-    uint8_t *stage[2] = {};
-    size_t stage_cap[2] = {};
-    hipEvent_t stage_ev[2] = {};
-    bool stage_used[2] = {};
-    int stage_cur = 0;
+    PinnedStage stage;
     ~SegBatchDev();
 };
 

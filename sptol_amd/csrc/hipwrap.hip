@@ -154,6 +154,38 @@ void pinned_slot_free(void *p)
     g_pin_free.push_back(p);
 }
 
+int PinnedStage::upload(void *dev, const void *host, size_t n, hipStream_t s)
+{
+    if (!n) return 0;
+    const int k = cur;
+    cur ^= 1;
+    if (used[k]) SPTOL_HIP(hipEventSynchronize(ev[k]), "Pinned stage");
+    used[k] = false;
+    if (cap[k] < n) {
+        if (buf[k]) (void)hipHostFree(buf[k]);
+        buf[k] = nullptr;
+        cap[k] = 0;
+        const size_t c = n < 65536 ? 65536 : n * 2;
+        SPTOL_HIP(hipHostMalloc((void **)&buf[k], c, hipHostMallocDefault), "Pinned stage");
+        cap[k] = c;
+    }
+    if (!ev[k]) SPTOL_HIP(hipEventCreateWithFlags(&ev[k], hipEventDisableTiming), "Pinned stage");
+    std::memcpy(buf[k], host, n);
+    SPTOL_HIP(hipMemcpyAsync(dev, buf[k], n, hipMemcpyHostToDevice, s), "Pinned stage");
+    SPTOL_HIP(hipEventRecord(ev[k], s), "Pinned stage");
+    used[k] = true;
+    return 0;
+}
+
+PinnedStage::~PinnedStage()
+{
+    for (int k = 0; k < 2; ++k) {
+        if (used[k]) (void)hipEventSynchronize(ev[k]);
+        if (buf[k]) (void)hipHostFree(buf[k]);
+        if (ev[k]) (void)hipEventDestroy(ev[k]);
+    }
+}
+
 int dmalloc(void **ptr, size_t bytes, const char *module)
 {
     *ptr = nullptr;

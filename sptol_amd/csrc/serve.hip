@@ -11,8 +11,9 @@
 //     and preparation overlap with each other and with compute;
 //   * a compute thread advances every active request by one ALS mode-step
 //     per tick: the MTTKRPs of all of them run as one persistent batched
-//     launch, followed by each request's FP64 solve / normalise / Gram
-//     kernels; at the end of an iteration the request's fit is copied back
+//     launch, and their FP64 solve / normalise / Gram (and fit) steps as
+//     one launch per phase over all of them; at the end of an iteration the
+//     request's fit is copied back
 //     behind an event, and the request sits out ticks until the event has
 //     completed, so the host never blocks the device to read a fit;
 //   * a finished request's final kernels are queued behind an event and the
@@ -319,8 +320,10 @@ static void compute_loop(sptolServer srv)
     hipStream_t s = srv->compute_stream;
     std::vector<Request *> active;
     BatchBuffers bb;
+    AlsBatchBuffers ab;
     std::vector<BatchDesc> descs[2];
     std::vector<Request *> stepped;
+    std::vector<CpdJob *> jobs;
     for (;;) {
         {
             std::unique_lock<std::mutex> lk(srv->mu);
@@ -373,14 +376,20 @@ static void compute_loop(sptolServer srv)
         }
         int err = 0;
         if (srv->cfg.batch_mttkrp) {
+            // every stepped request's MTTKRP in one persistent launch per
+            // factor type, then their solve / normalise / Gram / fit steps
+            // as one set of launches
             StreamScope scope(s);   // batch buffers that grow were used on s only
             for (int k = 0; k < 2 && !err; ++k)
                 err = batch_launch(descs[k], k ? SPTOL_BF16 : SPTOL_F32, srv->cfg.tile_scale, srv->cfg.blocks_per_cu, bb, s);
+            jobs.clear();
+            for (auto *r : stepped) jobs.push_back(r->job);
+            if (!err) err = als_step_batch(jobs, ab, s);
         }
         for (auto *r : stepped) {
             int e = err;
             if (!e && !srv->cfg.batch_mttkrp) e = r->job->mttkrp(s);
-            if (!e) e = r->job->after_mttkrp(s);
+            if (!e && !srv->cfg.batch_mttkrp) e = r->job->after_mttkrp(s);
             if (!e && r->job->fit_pending) {
                 hipError_t he = hipEventRecord(r->fit_event, s);
                 if (he != hipSuccess) e = SPTERR_CUDA_ERROR + (int)he;

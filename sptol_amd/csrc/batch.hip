@@ -483,11 +483,11 @@ void make_desc(sptolDevCoo X, const sptolDevMatrix *mats, const uint32_t *mats_o
     d->nnz = X->nnz;
     const SortedCopy &sc = X->sorted[mode];
     for (uint32_t m = 0; m < X->nmodes; ++m) {
-        d->inds[m] = sc.ready ? sc.inds[m] : X->inds[m];
+        d->inds[m] = sc.vals ? sc.inds[m] : X->inds[m];
         d->mats[m] = mats[m].values;
         d->order[m] = mats_order[m];
     }
-    d->vals = sc.ready ? sc.vals : X->vals;
+    d->vals = sc.vals ? sc.vals : X->vals;
     d->seg = (sc.ready && sc.seg.ready && seg_ok(X->nmodes, out->stride)) ? const_cast<SegView *>(&sc.seg) : nullptr;
     d->out = (float *)out->values;
     d->out_len = (uint64_t)X->ndims[mode] * out->stride;

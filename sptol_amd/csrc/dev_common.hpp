@@ -100,8 +100,11 @@ struct SegView {
 };
 
 // Pack a mode-sorted SoA copy (inds[0..N-1], vals) into v (allocates).
+// With perm, inds and vals are in any order and nonzero perm[z] becomes
+// record z; sorted_rows[z] is then its mode index, and copy_vals must be set.
 int seg_build(SegView &v, uint32_t nmodes, uint32_t mode, uint32_t nrows, uint64_t nnz,
-              uint32_t *const *inds, const float *vals, bool copy_vals, hipStream_t s, const uint32_t *ndims);
+              uint32_t *const *inds, const float *vals, bool copy_vals, hipStream_t s, const uint32_t *ndims,
+              const uint32_t *perm = nullptr, const uint32_t *sorted_rows = nullptr);
 // Whether the segmented kernel handles this shape.
 bool seg_ok(uint32_t nmodes, uint32_t stride);
 // Lanes per nonzero (one float4 of columns each) for a row stride.

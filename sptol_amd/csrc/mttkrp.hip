@@ -531,6 +531,16 @@ int coo_prepare(sptolDevCoo h, uint32_t mode, hipStream_t s)
                                                endbit, s);
     }
     if (e != hipSuccess) { cleanup(); SPTOL_HIP(e, "DevCoo Prepare"); }
+    if (h->nmodes == 3 || h->nmodes == 4) {
+        // 3- and 4-mode tensors: the packed records are gathered straight
+        // through the permutation; no sorted SoA copy is kept (the atomic
+        // fallback kernel reads the tensor in its own order)
+        r = seg_build(sc.seg, h->nmodes, mode, h->ndims[mode], n, h->inds, h->vals, true, s, h->ndims, perm2, keys2);
+        cleanup();
+        if (r) return r;
+        sc.ready = true;
+        return 0;
+    }
     for (uint32_t m = 0; m < h->nmodes && !r; ++m) {
         r = dmalloc((void **)&sc.inds[m], n * 4, "DevCoo Prepare");
         if (!r) hipLaunchKernelGGL(k_gather<uint32_t>, dim3(grid_for(n)), dim3(256), 0, s, sc.inds[m], h->inds[m], perm2, n);
@@ -539,8 +549,6 @@ int coo_prepare(sptolDevCoo h, uint32_t mode, hipStream_t s)
     if (!r) hipLaunchKernelGGL(k_gather<float>, dim3(grid_for(n)), dim3(256), 0, s, sc.vals, h->vals, perm2, n);
     hipError_t e2 = hipStreamSynchronize(s);
     cleanup();
-    if (!r && e2 == hipSuccess && (h->nmodes == 3 || h->nmodes == 4))
-        r = seg_build(sc.seg, h->nmodes, mode, h->ndims[mode], n, sc.inds, sc.vals, false, s, h->ndims);
     if (r) {
         for (uint32_t m = 0; m < h->nmodes; ++m) { dfree(sc.inds[m]); sc.inds[m] = nullptr; }
         dfree(sc.vals); sc.vals = nullptr;
@@ -618,10 +626,10 @@ int sptol::coo_mttkrp(sptolDevCoo X, const sptolDevMatrix *mats, const uint32_t 
     a.stride = out->stride;
     for (uint32_t m = 0; m < X->nmodes; ++m) {
         a.order[m] = mats_order[m];
-        a.inds[m] = sc.ready ? sc.inds[m] : X->inds[m];
+        a.inds[m] = sc.vals ? sc.inds[m] : X->inds[m];
         a.mats[m] = mats[m].values;
     }
-    a.vals = sc.ready ? sc.vals : X->vals;
+    a.vals = sc.vals ? sc.vals : X->vals;
     a.out = (float *)out->values;
     SPTOL_TRY(launch_fill_zero(a.out, (uint64_t)X->ndims[mode] * out->stride, s));
     return launch_mttkrp_coo(a, mats[0].dtype, s);

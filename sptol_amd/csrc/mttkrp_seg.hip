@@ -59,14 +59,17 @@ __global__ __launch_bounds__(256) void k_seg_carry(const SegArgs a, uint32_t *__
     seg_carry_slots(a.carry, a.carry_row, a.nwaves, a.out, a.S, a.lpn, k0, k1, rows, part);
 }
 
+// rec[z] (and vout[z]) from nonzero p = perm[z] (p = z without perm).
 __global__ void k_seg_pack(uint4 *__restrict__ rec, float *__restrict__ vout, const uint32_t *__restrict__ i0,
                            const uint32_t *__restrict__ i1, const uint32_t *__restrict__ i2,
-                           const uint32_t *__restrict__ i3, const float *__restrict__ vals, uint64_t nnz, uint32_t nm)
+                           const uint32_t *__restrict__ i3, const float *__restrict__ vals,
+                           const uint32_t *__restrict__ perm, uint64_t nnz, uint32_t nm)
 {
     for (uint64_t z = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; z < nnz; z += (uint64_t)gridDim.x * blockDim.x) {
-        const uint32_t w = nm == 3 ? __float_as_uint(vals[z]) : i3[z];
-        rec[z] = make_uint4(i0[z], i1[z], i2[z], w);
-        if (vout) vout[z] = vals[z];
+        const uint64_t p = perm ? (uint64_t)perm[z] : z;
+        const uint32_t w = nm == 3 ? __float_as_uint(vals[p]) : i3[p];
+        rec[z] = make_uint4(i0[p], i1[p], i2[p], w);
+        if (vout) vout[z] = vals[p];
     }
 }
 
@@ -104,9 +107,11 @@ uint32_t seg_lpn(uint32_t stride)
 }
 
 int seg_build(SegView &v, uint32_t nmodes, uint32_t mode, uint32_t nrows, uint64_t nnz, uint32_t *const *inds,
-              const float *vals, bool copy_vals, hipStream_t s, const uint32_t *ndims)
+              const float *vals, bool copy_vals, hipStream_t s, const uint32_t *ndims, const uint32_t *perm,
+              const uint32_t *sorted_rows)
 {
     v.release();
+    if (perm && (!sorted_rows || !copy_vals)) SPTOL_CHECK(SPTERR_VALUE_ERROR, "Seg view", "perm needs sorted rows and copied values");
     if (nmodes != 3 && nmodes != 4) SPTOL_CHECK(SPTERR_VALUE_ERROR, "Seg view", "nmodes must be 3 or 4");
     v.nmodes = nmodes;
     v.mode = mode;
@@ -136,8 +141,9 @@ int seg_build(SegView &v, uint32_t nmodes, uint32_t mode, uint32_t nrows, uint64
             }
         }
         hipLaunchKernelGGL(k_seg_pack, dim3(seg_grid(nnz)), dim3(256), 0, s, v.rec, vout, inds[0], inds[1], inds[2],
-                           nmodes == 4 ? inds[3] : inds[0], vals, nnz, nmodes);
-        hipLaunchKernelGGL(k_seg_touch, dim3(seg_grid(nnz)), dim3(256), 0, s, v.touched, inds[mode], nnz);
+                           nmodes == 4 ? inds[3] : inds[0], vals, perm, nnz, nmodes);
+        hipLaunchKernelGGL(k_seg_touch, dim3(seg_grid(nnz)), dim3(256), 0, s, v.touched,
+                           perm ? sorted_rows : inds[mode], nnz);
     }
     e = hipGetLastError();
     if (e == hipSuccess) e = hipStreamSynchronize(s);

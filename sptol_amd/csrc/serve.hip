@@ -91,8 +91,13 @@ static size_t estimate_bytes(const Request &r)
 {
     const uint32_t N = r.nmodes;
     const uint32_t S = ((r.R - 1) / 8 + 1) * 8;
-    size_t b = (size_t)r.nnz * (4 * N + 4) * (1 + N);   // tensor + one sorted copy per mode
-    b += (size_t)r.nnz * 16 + (1u << 20);               // sort temporaries of the preparation
+    size_t b = (size_t)r.nnz * (4 * N + 4);                    // tensor
+    if (N == 3 || N == 4)
+        b += (size_t)r.nnz * (N == 4 ? 20 : 16) * N;           // one packed mode-sorted view per mode
+    else
+        b += (size_t)r.nnz * (4 * N + 4) * N;                  // one sorted copy per mode
+    for (uint32_t m = 0; m < N; ++m) b += ((size_t)r.ndims[m] + 31) / 32 * 4;   // touched-row bitmaps
+    b += (size_t)r.nnz * 16 + (1u << 20);                      // sort temporaries of the preparation
     for (uint32_t m = 0; m < N; ++m) b += (size_t)r.ndims[m] * S * 4;
     b += CpdJob::workspace_bytes(r.ndims, N, r.R, r.fdt);
     return b;

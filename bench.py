@@ -32,6 +32,9 @@ from sptol_amd.parti import (CooTensor, Matrix, SPTOL_BF16, SPTOL_F32, check, lo
 
 HBM_PEAK = 8.0e12
 DIMS = [12_000_000, 9_000_000, 28_000_000]
+HICOO_NNZ = 100_000_000
+# counter summary of k_mttkrp_seg on the full-size HiCOO workload
+TRAFFIC_PROFILE = "k_mttkrp_seg_hicoo_12Mx9Mx28M_100Mnnz_R32"
 
 
 def log(*a):
@@ -129,6 +132,22 @@ def cpu_baseline(lib, coo_inds, vals, dims, factors, order_of, budget_s=20.0):
 KERNEL_SYMBOL = {"mttkrp_seg": "k_mttkrp_seg", "mttkrp_hicoo": "k_mttkrp_hicoo_tiled", "mttkrp_coo": "k_mttkrp_coo"}
 
 
+def profile_traffic(name):
+    """HBM bytes per launch measured by rocprofv3 FETCH_SIZE / WRITE_SIZE runs
+    (scaled by the copy calibration), read from the committed summary
+    profiles/<name>.md (its `traffic_bytes_per_launch:` line); None when the
+    summary is absent."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", name + ".md")
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return int(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
 def alg_bytes(kname, nnz, dims, U, mode, R, nb=0, nk=0, factor_bytes=4):
     """Algorithmic HBM bytes of one MTTKRP launch on `mode`: the nonzero
     stream, one read of each distinct factor row gathered (U[m] rows of R
@@ -183,6 +202,7 @@ def timed_steps(lib, step, kname, steps, warmup, extra=("fill_zero", "mttkrp_seg
 
 def run_hicoo(args, lib):
     nnz = int(args.nnz)
+    full = nnz == HICOO_NNZ
     R = 32
     t0 = time.time()
     dims, inds, vals = gen.zipf_tensor(DIMS, nnz, 1.1, seed=20260101, values="lognormal",
@@ -248,7 +268,9 @@ def run_hicoo(args, lib):
                    "values": "lognormal(0,1)", "R": R, "sb_bits": 7, "sk_bits": 14,
                    "nb": nb, "nk": nk, "distinct_per_mode": U},
         "roofline": {"bound": "hbm", "achieved": round(achieved / 1e9, 2), "peak": HBM_PEAK / 1e9,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 4),
+                     "traffic": profile_traffic(TRAFFIC_PROFILE) if full else None,
+                     "traffic_source": f"profiles/{TRAFFIC_PROFILE}.md" if full else None,
                      "kernel": KERNEL_SYMBOL[kname], "kernel_avg_ms": round(k_ms, 4),
                      "algorithmic_bytes_per_launch": int(B_avg),
                      "carry_kernel_avg_ms": round(t["mttkrp_seg_carry_ms"], 4),

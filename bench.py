@@ -7,10 +7,15 @@ index drawn from a bounded Zipf(1.1) over randomly permuted ids, values
 lognormal(0,1) float32, HiCOO block 2^7 and superblock 2^14, float32
 factors U[0,1) with R=32.  One step = one MTTKRP on each of the 3 modes
 (mats_order = [n, n+1, n+2] mod 3), tensor, factors and outputs resident in
-HBM.  The same tensor in COO form (one mode-sorted copy per mode) is timed
-beside it.
+HBM.  The same tensor in COO form (one mode-sorted view per mode) is timed
+beside it.  Other workloads: --workload coo | cpd | batch | serve | copy.
 
-Prints one JSON line (see DESIGN.md for the field definitions).
+Prints one JSON line: metric, value, unit, timing fields, the workload's
+config, `roofline` for the dominant kernel (algorithmic bytes per launch
+over its average duration measured with HIP events on its launch stream,
+against 8 TB/s; `traffic` = measured HBM bytes per launch from the committed
+rocprofv3 counter summary) and `cpu_baseline` (the compiled reference or
+the oracle on a bounded sample, on this host's cores).
 """
 from __future__ import annotations
 
@@ -658,8 +663,8 @@ def main():
     ap.add_argument("--no-coo", action="store_true", help="hicoo: skip the COO comparison")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path (see DESIGN.md, "
-              "'How the path would shard'); run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU only: the library has no multi-GPU path (sharding is a "
+              "design note, not code); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     lib = load_library()
     n = C.c_int(0)

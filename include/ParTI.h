@@ -198,6 +198,7 @@ int  sptDumpMatrix(sptMatrix *mtx, FILE *fp);                                   
 int  sptNewRankMatrix(sptRankMatrix *mtx, sptIndex const nrows, sptElementIndex const ncols); /* rankmatrix.c:38 */
 int  sptConstantRankMatrix(sptRankMatrix *mtx, sptValue const val);                         /* rankmatrix.c:87 */
 void sptFreeRankMatrix(sptRankMatrix *mtx);                                              /* rankmatrix.c:127 */
+int  sptRandomizeRankMatrix(sptRankMatrix *mtx, sptIndex const nrows, sptElementIndex const ncols); /* rankmatrix.c:70 */
 int  sptMatrix2Norm(sptMatrix * const A, sptValue * const lambda);                      /* matrix.c:390 */
 int  sptMatrixMaxNorm(sptMatrix * const A, sptValue * const lambda);                    /* matrix.c:473 */
 
@@ -211,6 +212,17 @@ double SparseTensorFrobeniusNormSquared(sptSparseTensor const * const spten);   
 int    sptLoadSparseTensor(sptSparseTensor *tsr, sptIndex start_index, FILE *fp);            /* load.c:31 */
 int    sptDumpSparseTensor(const sptSparseTensor *tsr, sptIndex start_index, FILE *fp);      /* sptensor/dump.c:29 */
 void   sptSparseTensorSortIndex(sptSparseTensor *tsr, int force, int tk);                    /* sort.c:559 */
+/* Ingest additions (no counterpart in the reference, which keeps duplicated
+ * coordinates as separate nonzeros, load.c:31-86):
+ *   sptolSparseTensorMergeDuplicates: sort lexicographically and merge each
+ *     run of equal coordinates into one nonzero (FP64 sum, rounded once);
+ *   sptolSave/LoadSparseTensorBinary: a binary file of the COO arrays
+ *     ("SPTOLBT1" header, then u32 index arrays and f32 values), read or
+ *     written with one sequential transfer per array; the loader rejects
+ *     indices >= ndims. */
+int    sptolSparseTensorMergeDuplicates(sptSparseTensor *tsr, int tk);
+int    sptolSaveSparseTensorBinary(const sptSparseTensor *tsr, const char *path);
+int    sptolLoadSparseTensorBinary(sptSparseTensor *tsr, const char *path);
 void   sptSparseTensorSortIndexRowBlock(sptSparseTensor *tsr, int force,
            const sptNnzIndex begin, const sptNnzIndex end,
            const sptElementIndex sk_bits, int const tk);                                     /* sort.c:462 */
@@ -266,6 +278,9 @@ int sptOmpMTTKRPHiCOO_MatrixTiling_Scheduled_Reduce(sptSparseTensorHiCOO const *
 /* ---------------------------------------------------------------------- */
 int    sptNewKruskalTensor(sptKruskalTensor *ktsr, sptIndex nmodes, const sptIndex ndims[], sptIndex rank); /* ktensor.c:33 */
 void   sptFreeKruskalTensor(sptKruskalTensor *ktsr);                                          /* ktensor.c:79 */
+int    sptNewRankKruskalTensor(sptRankKruskalTensor *ktsr, sptIndex nmodes, const sptIndex ndims[],
+                               sptElementIndex rank);                                        /* rankktensor.c:33 */
+void   sptFreeRankKruskalTensor(sptRankKruskalTensor *ktsr);                                  /* rankktensor.c:78 */
 int    sptDumpKruskalTensor(sptKruskalTensor *ktsr, FILE *fp);                                /* ktensor/dump.c:24 */
 /* fit = 1 - ||X - K|| / ||X||, with ||K||^2 from the full symmetric Gram
  * Hadamard product (the reference reads an unwritten triangle, ktensor.c:151). */
@@ -284,6 +299,12 @@ int sptOmpCpdAls(sptSparseTensor const * const spten, sptIndex const rank, sptIn
  * Returns the last fit. */
 double CpdAlsStep(sptSparseTensor const * const spten, sptIndex const rank, sptIndex const niters,
         double const tol, sptMatrix ** mats, sptValue * const lambda);                       /* cpd.c:32 */
+/* ALS over a HiCOO tensor on the GPU (rank <= 64), as sptCpdAls; the factors
+ * are nmodes+1 sptRankMatrix structs, the last one emptied. */
+int sptCpdAlsHiCOO(sptSparseTensorHiCOO const * const hitsr, sptIndex const rank, sptIndex const niters,
+        double const tol, sptRankKruskalTensor * ktensor);                                  /* hicoo/cpd.c:149 */
+int sptOmpCpdAlsHiCOO(sptSparseTensorHiCOO const * const hitsr, sptIndex const rank, sptIndex const niters,
+        double const tol, const int tk, int balanced, sptRankKruskalTensor * ktensor);      /* hicoo/cpd_omp.c:190 */
 
 #ifdef __cplusplus
 }

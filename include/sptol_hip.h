@@ -91,6 +91,10 @@ int  sptolDevCooCreate(sptolDevCoo *h, const sptSparseTensor *X);
  * index; later MTTKRPs on that mode reduce runs of equal output rows in
  * registers instead of issuing one atomic per nonzero. */
 int  sptolDevCooPrepareMode(sptolDevCoo h, sptIndex mode);
+/* A device COO tensor read straight from a binary tensor file
+ * (sptolSaveSparseTensorBinary), streamed through pinned chunks so that the
+ * file read overlaps the copies; indices >= ndims are rejected. */
+int  sptolDevCooLoadBinary(sptolDevCoo *h, const char *path, void *stream);
 void sptolDevCooFree(sptolDevCoo h);
 sptNnzIndex sptolDevCooNnz(sptolDevCoo h);
 
@@ -191,6 +195,11 @@ typedef struct {
  * norms and fit in FP64.  tol <= 0 runs all niters. */
 int  sptolCpdAlsDevice(sptolDevCoo X, sptIndex rank, sptIndex niters, double tol,
                        sptolDevMatrix *mats, sptValue *lambda, sptolCpdStats *stats, void *stream);
+/* The same loop over a device HiCOO tensor (src/sptensor/hicoo/cpd.c:34-135):
+ * every mode's view is prepared first, the MTTKRP is the HiCOO one.  The
+ * exact fit is not available here (stats->exact_fit is ignored). */
+int  sptolCpdAlsHicooDevice(sptolDevHicoo X, sptIndex rank, sptIndex niters, double tol,
+                            sptolDevMatrix *mats, sptValue *lambda, sptolCpdStats *stats, void *stream);
 
 /* ---------------------------------------------------------------------- */
 /* Continuous-batching CPD-ALS server                                     */
@@ -235,6 +244,11 @@ int  sptolServerSubmit(sptolServer s, const sptSparseTensor *X, sptIndex rank, s
                        uint64_t seed, int factor_dtype, uint64_t *request_id);
 /* Block until the request finished (timeout_s < 0: no limit).  Returns
  * SPTERR_NO_MORE on timeout. */
+/* The same for a tensor in a binary tensor file (sptolSaveSparseTensorBinary):
+ * the header is read now, the arrays are streamed to the device at
+ * admission, so the file must stay unchanged until then. */
+int  sptolServerSubmitFile(sptolServer s, const char *path, sptIndex rank, sptIndex niters, double tol,
+                           uint64_t seed, int factor_dtype, uint64_t *request_id);
 int  sptolServerWait(sptolServer s, uint64_t request_id, double timeout_s, sptolRequestInfo *info);
 /* Copy a finished request's factors (mats[m]: ndims[m] x rank, stride as
  * sptMatrix) and lambda (rank floats); either may be NULL. */

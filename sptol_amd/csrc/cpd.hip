@@ -781,7 +781,7 @@ int als_step_batch(const std::vector<CpdJob *> &jobs, AlsBatchBuffers &bb, hipSt
             if (rank_class(j->R) != c) continue;
             AlsStepDesc q;
             std::memset(&q, 0, sizeof(q));
-            const uint32_t m = j->m, I = j->X->ndims[m];
+            const uint32_t m = j->m, I = j->ndims[m];
             q.M = j->w.M;
             q.A = (float *)j->mats[m].values;
             q.Abf = j->fdt == SPTOL_BF16 ? j->w.bf[m] : nullptr;
@@ -882,15 +882,38 @@ size_t CpdJob::workspace_bytes(const uint32_t *ndims, uint32_t N, uint32_t R, in
 int CpdJob::setup(sptolDevCoo X_, uint32_t R_, uint32_t niters_, double tol_, sptolDevMatrix *mats_, int fdt_,
                   hipStream_t s)
 {
-    X = X_; N = X->nmodes; R = R_; niters = niters_; tol = tol_; mats = mats_; fdt = fdt_;
+    X = X_;
+    H = nullptr;
+    N = X->nmodes;
+    for (uint32_t m = 0; m < N; ++m) ndims[m] = X->ndims[m];
+    nnz = X->nnz;
+    vals = X->vals;
+    return init(R_, niters_, tol_, mats_, fdt_, s);
+}
+
+int CpdJob::setup_hicoo(sptolDevHicoo H_, uint32_t R_, uint32_t niters_, double tol_, sptolDevMatrix *mats_, int fdt_,
+                        hipStream_t s)
+{
+    X = nullptr;
+    H = H_;
+    N = H->nmodes;
+    for (uint32_t m = 0; m < N; ++m) ndims[m] = H->ndims[m];
+    nnz = H->nnz;
+    vals = H->vals;
+    return init(R_, niters_, tol_, mats_, fdt_, s);
+}
+
+int CpdJob::init(uint32_t R_, uint32_t niters_, double tol_, sptolDevMatrix *mats_, int fdt_, hipStream_t s)
+{
+    R = R_; niters = niters_; tol = tol_; mats = mats_; fdt = fdt_;
     if (R < 1 || R > (uint32_t)kMaxRank) SPTOL_CHECK(SPTERR_VALUE_ERROR, "CPD", "rank must be in [1, 64]");
     if (fdt != SPTOL_F32 && fdt != SPTOL_BF16) SPTOL_CHECK(SPTERR_VALUE_ERROR, "CPD", "bad factor dtype");
     maxI = 0;
     for (uint32_t m = 0; m < N; ++m) {
-        if (mats[m].dtype != SPTOL_F32 || mats[m].ncols != R || mats[m].nrows != X->ndims[m] ||
+        if (mats[m].dtype != SPTOL_F32 || mats[m].ncols != R || mats[m].nrows != ndims[m] ||
             mats[m].stride != mats[0].stride)
             SPTOL_CHECK(SPTERR_SHAPE_MISMATCH, "CPD", "factors must be FP32 ndims[m] x rank");
-        maxI = X->ndims[m] > maxI ? X->ndims[m] : maxI;
+        maxI = ndims[m] > maxI ? ndims[m] : maxI;
     }
     S = mats[0].stride;
     SPTOL_TRY(dmalloc((void **)&w.M, (size_t)(maxI ? maxI : 1) * S * 4, "CPD"));
@@ -905,7 +928,7 @@ int CpdJob::setup(sptolDevCoo X_, uint32_t R_, uint32_t niters_, double tol_, sp
     SPTOL_TRY(dmalloc((void **)&w.fit, 8, "CPD"));
     if (fdt == SPTOL_BF16)
         for (uint32_t m = 0; m < N; ++m)
-            SPTOL_TRY(dmalloc((void **)&w.bf[m], (size_t)(X->ndims[m] ? X->ndims[m] : 1) * S * 2, "CPD"));
+            SPTOL_TRY(dmalloc((void **)&w.bf[m], (size_t)(ndims[m] ? ndims[m] : 1) * S * 2, "CPD"));
     for (uint32_t m = 0; m < N; ++m) {
         mt[m] = mats[m];
         if (fdt == SPTOL_BF16) { mt[m].dtype = SPTOL_BF16; mt[m].values = w.bf[m]; }
@@ -914,17 +937,17 @@ int CpdJob::setup(sptolDevCoo X_, uint32_t R_, uint32_t niters_, double tol_, sp
     static_assert((kMaxRank + 1) * sizeof(double) <= kPinnedSlotBytes, "fit and weights fit one pinned slot");
     SPTOL_TRY(pinned_slot((void **)&h_fit));
     // ||X||^2 and the initial Grams (cpd.c:65-70)
-    hipLaunchKernelGGL(k_sqnorm_partial, dim3(kRedBlocks), dim3(256), 0, s, X->vals, X->nnz, w.partial);
+    hipLaunchKernelGGL(k_sqnorm_partial, dim3(kRedBlocks), dim3(256), 0, s, vals, nnz, w.partial);
     hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, s, w.partial, (uint32_t)kRedBlocks, 1u, w.normx);
     // (k_sqnorm_partial runs kRedBlocks blocks over the nonzeros)
     hipLaunchKernelGGL(k_set, dim3(1), dim3(kMaxRank), 0, s, w.lambda, R, 1.0);
     SPTOL_HIP(hipGetLastError(), "CPD init");
     for (uint32_t m = 0; m < N; ++m) {
-        SPTOL_TRY(gram((const float *)mats[m].values, X->ndims[m], R, S, w.ata + (size_t)m * R * R, w.partial, s));
+        SPTOL_TRY(gram((const float *)mats[m].values, ndims[m], R, S, w.ata + (size_t)m * R * R, w.partial, s));
         if (fdt == SPTOL_BF16) {
             hipLaunchKernelGGL(k_set, dim3(1), dim3(kMaxRank), 0, s, w.norms, R, 1.0);
-            hipLaunchKernelGGL(k_scale, dim3(grid_rows((uint64_t)X->ndims[m] * S, 256)), dim3(256), 0, s,
-                               (float *)mats[m].values, w.bf[m], X->ndims[m], R, S, w.norms);
+            hipLaunchKernelGGL(k_scale, dim3(grid_rows((uint64_t)ndims[m] * S, 256)), dim3(256), 0, s,
+                               (float *)mats[m].values, w.bf[m], ndims[m], R, S, w.norms);
         }
     }
     it = 0; m = 0; fit = oldfit = 0.0; iters = 0;
@@ -944,13 +967,14 @@ int CpdJob::mttkrp(hipStream_t s)
 {
     uint32_t o[kMaxModes];
     order(o);
-    Mout.nrows = X->ndims[m];
+    Mout.nrows = ndims[m];
+    if (H) return hicoo_mttkrp(H, mt, o, m, &Mout, s);
     return coo_mttkrp(X, mt, o, m, &Mout, s);
 }
 
 int CpdJob::after_mttkrp(hipStream_t s)
 {
-    const uint32_t I = X->ndims[m];
+    const uint32_t I = ndims[m];
     SPTOL_TRY(solve(w.M, (float *)mats[m].values, I, R, S, w, N, m, s));
     SPTOL_TRY(colnorm((float *)mats[m].values, fdt == SPTOL_BF16 ? w.bf[m] : nullptr, I, R, S, it == 0 ? 0 : 1, 0,
                       w, s));
@@ -958,9 +982,9 @@ int CpdJob::after_mttkrp(hipStream_t s)
     if (++m < N) return 0;
     // end of an iteration: fit from the last mode's MTTKRP, still in w.M
     const uint32_t last = N - 1;
-    const uint32_t nfit = red_blocks(X->ndims[last], 256);
+    const uint32_t nfit = red_blocks(ndims[last], 256);
     hipLaunchKernelGGL(k_inner_partial, dim3(nfit), dim3(256), 0, s, (const float *)mats[last].values, w.M,
-                       X->ndims[last], R, S, w.partial);
+                       ndims[last], R, S, w.partial);
     hipLaunchKernelGGL(k_fit, dim3(1), dim3(256), 0, s, w.ata, N, R, w.lambda, w.partial, nfit, 0, w.normx, w.fit);
     SPTOL_HIP(hipGetLastError(), "CPD fit");
     SPTOL_HIP(hipMemcpyAsync(h_fit, w.fit, sizeof(double), hipMemcpyDeviceToHost, s), "CPD fit");
@@ -1002,7 +1026,7 @@ void CpdJob::fit_ready()
 // pinned slot.
 int CpdJob::finish_enqueue(hipStream_t s)
 {
-    if (exact_fit && N <= 4) {
+    if (exact_fit && X && N <= 4) {
         InnerArgs ia;
         std::memset(&ia, 0, sizeof(ia));
         for (uint32_t q = 0; q < N; ++q) { ia.inds[q] = X->inds[q]; ia.mats[q] = (const float *)mats[q].values; }
@@ -1023,14 +1047,14 @@ int CpdJob::finish_enqueue(hipStream_t s)
         SPTOL_HIP(hipMemcpyAsync(h_fit, w.fit, sizeof(double), hipMemcpyDeviceToHost, s), "CPD exact fit");
     }
     for (uint32_t q = 0; q < N; ++q)
-        SPTOL_TRY(colnorm((float *)mats[q].values, nullptr, X->ndims[q], R, S, 0, 1, w, s));
+        SPTOL_TRY(colnorm((float *)mats[q].values, nullptr, ndims[q], R, S, 0, 1, w, s));
     SPTOL_HIP(hipMemcpyAsync(h_fit + 1, w.lambda, (size_t)R * 8, hipMemcpyDeviceToHost, s), "CPD");
     return 0;
 }
 
 void CpdJob::finish_read(sptValue *lambda_out)
 {
-    if (exact_fit && N <= 4) fit = h_fit[0];
+    if (exact_fit && X && N <= 4) fit = h_fit[0];
     for (uint32_t r = 0; r < R; ++r) lambda_out[r] = (sptValue)h_fit[1 + r];
 }
 
@@ -1048,17 +1072,14 @@ using namespace sptol;
 
 extern "C" {
 
-int sptolCpdAlsDevice(sptolDevCoo X, sptIndex rank, sptIndex niters, double tol, sptolDevMatrix *mats,
-                      sptValue *lambda_out, sptolCpdStats *stats, void *stream)
+}  // extern "C"
+
+namespace sptol {
+
+// The ALS loop of a set-up job, the final weights, and the statistics.
+static int run_job(CpdJob &job, sptValue *lambda_out, sptolCpdStats *stats, hipStream_t s,
+                   std::chrono::steady_clock::time_point t0)
 {
-    hipStream_t s = stream_of(stream);
-    // the workspace below is freed after the final synchronisation
-    StreamScope scope(s);
-    auto t0 = std::chrono::steady_clock::now();
-    for (uint32_t m = 0; m < X->nmodes; ++m) SPTOL_TRY(coo_prepare(X, m, s));
-    CpdJob job;
-    job.exact_fit = stats ? stats->exact_fit != 0 : false;
-    SPTOL_TRY(job.setup(X, rank, niters, tol, mats, stats ? stats->factor_dtype : SPTOL_F32, s));
     while (!job.done) {
         SPTOL_TRY(job.mttkrp(s));
         SPTOL_TRY(job.after_mttkrp(s));
@@ -1074,6 +1095,36 @@ int sptolCpdAlsDevice(sptolDevCoo X, sptIndex rank, sptIndex niters, double tol,
         stats->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     }
     return 0;
+}
+
+}  // namespace sptol
+
+extern "C" {
+
+int sptolCpdAlsDevice(sptolDevCoo X, sptIndex rank, sptIndex niters, double tol, sptolDevMatrix *mats,
+                      sptValue *lambda_out, sptolCpdStats *stats, void *stream)
+{
+    hipStream_t s = stream_of(stream);
+    // the workspace below is freed after the final synchronisation
+    StreamScope scope(s);
+    auto t0 = std::chrono::steady_clock::now();
+    SPTOL_TRY(coo_prepare_all(X, s));
+    CpdJob job;
+    job.exact_fit = stats ? stats->exact_fit != 0 : false;
+    SPTOL_TRY(job.setup(X, rank, niters, tol, mats, stats ? stats->factor_dtype : SPTOL_F32, s));
+    return run_job(job, lambda_out, stats, s, t0);
+}
+
+int sptolCpdAlsHicooDevice(sptolDevHicoo X, sptIndex rank, sptIndex niters, double tol, sptolDevMatrix *mats,
+                           sptValue *lambda_out, sptolCpdStats *stats, void *stream)
+{
+    hipStream_t s = stream_of(stream);
+    for (uint32_t m = 0; m < X->nmodes; ++m) SPTOL_TRY(hicoo_prepare(X, m, s));
+    StreamScope scope(s);   // the workspace below is freed after the final synchronisation
+    auto t0 = std::chrono::steady_clock::now();
+    CpdJob job;
+    SPTOL_TRY(job.setup_hicoo(X, rank, niters, tol, mats, stats ? stats->factor_dtype : SPTOL_F32, s));
+    return run_job(job, lambda_out, stats, s, t0);
 }
 
 // The reference's exported ALS body (cpd.c:32-137) with caller-supplied
@@ -1176,6 +1227,82 @@ int sptOmpCpdAls(sptSparseTensor const * const spten, sptIndex const rank, sptIn
     (void)tk;
     (void)use_reduce;
     return sptCpdAls(spten, rank, niters, tol, ktensor);
+}
+
+// sptCpdAlsHiCOO (hicoo/cpd.c:149-188 with spt_CpdAlsStepHiCOO, :34-135):
+// seeded U[0,1) factors, the ALS loop on the device over the HiCOO tensor,
+// factors and weights back in ktensor.  factors[nmodes] is the emptied
+// MTTKRP scratch, as the reference leaves it.
+int sptCpdAlsHiCOO(sptSparseTensorHiCOO const * const hitsr, sptIndex const rank, sptIndex const niters,
+                   double const tol, sptRankKruskalTensor * ktensor)
+{
+    const uint32_t N = hitsr->nmodes;
+    if (N < 2 || N > (uint32_t)kMaxModes) SPTOL_CHECK(SPTERR_VALUE_ERROR, "CPU HiCOO SpTns CPD-ALS", "nmodes must be in [2, 8]");
+    if (rank < 1 || rank > (uint32_t)kMaxRank) SPTOL_CHECK(SPTERR_VALUE_ERROR, "CPU HiCOO SpTns CPD-ALS", "rank must be in [1, 64]");
+    if (!ktensor->lambda) SPTOL_CHECK(SPTERR_VALUE_ERROR, "CPU HiCOO SpTns CPD-ALS", "ktensor not created by sptNewRankKruskalTensor");
+    sptIndex max_dim = sptMaxIndexArray(hitsr->ndims, N);
+    sptRankMatrix **mats = (sptRankMatrix **)calloc(N + 1, sizeof(sptRankMatrix *));
+    SPTOL_CHECK_OS(!mats, "CPU HiCOO SpTns CPD-ALS");
+    int r = 0;
+    for (uint32_t m = 0; m <= N && !r; ++m) {
+        mats[m] = (sptRankMatrix *)calloc(1, sizeof(sptRankMatrix));
+        if (!mats[m]) { r = SPTERR_OS_ERROR + ENOMEM; break; }
+        r = sptNewRankMatrix(mats[m], m < N ? hitsr->ndims[m] : max_dim, (sptElementIndex)rank);
+        if (!r) r = m < N ? sptRandomizeRankMatrix(mats[m], hitsr->ndims[m], (sptElementIndex)rank)
+                          : sptConstantRankMatrix(mats[m], 0);
+    }
+    sptolDevHicoo H = nullptr;
+    sptolDevMatrix dm[kMaxModes];
+    std::memset(dm, 0, sizeof(dm));
+    // a sptMatrix view of a rank matrix (same row stride rule)
+    auto view = [](sptRankMatrix *a) {
+        sptMatrix v;
+        v.nrows = a->nrows; v.ncols = a->ncols; v.cap = a->cap; v.stride = a->stride; v.values = a->values;
+        return v;
+    };
+    if (!r) r = sptolDevHicooCreate(&H, hitsr);
+    for (uint32_t m = 0; m < N && !r; ++m) {
+        r = sptolDevMatrixCreate(&dm[m], hitsr->ndims[m], rank, SPTOL_F32);
+        sptMatrix v = view(mats[m]);
+        if (!r) r = sptolDevMatrixUpload(&dm[m], &v);
+    }
+    sptolCpdStats st;
+    std::memset(&st, 0, sizeof(st));
+    st.factor_dtype = SPTOL_F32;
+    auto t0 = std::chrono::steady_clock::now();
+    if (!r) r = sptolCpdAlsHicooDevice(H, rank, niters, tol, dm, ktensor->lambda, &st, nullptr);
+    for (uint32_t m = 0; m < N && !r; ++m) {
+        sptMatrix v = view(mats[m]);
+        r = sptolDevMatrixDownload(&v, &dm[m]);
+    }
+    for (uint32_t m = 0; m < N; ++m) sptolDevMatrixFree(&dm[m]);
+    sptolDevHicooFree(H);
+    if (r) {
+        for (uint32_t m = 0; m <= N; ++m)
+            if (mats[m]) { sptFreeRankMatrix(mats[m]); free(mats[m]); }
+        free(mats);
+        return r;
+    }
+    if (verbose())
+        fprintf(stdout, "[GPU  HiCOO SpTns CPD-ALS]: %.9lf s\n",
+                std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count());
+    sptFreeRankMatrix(mats[N]);
+    ktensor->fit = st.fit;
+    if (ktensor->factors) {
+        for (uint32_t m = 0; m <= ktensor->nmodes; ++m)
+            if (ktensor->factors[m]) { sptFreeRankMatrix(ktensor->factors[m]); free(ktensor->factors[m]); }
+        free(ktensor->factors);
+    }
+    ktensor->factors = mats;   // owned by the Kruskal tensor (freed by sptFreeRankKruskalTensor)
+    return 0;
+}
+
+int sptOmpCpdAlsHiCOO(sptSparseTensorHiCOO const * const hitsr, sptIndex const rank, sptIndex const niters,
+                      double const tol, const int tk, int balanced, sptRankKruskalTensor * ktensor)
+{
+    (void)tk;
+    (void)balanced;
+    return sptCpdAlsHiCOO(hitsr, rank, niters, tol, ktensor);
 }
 
 }  // extern "C"

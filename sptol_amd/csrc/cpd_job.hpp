@@ -36,7 +36,11 @@ struct AlsBatchBuffers {
 };
 
 struct CpdJob {
-    sptolDevCoo X = nullptr;
+    sptolDevCoo X = nullptr;              // the tensor: COO ...
+    sptolDevHicoo H = nullptr;            // ... or HiCOO
+    uint32_t ndims[kMaxModes] = {};
+    uint64_t nnz = 0;
+    const float *vals = nullptr;          // the tensor's values (device)
     uint32_t N = 0, R = 0, S = 0, maxI = 0;
     int fdt = SPTOL_F32;
     uint32_t niters = 0;
@@ -59,7 +63,11 @@ struct CpdJob {
     // Allocate, ||X||^2, initial Grams (and BF16 copies).  Every mode of X
     // must already have its sorted copy (coo_prepare).
     int setup(sptolDevCoo X, uint32_t R, uint32_t niters, double tol, sptolDevMatrix *mats, int fdt, hipStream_t s);
+    // The same over a device HiCOO tensor whose every mode has its view.
+    int setup_hicoo(sptolDevHicoo H, uint32_t R, uint32_t niters, double tol, sptolDevMatrix *mats, int fdt,
+                    hipStream_t s);
     void order(uint32_t *o) const;        // mats_order of the current mode
+    int init(uint32_t R, uint32_t niters, double tol, sptolDevMatrix *mats, int fdt, hipStream_t s);
     int mttkrp(hipStream_t s);            // the current mode's MTTKRP into Mout
     // Solve, normalise and Gram for the current mode, then advance; at the
     // end of an iteration queue the fit and its copy to the host and set

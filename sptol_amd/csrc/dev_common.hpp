@@ -41,6 +41,12 @@ void dfree(void *ptr);
 // before the dfree call (work enqueued on s, or on streams s was made to
 // wait for, or already complete).  dmalloc calls in the scope may reuse such
 // a block before that work has run, as their own work is queued behind it.
+// Page-locked chunks of kPinnedChunkBytes for streaming uploads (a
+// process-lifetime pool shared by all threads).
+constexpr size_t kPinnedChunkBytes = (size_t)16 << 20;
+int pinned_chunk(void **p);
+void pinned_chunk_free(void *p);
+
 // Page-locked host slot of kPinnedSlotBytes (process-lifetime slabs).
 constexpr size_t kPinnedSlotBytes = 1024;
 int pinned_slot(void **p);
@@ -102,9 +108,13 @@ struct SegView {
 // Pack a mode-sorted SoA copy (inds[0..N-1], vals) into v (allocates).
 // With perm, inds and vals are in any order and nonzero perm[z] becomes
 // record z; sorted_rows[z] is then its mode index, and copy_vals must be set.
+// With rec0 as well (records of the tensor in its own order, from
+// seg_pack_unsorted), each record is one 16-byte gather from rec0.
 int seg_build(SegView &v, uint32_t nmodes, uint32_t mode, uint32_t nrows, uint64_t nnz,
               uint32_t *const *inds, const float *vals, bool copy_vals, hipStream_t s, const uint32_t *ndims,
-              const uint32_t *perm = nullptr, const uint32_t *sorted_rows = nullptr);
+              const uint32_t *perm = nullptr, const uint32_t *sorted_rows = nullptr, const uint4 *rec0 = nullptr);
+int seg_pack_unsorted(uint4 *rec0, uint32_t nmodes, uint64_t nnz, uint32_t *const *inds, const float *vals,
+                      hipStream_t s);
 // Whether the segmented kernel handles this shape.
 bool seg_ok(uint32_t nmodes, uint32_t stride);
 // Lanes per nonzero (one float4 of columns each) for a row stride.
@@ -182,6 +192,18 @@ int dup_on(void **dst, const void *src, size_t bytes, hipStream_t s, const char 
 // Stream-aware forms of sptolDevCooCreate / sptolDevCooPrepareMode.
 int coo_create(sptolDevCoo *h, const sptSparseTensor *X, hipStream_t s);
 int coo_prepare(sptolDevCoo h, uint32_t mode, hipStream_t s);
+// Every mode (3- and 4-mode tensors pack their records once and gather the
+// views from them).
+int coo_prepare_all(sptolDevCoo h, hipStream_t s);
+// Device HiCOO MTTKRP on stream s (sptolDevMTTKRPHiCOO without the checks)
+// and the stream-aware form of sptolDevHicooPrepareMode.
+int hicoo_mttkrp(sptolDevHicoo X, const sptolDevMatrix *mats, const uint32_t *order, uint32_t mode,
+                 sptolDevMatrix *out, hipStream_t s);
+int hicoo_prepare(sptolDevHicoo h, uint32_t mode, hipStream_t s);
+// Device COO tensor straight from a binary tensor file (the layout of
+// sptolSaveSparseTensorBinary), streamed through pinned chunks; indices are
+// checked on the device.  Synchronises s.
+int coo_load_binary(sptolDevCoo *h, const char *path, hipStream_t s);
 // *finite = no NaN or infinity among vals[0..n) (synchronises s).
 int check_finite(const float *vals, uint64_t n, hipStream_t s, bool *finite);
 // Device MTTKRP on stream s (sptolDevMTTKRP without the argument checks).

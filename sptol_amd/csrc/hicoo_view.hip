@@ -489,7 +489,13 @@ int sptolSetHicooViewKind(int kind)
     return 0;
 }
 
-int sptolDevHicooPrepareMode(sptolDevHicoo h, sptIndex mode)
+int sptolDevHicooPrepareMode(sptolDevHicoo h, sptIndex mode) { return hicoo_prepare(h, mode, nullptr); }
+
+}  // extern "C"
+
+namespace sptol {
+
+int hicoo_prepare(sptolDevHicoo h, uint32_t mode, hipStream_t s)
 {
     if (mode >= h->nmodes) SPTOL_CHECK(SPTERR_VALUE_ERROR, "HiCOO view", "mode out of range");
     if (!h->views) h->views = new sptolHicooViews();
@@ -499,13 +505,9 @@ int sptolDevHicooPrepareMode(sptolDevHicoo h, sptIndex mode)
     bool use_seg = false;
     if (g_view_kind == SPTOL_HICOO_VIEW_SEG) use_seg = seg_shape;
     else if (g_view_kind == SPTOL_HICOO_VIEW_AUTO) use_seg = seg_shape && h->nnz < 8 * h->nb;
-    if (use_seg) return build_hicoo_seg(h, mode, v, nullptr);
-    return build_hicoo_view(h, mode, v, nullptr);
+    if (use_seg) return build_hicoo_seg(h, mode, v, s);
+    return build_hicoo_view(h, mode, v, s);
 }
-
-}  // extern "C"
-
-namespace sptol {
 
 void free_hicoo_views(sptolDevHicoo h)
 {

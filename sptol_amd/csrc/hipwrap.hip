@@ -154,6 +154,32 @@ void pinned_slot_free(void *p)
     g_pin_free.push_back(p);
 }
 
+// Pinned chunks for streaming file data to the device (kPinnedChunkBytes
+// each), kept for the life of the process and shared by all threads.
+static std::mutex g_chunk_mu;
+static std::vector<void *> g_chunk_free;
+
+int pinned_chunk(void **p)
+{
+    {
+        std::lock_guard<std::mutex> lk(g_chunk_mu);
+        if (!g_chunk_free.empty()) {
+            *p = g_chunk_free.back();
+            g_chunk_free.pop_back();
+            return 0;
+        }
+    }
+    SPTOL_HIP(hipHostMalloc(p, kPinnedChunkBytes, hipHostMallocDefault), "HipWrap pinned chunk");
+    return 0;
+}
+
+void pinned_chunk_free(void *p)
+{
+    if (!p) return;
+    std::lock_guard<std::mutex> lk(g_chunk_mu);
+    g_chunk_free.push_back(p);
+}
+
 int PinnedStage::upload(void *dev, const void *host, size_t n, hipStream_t s)
 {
     if (!n) return 0;

@@ -259,6 +259,16 @@ int sptConstantRankMatrix(sptRankMatrix *mtx, sptValue const val)
     return 0;
 }
 
+/* Seeded U[0,1) entries, the stream of sptRandomizeMatrix (the reference
+ * reseeds with time(NULL) per call, rankmatrix.c:70-78). */
+int sptRandomizeRankMatrix(sptRankMatrix *mtx, sptIndex const nrows, sptElementIndex const ncols)
+{
+    for (sptIndex i = 0; i < nrows; ++i)
+        for (sptElementIndex j = 0; j < ncols; ++j)
+            mtx->values[(size_t)i * mtx->stride + j] = sptol_rng_uniform01();
+    return 0;
+}
+
 void sptFreeRankMatrix(sptRankMatrix *mtx)
 {
     free(mtx->values);
@@ -321,6 +331,45 @@ void sptFreeKruskalTensor(sptKruskalTensor *ktsr)
         for (sptIndex m = 0; m < ktsr->nmodes + 1; ++m) {
             if (ktsr->factors[m]) {
                 sptFreeMatrix(ktsr->factors[m]);
+                free(ktsr->factors[m]);
+            }
+        }
+        free(ktsr->factors);
+    }
+    free(ktsr->ndims);
+    free(ktsr->lambda);
+    ktsr->ndims = NULL;
+    ktsr->lambda = NULL;
+    ktsr->factors = NULL;
+    ktsr->rank = 0;
+    ktsr->fit = 0.0;
+    ktsr->nmodes = 0;
+}
+
+/* Rank Kruskal tensor (rankktensor.c:33-44, 78-88). */
+int sptNewRankKruskalTensor(sptRankKruskalTensor *ktsr, sptIndex nmodes, const sptIndex ndims[], sptElementIndex rank)
+{
+    ktsr->nmodes = nmodes;
+    ktsr->rank = rank;
+    ktsr->ndims = (sptIndex *)malloc(nmodes * sizeof(sptIndex));
+    SPTOL_CHECK_OS(!ktsr->ndims, "RankKruskalTns New");
+    memcpy(ktsr->ndims, ndims, nmodes * sizeof(sptIndex));
+    ktsr->lambda = (sptValue *)malloc((rank ? rank : 1) * sizeof(sptValue));
+    SPTOL_CHECK_OS(!ktsr->lambda, "RankKruskalTns New");
+    ktsr->fit = 0.0;
+    ktsr->factors = NULL;
+    return 0;
+}
+
+/* Frees the factors as sptFreeKruskalTensor does: sptCpdAlsHiCOO sets
+ * nmodes+1 heap-allocated sptRankMatrix structs (the last is the emptied
+ * MTTKRP scratch). */
+void sptFreeRankKruskalTensor(sptRankKruskalTensor *ktsr)
+{
+    if (ktsr->factors) {
+        for (sptIndex m = 0; m < ktsr->nmodes + 1; ++m) {
+            if (ktsr->factors[m]) {
+                sptFreeRankMatrix(ktsr->factors[m]);
                 free(ktsr->factors[m]);
             }
         }

@@ -6,6 +6,8 @@
 #include <errno.h>
 #include <string.h>
 #include <stdlib.h>
+#include <stdio.h>
+#include <stdint.h>
 
 #ifdef __cplusplus
 extern "C" {
@@ -47,6 +49,10 @@ int sptol_radix_sort_u64(uint64_t *keys, uint64_t *idx, uint64_t n, unsigned bit
 /* Apply perm (new position i takes old element perm[i]) to tensor rows
  * [begin, begin+n) of a COO tensor's index and value arrays. */
 int sptol_permute_coo(sptSparseTensor *tsr, uint64_t begin, const uint64_t *perm, uint64_t n);
+
+/* Binary tensor file (sptolSaveSparseTensorBinary): read and check the
+ * header; on return fp is positioned at inds[0].  ndims holds 8 entries. */
+int sptol_read_binary_header(FILE *fp, uint32_t *nmodes, uint64_t *nnz, uint32_t *ndims);
 
 /* Number of bits needed to hold values in [0, n). */
 static inline unsigned sptol_bits_for(uint64_t n)

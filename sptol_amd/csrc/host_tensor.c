@@ -431,6 +431,128 @@ int sptLoadSparseTensor(sptSparseTensor *tsr, sptIndex start_index, FILE *fp)
     return 0;
 }
 
+/* ------------------------------------------------------------------------ */
+/* Ingest: duplicate merging and the binary tensor file                     */
+/* ------------------------------------------------------------------------ */
+
+/* The reference keeps duplicated coordinates as separate nonzeros
+ * (load.c:31-86 only drops exact zeros).  Here they are merged: a stable
+ * lexicographic sort (sptSparseTensorSortIndex), then each run of equal
+ * coordinates becomes one nonzero whose value is the FP64 sum of the run
+ * in input order, rounded once.  The tensor ends sorted. */
+int sptolSparseTensorMergeDuplicates(sptSparseTensor *tsr, int tk)
+{
+    if (tsr->nnz < 2) return 0;
+    sptSparseTensorSortIndex(tsr, 1, tk);
+    const sptIndex N = tsr->nmodes;
+    sptNnzIndex w = 0;
+    for (sptNnzIndex r = 0; r < tsr->nnz;) {
+        sptNnzIndex e = r + 1;
+        for (; e < tsr->nnz; ++e) {
+            sptIndex m = 0;
+            while (m < N && tsr->inds[m].data[e] == tsr->inds[m].data[r]) ++m;
+            if (m < N) break;
+        }
+        double acc = 0.0;
+        for (sptNnzIndex k = r; k < e; ++k) acc += (double)tsr->values.data[k];
+        for (sptIndex m = 0; m < N; ++m) tsr->inds[m].data[w] = tsr->inds[m].data[r];
+        tsr->values.data[w] = (sptValue)acc;
+        ++w;
+        r = e;
+    }
+    tsr->nnz = w;
+    for (sptIndex m = 0; m < N; ++m) tsr->inds[m].len = w;
+    tsr->values.len = w;
+    return 0;
+}
+
+static const char kBinMagic[8] = {'S', 'P', 'T', 'O', 'L', 'B', 'T', '1'};
+
+/* Header: 8-byte magic "SPTOLBT1", u32 version (1), u32 nmodes, u64 nnz,
+ * u32 ndims[nmodes], zero padding to a multiple of 8 bytes; then
+ * inds[0..nmodes-1] (u32 x nnz each) and the values (f32 x nnz), all
+ * little-endian.  One sequential read or write per array. */
+static size_t binary_header_bytes(uint32_t nmodes) { return (24 + 4 * (size_t)nmodes + 7) / 8 * 8; }
+
+int sptolSaveSparseTensorBinary(const sptSparseTensor *tsr, const char *path)
+{
+    if (tsr->nmodes < 1 || tsr->nmodes > 8) SPTOL_CHECK(SPTERR_VALUE_ERROR, "SpTns Save", "nmodes must be in [1, 8]");
+    FILE *fp = fopen(path, "wb");
+    SPTOL_CHECK_OS(!fp, "SpTns Save");
+    unsigned char hdr[64];
+    memset(hdr, 0, sizeof(hdr));
+    const uint32_t version = 1, nm = tsr->nmodes;
+    const uint64_t nnz = tsr->nnz;
+    memcpy(hdr, kBinMagic, 8);
+    memcpy(hdr + 8, &version, 4);
+    memcpy(hdr + 12, &nm, 4);
+    memcpy(hdr + 16, &nnz, 8);
+    for (uint32_t m = 0; m < nm; ++m) memcpy(hdr + 24 + 4 * m, &tsr->ndims[m], 4);
+    int ok = fwrite(hdr, 1, binary_header_bytes(nm), fp) == binary_header_bytes(nm);
+    for (uint32_t m = 0; m < nm && ok; ++m)
+        ok = fwrite(tsr->inds[m].data, 4, nnz, fp) == nnz;
+    if (ok) ok = fwrite(tsr->values.data, 4, nnz, fp) == nnz;
+    if (fclose(fp) != 0) ok = 0;
+    SPTOL_CHECK_OS(!ok, "SpTns Save");
+    return 0;
+}
+
+int sptol_read_binary_header(FILE *fp, uint32_t *nmodes, uint64_t *nnz, uint32_t *ndims)
+{
+    unsigned char hdr[64];
+    if (fread(hdr, 1, 24, fp) != 24 || memcmp(hdr, kBinMagic, 8) != 0)
+        SPTOL_CHECK(SPTERR_VALUE_ERROR, "SpTns Load Binary", "not a binary tensor file");
+    uint32_t version, nm;
+    memcpy(&version, hdr + 8, 4);
+    memcpy(&nm, hdr + 12, 4);
+    memcpy(nnz, hdr + 16, 8);
+    if (version != 1) SPTOL_CHECK(SPTERR_VALUE_ERROR, "SpTns Load Binary", "unknown format version");
+    if (nm < 1 || nm > 8) SPTOL_CHECK(SPTERR_VALUE_ERROR, "SpTns Load Binary", "nmodes must be in [1, 8]");
+    const size_t rest = binary_header_bytes(nm) - 24;
+    if (fread(hdr + 24, 1, rest, fp) != rest) SPTOL_CHECK(SPTERR_VALUE_ERROR, "SpTns Load Binary", "truncated header");
+    for (uint32_t m = 0; m < nm; ++m) memcpy(&ndims[m], hdr + 24 + 4 * m, 4);
+    *nmodes = nm;
+    /* the file must hold every array */
+    long here = ftell(fp);
+    if (here < 0 || fseek(fp, 0, SEEK_END) != 0) SPTOL_CHECK_OS(1, "SpTns Load Binary");
+    long end = ftell(fp);
+    if (end < 0 || fseek(fp, here, SEEK_SET) != 0) SPTOL_CHECK_OS(1, "SpTns Load Binary");
+    if ((uint64_t)(end - here) < *nnz * 4ull * (nm + 1ull))
+        SPTOL_CHECK(SPTERR_VALUE_ERROR, "SpTns Load Binary", "file shorter than its arrays");
+    return 0;
+}
+
+int sptolLoadSparseTensorBinary(sptSparseTensor *tsr, const char *path)
+{
+    memset(tsr, 0, sizeof(*tsr));
+    FILE *fp = fopen(path, "rb");
+    SPTOL_CHECK_OS(!fp, "SpTns Load Binary");
+    uint32_t nm = 0, nd[8];
+    uint64_t nnz = 0;
+    int r = sptol_read_binary_header(fp, &nm, &nnz, nd);
+    if (!r) r = sptNewSparseTensor(tsr, nm, nd);
+    for (uint32_t m = 0; m < nm && !r; ++m) {
+        sptFreeIndexVector(&tsr->inds[m]);
+        r = sptNewIndexVector(&tsr->inds[m], nnz, nnz);
+        if (!r && fread(tsr->inds[m].data, 4, nnz, fp) != nnz) r = SPTERR_OS_ERROR + (errno ? errno : EIO);
+    }
+    if (!r) {
+        sptFreeValueVector(&tsr->values);
+        r = sptNewValueVector(&tsr->values, nnz, nnz);
+    }
+    if (!r && fread(tsr->values.data, 4, nnz, fp) != nnz) r = SPTERR_OS_ERROR + (errno ? errno : EIO);
+    fclose(fp);
+    if (r) {
+        sptol_complain("SpTns Load Binary", r, __FILE__, __LINE__, "read failed");
+        return r;
+    }
+    tsr->nnz = nnz;
+    for (uint32_t m = 0; m < nm; ++m)
+        for (uint64_t z = 0; z < nnz; ++z)
+            if (tsr->inds[m].data[z] >= nd[m]) SPTOL_CHECK(SPTERR_VALUE_ERROR, "SpTns Load Binary", "index >= ndims");
+    return 0;
+}
+
 int sptDumpSparseTensor(const sptSparseTensor *tsr, sptIndex start_index, FILE *fp)
 {
     int iores = fprintf(fp, "%" PRIu32 "\n", tsr->nmodes);

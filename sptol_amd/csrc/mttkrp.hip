@@ -25,6 +25,8 @@
 //    U consecutive nonzeros), then forms coordinates as
 //    (binds << sb_bits) + einds from 1-byte element offsets.
 #include "dev_common.hpp"
+#include <algorithm>
+#include <cerrno>
 #include <hipcub/hipcub.hpp>
 #include <vector>
 #include <cstdio>
@@ -467,6 +469,92 @@ int check_finite(const float *vals, uint64_t n, hipStream_t s, bool *finite)
     return 0;
 }
 
+// Reject any index >= ndims on the device (t's arrays are on s).
+static int check_inds_dev(sptolDevCoo t, hipStream_t s, const char *module)
+{
+    if (!t->nnz) return 0;
+    uint32_t *bad = nullptr;
+    SPTOL_TRY(dmalloc((void **)&bad, 4, module));
+    uint32_t hb = 0;
+    hipError_t e = hipMemsetAsync(bad, 0, 4, s);
+    for (uint32_t m = 0; m < t->nmodes && e == hipSuccess; ++m) {
+        hipLaunchKernelGGL(k_check_inds, dim3(grid_for(t->nnz)), dim3(256), 0, s, t->inds[m], t->nnz, t->ndims[m], bad);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(&hb, bad, 4, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    dfree(bad);
+    SPTOL_HIP(e, module);
+    if (hb) SPTOL_CHECK(SPTERR_VALUE_ERROR, module, "index >= ndims");
+    return 0;
+}
+
+int coo_load_binary(sptolDevCoo *h, const char *path, hipStream_t s)
+{
+    *h = nullptr;
+    StreamScope scope(s);   // everything freed here was used on s only
+    FILE *fp = fopen(path, "rb");
+    if (!fp) {
+        const int en = errno ? errno : ENOENT;
+        sptol_complain("DevCoo Load", SPTERR_OS_ERROR + en, __FILE__, __LINE__, strerror(en));
+        return SPTERR_OS_ERROR + en;
+    }
+    uint32_t nm = 0, nd[8] = {};
+    uint64_t nnz = 0;
+    int r = sptol_read_binary_header(fp, &nm, &nnz, nd);
+    if (!r && nm < 2) {
+        sptol_complain("DevCoo Load", SPTERR_VALUE_ERROR, __FILE__, __LINE__, "nmodes must be in [2, 8]");
+        r = SPTERR_VALUE_ERROR;
+    }
+    if (r) { fclose(fp); return r; }
+    sptolDevCoo t = new sptolDevCoo_s();
+    t->nmodes = nm;
+    t->nnz = nnz;
+    for (uint32_t m = 0; m < nm; ++m) t->ndims[m] = nd[m];
+    for (uint32_t m = 0; m <= nm && !r; ++m)
+        r = dmalloc(m < nm ? (void **)&t->inds[m] : (void **)&t->vals, nnz * 4, "DevCoo Load");
+    // every array in file order, chunk by chunk: the read of a chunk
+    // overlaps the copy of the previous one
+    void *buf[2] = {};
+    hipEvent_t ev[2] = {};
+    bool used[2] = {false, false};
+    hipError_t e = hipSuccess;
+    for (int k = 0; k < 2 && !r; ++k) {
+        r = pinned_chunk(&buf[k]);
+        if (!r && (e = hipEventCreateWithFlags(&ev[k], hipEventDisableTiming)) != hipSuccess)
+            r = SPTERR_CUDA_ERROR + (int)e;
+    }
+    int k = 0;
+    for (uint32_t m = 0; m <= nm && !r; ++m) {
+        char *dst = (char *)(m < nm ? (void *)t->inds[m] : (void *)t->vals);
+        for (uint64_t off = 0; off < nnz * 4 && !r; off += kPinnedChunkBytes) {
+            const size_t n = (size_t)std::min<uint64_t>(kPinnedChunkBytes, nnz * 4 - off);
+            if (used[k] && (e = hipEventSynchronize(ev[k])) != hipSuccess) { r = SPTERR_CUDA_ERROR + (int)e; break; }
+            if (fread(buf[k], 1, n, fp) != n) {
+                sptol_complain("DevCoo Load", SPTERR_OS_ERROR + EIO, __FILE__, __LINE__, "short read");
+                r = SPTERR_OS_ERROR + EIO;
+                break;
+            }
+            e = hipMemcpyAsync(dst + off, buf[k], n, hipMemcpyHostToDevice, s);
+            if (e == hipSuccess) e = hipEventRecord(ev[k], s);
+            if (e != hipSuccess) { r = SPTERR_CUDA_ERROR + (int)e; break; }
+            used[k] = true;
+            k ^= 1;
+        }
+    }
+    fclose(fp);
+    e = hipStreamSynchronize(s);
+    if (!r && e != hipSuccess) r = SPTERR_CUDA_ERROR + (int)e;
+    for (int q = 0; q < 2; ++q) {
+        pinned_chunk_free(buf[q]);
+        if (ev[q]) (void)hipEventDestroy(ev[q]);
+    }
+    if (!r) r = check_inds_dev(t, s, "DevCoo Load");
+    if (r) { sptolDevCooFree(t); return r; }
+    *h = t;
+    return 0;
+}
+
 int coo_create(sptolDevCoo *h, const sptSparseTensor *X, hipStream_t s)
 {
     *h = nullptr;
@@ -485,26 +573,35 @@ int coo_create(sptolDevCoo *h, const sptSparseTensor *X, hipStream_t s)
     // The kernels index factor rows with these values, so reject any index
     // >= ndims before any kernel uses them (the reference trusts its input;
     // here an out-of-range index would be a device fault).
-    if (X->nnz) {
-        uint32_t *bad = nullptr;
-        if ((r = dmalloc((void **)&bad, 4, "DevCoo New"))) { sptolDevCooFree(t); return r; }
-        uint32_t hb = 0;
-        hipError_t e = hipMemsetAsync(bad, 0, 4, s);
-        for (uint32_t m = 0; m < t->nmodes && e == hipSuccess; ++m) {
-            hipLaunchKernelGGL(k_check_inds, dim3(grid_for(X->nnz)), dim3(256), 0, s, t->inds[m], X->nnz, X->ndims[m], bad);
-            e = hipGetLastError();
-        }
-        if (e == hipSuccess) e = hipMemcpyAsync(&hb, bad, 4, hipMemcpyDeviceToHost, s);
-        if (e == hipSuccess) e = hipStreamSynchronize(s);
-        dfree(bad);
-        if (e != hipSuccess) { sptolDevCooFree(t); SPTOL_HIP(e, "DevCoo New"); }
-        if (hb) { sptolDevCooFree(t); SPTOL_CHECK(SPTERR_VALUE_ERROR, "DevCoo New", "index >= ndims"); }
-    }
+    if ((r = check_inds_dev(t, s, "DevCoo New"))) { sptolDevCooFree(t); return r; }
     *h = t;
     return 0;
 }
 
-int coo_prepare(sptolDevCoo h, uint32_t mode, hipStream_t s)
+static int coo_prepare_from(sptolDevCoo h, uint32_t mode, hipStream_t s, const uint4 *rec0);
+
+int coo_prepare(sptolDevCoo h, uint32_t mode, hipStream_t s) { return coo_prepare_from(h, mode, s, nullptr); }
+
+int coo_prepare_all(sptolDevCoo h, hipStream_t s)
+{
+    bool todo = false;
+    for (uint32_t m = 0; m < h->nmodes; ++m) todo = todo || !h->sorted[m].ready;
+    if (!todo) return 0;
+    if ((h->nmodes != 3 && h->nmodes != 4) || h->nnz == 0 || h->nnz >= 0xFFFFFFFFull) {
+        for (uint32_t m = 0; m < h->nmodes; ++m) SPTOL_TRY(coo_prepare(h, m, s));
+        return 0;
+    }
+    StreamScope scope(s);   // the packed records are used on s only
+    uint4 *rec0 = nullptr;
+    SPTOL_TRY(dmalloc((void **)&rec0, h->nnz * sizeof(uint4), "DevCoo Prepare"));
+    int r = seg_pack_unsorted(rec0, h->nmodes, h->nnz, h->inds, h->vals, s);
+    for (uint32_t m = 0; m < h->nmodes && !r; ++m) r = coo_prepare_from(h, m, s, rec0);
+    (void)hipStreamSynchronize(s);
+    dfree(rec0);
+    return r;
+}
+
+static int coo_prepare_from(sptolDevCoo h, uint32_t mode, hipStream_t s, const uint4 *rec0)
 {
     if (mode >= h->nmodes) SPTOL_CHECK(SPTERR_VALUE_ERROR, "DevCoo Prepare", "mode out of range");
     SortedCopy &sc = h->sorted[mode];
@@ -535,7 +632,8 @@ int coo_prepare(sptolDevCoo h, uint32_t mode, hipStream_t s)
         // 3- and 4-mode tensors: the packed records are gathered straight
         // through the permutation; no sorted SoA copy is kept (the atomic
         // fallback kernel reads the tensor in its own order)
-        r = seg_build(sc.seg, h->nmodes, mode, h->ndims[mode], n, h->inds, h->vals, true, s, h->ndims, perm2, keys2);
+        r = seg_build(sc.seg, h->nmodes, mode, h->ndims[mode], n, h->inds, h->vals, true, s, h->ndims, perm2, keys2,
+                      rec0);
         cleanup();
         if (r) return r;
         sc.ready = true;
@@ -565,6 +663,11 @@ int coo_prepare(sptolDevCoo h, uint32_t mode, hipStream_t s)
 extern "C" {
 
 int sptolDevCooCreate(sptolDevCoo *h, const sptSparseTensor *X) { return coo_create(h, X, nullptr); }
+
+int sptolDevCooLoadBinary(sptolDevCoo *h, const char *path, void *stream)
+{
+    return coo_load_binary(h, path, stream_of(stream));
+}
 
 int sptolDevCooPrepareMode(sptolDevCoo h, sptIndex mode) { return coo_prepare(h, mode, nullptr); }
 
@@ -704,6 +807,14 @@ int sptolDevMTTKRPHiCOO(sptolDevHicoo X, const sptolDevMatrix *mats, const sptIn
                         sptolDevMatrix *out, void *stream)
 {
     SPTOL_TRY(check_dev_mats(X->nmodes, X->ndims, mats, mats_order, mode, out, "DevHiCOO MTTKRP"));
+    return hicoo_mttkrp(X, mats, mats_order, mode, out, stream_of(stream));
+}
+
+}  // extern "C"
+
+int sptol::hicoo_mttkrp(sptolDevHicoo X, const sptolDevMatrix *mats, const uint32_t *mats_order, uint32_t mode,
+                        sptolDevMatrix *out, hipStream_t s)
+{
     MttkrpArgs a;
     std::memset(&a, 0, sizeof(a));
     a.nnz = X->nnz;
@@ -722,15 +833,12 @@ int sptolDevMTTKRPHiCOO(sptolDevHicoo X, const sptolDevMatrix *mats, const sptIn
     a.nb = X->nb;
     a.sb_bits = X->sb_bits;
     a.out = (float *)out->values;
-    hipStream_t s = stream_of(stream);
     int used = 0;
     SPTOL_TRY(launch_hicoo_view(X, a, mats[0].dtype, s, &used));
     if (used) return 0;
     SPTOL_TRY(launch_fill_zero(a.out, (uint64_t)X->ndims[mode] * out->stride, s));
     return launch_mttkrp_hicoo(a, mats[0].dtype, s);
 }
-
-}  // extern "C"
 
 // ---------------------------------------------------------------------------
 // ParTI!-compatible host entry points: upload, run on the device, download.

@@ -73,6 +73,18 @@ __global__ void k_seg_pack(uint4 *__restrict__ rec, float *__restrict__ vout, co
     }
 }
 
+// rec[z] = rec0[perm[z]] (and vout[z] = vals[perm[z]]): one 16-byte gather
+// per nonzero from records packed in the tensor's own order.
+__global__ void k_seg_gather_rec(uint4 *__restrict__ rec, float *__restrict__ vout, const uint4 *__restrict__ rec0,
+                                 const float *__restrict__ vals, const uint32_t *__restrict__ perm, uint64_t nnz)
+{
+    for (uint64_t z = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; z < nnz; z += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t p = perm[z];
+        rec[z] = rec0[p];
+        if (vout) vout[z] = vals[p];
+    }
+}
+
 __global__ void k_seg_touch(uint32_t *__restrict__ touched, const uint32_t *__restrict__ rows, uint64_t nnz)
 {
     for (uint64_t z = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; z < nnz; z += (uint64_t)gridDim.x * blockDim.x) {
@@ -106,9 +118,21 @@ uint32_t seg_lpn(uint32_t stride)
     return lpn;
 }
 
+// Records of every nonzero in the tensor's own order (the layout of
+// SegView::rec), for seg_build's gather through a sort permutation.
+int seg_pack_unsorted(uint4 *rec0, uint32_t nmodes, uint64_t nnz, uint32_t *const *inds, const float *vals,
+                      hipStream_t s)
+{
+    if (!nnz) return 0;
+    hipLaunchKernelGGL(k_seg_pack, dim3(seg_grid(nnz)), dim3(256), 0, s, rec0, (float *)nullptr, inds[0], inds[1],
+                       inds[2], nmodes == 4 ? inds[3] : inds[0], vals, (const uint32_t *)nullptr, nnz, nmodes);
+    SPTOL_HIP(hipGetLastError(), "Seg view");
+    return 0;
+}
+
 int seg_build(SegView &v, uint32_t nmodes, uint32_t mode, uint32_t nrows, uint64_t nnz, uint32_t *const *inds,
               const float *vals, bool copy_vals, hipStream_t s, const uint32_t *ndims, const uint32_t *perm,
-              const uint32_t *sorted_rows)
+              const uint32_t *sorted_rows, const uint4 *rec0)
 {
     v.release();
     if (perm && (!sorted_rows || !copy_vals)) SPTOL_CHECK(SPTERR_VALUE_ERROR, "Seg view", "perm needs sorted rows and copied values");
@@ -140,8 +164,12 @@ int seg_build(SegView &v, uint32_t nmodes, uint32_t mode, uint32_t nrows, uint64
                 v.vals = const_cast<float *>(vals);
             }
         }
-        hipLaunchKernelGGL(k_seg_pack, dim3(seg_grid(nnz)), dim3(256), 0, s, v.rec, vout, inds[0], inds[1], inds[2],
-                           nmodes == 4 ? inds[3] : inds[0], vals, perm, nnz, nmodes);
+        if (perm && rec0)
+            hipLaunchKernelGGL(k_seg_gather_rec, dim3(seg_grid(nnz)), dim3(256), 0, s, v.rec, vout, rec0, vals, perm,
+                               nnz);
+        else
+            hipLaunchKernelGGL(k_seg_pack, dim3(seg_grid(nnz)), dim3(256), 0, s, v.rec, vout, inds[0], inds[1],
+                               inds[2], nmodes == 4 ? inds[3] : inds[0], vals, perm, nnz, nmodes);
         hipLaunchKernelGGL(k_seg_touch, dim3(seg_grid(nnz)), dim3(256), 0, s, v.touched,
                            perm ? sorted_rows : inds[mode], nnz);
     }

@@ -37,6 +37,9 @@
 #include <algorithm>
 #include <cstring>
 #include <chrono>
+#include <string>
+#include <cerrno>
+#include <cstdio>
 
 namespace sptol {
 
@@ -55,6 +58,7 @@ struct Request {
     std::vector<float> vals;
     const uint32_t *src_inds[kMaxModes] = {};
     const float *src_vals = nullptr;
+    std::string path;              // or a binary tensor file, read at admission
     uint32_t R = 0, niters = 0;
     double tol = 0.0;
     uint64_t seed = 0;
@@ -97,7 +101,7 @@ static size_t estimate_bytes(const Request &r)
     else
         b += (size_t)r.nnz * (4 * N + 4) * N;                  // one sorted copy per mode
     for (uint32_t m = 0; m < N; ++m) b += ((size_t)r.ndims[m] + 31) / 32 * 4;   // touched-row bitmaps
-    b += (size_t)r.nnz * 16 + (1u << 20);                      // sort temporaries of the preparation
+    b += (size_t)r.nnz * 32 + (1u << 20);                      // packed records and sort temporaries
     for (uint32_t m = 0; m < N; ++m) b += (size_t)r.ndims[m] * S * 4;
     b += CpdJob::workspace_bytes(r.ndims, N, r.R, r.fdt);
     return b;
@@ -155,8 +159,17 @@ static void finish_request(sptolServer srv, Request *r, int err)
     srv->cv_done.notify_all();
 }
 
+static int prepare_loaded(sptolServer srv, Request *r, hipStream_t s);
+
 static int prepare_request(sptolServer srv, Request *r, hipStream_t s)
 {
+    if (!r->path.empty()) {
+        SPTOL_TRY(coo_load_binary(&r->X, r->path.c_str(), s));
+        bool same = r->X->nmodes == r->nmodes && r->X->nnz == r->nnz;
+        for (uint32_t m = 0; m < r->nmodes && same; ++m) same = r->X->ndims[m] == r->ndims[m];
+        if (!same) SPTOL_CHECK(SPTERR_VALUE_ERROR, "Serve", "tensor file changed after submission");
+        return prepare_loaded(srv, r, s);
+    }
     // host view of the copied tensor
     sptIndexVector iv[kMaxModes];
     sptIndex nd[kMaxModes], so[kMaxModes];
@@ -175,10 +188,17 @@ static int prepare_request(sptolServer srv, Request *r, hipStream_t s)
     X.values.len = X.values.cap = r->nnz;
     X.values.data = const_cast<float *>(r->src_vals);
     SPTOL_TRY(coo_create(&r->X, &X, s));
+    return prepare_loaded(srv, r, s);
+}
+
+// The request's tensor is on the device: check the values, prepare every
+// mode, create the factors and set up the job.
+static int prepare_loaded(sptolServer srv, Request *r, hipStream_t s)
+{
     bool finite = true;
     SPTOL_TRY(check_finite(r->X->vals, r->nnz, s, &finite));
     if (!finite) SPTOL_CHECK(SPTERR_VALUE_ERROR, "Serve", "non-finite value");
-    for (uint32_t m = 0; m < r->nmodes; ++m) SPTOL_TRY(coo_prepare(r->X, m, s));
+    SPTOL_TRY(coo_prepare_all(r->X, s));
     uint64_t st = r->seed;
     for (uint32_t m = 0; m < r->nmodes; ++m) {
         SPTOL_TRY(sptolDevMatrixCreate(&r->mats[m], r->ndims[m], r->R, SPTOL_F32));
@@ -478,6 +498,44 @@ int sptolServerSubmit(sptolServer srv, const sptSparseTensor *X, sptIndex rank, 
         r->vals.assign(X->values.data, X->values.data + X->nnz);
         r->src_vals = r->vals.data();
     }
+    r->R = rank;
+    r->niters = niters;
+    r->tol = tol;
+    r->seed = seed;
+    r->fdt = factor_dtype;
+    r->t_submit = clk::now();
+    std::lock_guard<std::mutex> lk(srv->mu);
+    r->id = srv->next_id++;
+    srv->all[r->id] = r;
+    srv->queued.push_back(r);
+    srv->cv_admit.notify_all();
+    *id = r->id;
+    return 0;
+}
+
+int sptolServerSubmitFile(sptolServer srv, const char *path, sptIndex rank, sptIndex niters, double tol,
+                          uint64_t seed, int factor_dtype, uint64_t *id)
+{
+    if (rank < 1 || rank > (uint32_t)kMaxRank) SPTOL_CHECK(SPTERR_VALUE_ERROR, "Serve", "rank must be in [1, 64]");
+    if (factor_dtype != SPTOL_F32 && factor_dtype != SPTOL_BF16) SPTOL_CHECK(SPTERR_VALUE_ERROR, "Serve", "bad dtype");
+    // the header sizes the request's reservation; the arrays are read at admission
+    FILE *fp = fopen(path, "rb");
+    if (!fp) {
+        const int en = errno ? errno : ENOENT;
+        sptol_complain("Serve", SPTERR_OS_ERROR + en, __FILE__, __LINE__, strerror(en));
+        return SPTERR_OS_ERROR + en;
+    }
+    uint32_t nm = 0, nd[8] = {};
+    uint64_t nnz = 0;
+    int rc = sptol_read_binary_header(fp, &nm, &nnz, nd);
+    fclose(fp);
+    if (rc) return rc;
+    if (nm < 2) SPTOL_CHECK(SPTERR_VALUE_ERROR, "Serve", "nmodes must be in [2, 8]");
+    Request *r = new Request();
+    r->nmodes = nm;
+    r->nnz = nnz;
+    for (uint32_t m = 0; m < nm; ++m) r->ndims[m] = nd[m];
+    r->path = path;
     r->R = rank;
     r->niters = niters;
     r->tol = tol;

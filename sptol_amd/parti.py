@@ -73,6 +73,12 @@ class sptKruskalTensor(C.Structure):
                 ("factors", C.POINTER(C.POINTER(sptMatrix)))]
 
 
+class sptRankKruskalTensor(C.Structure):
+    _fields_ = [("nmodes", sptIndex), ("rank", sptElementIndex), ("ndims", C.POINTER(sptIndex)),
+                ("lambda_", C.POINTER(sptValue)), ("fit", C.c_double),
+                ("factors", C.POINTER(C.POINTER(sptRankMatrix)))]
+
+
 class sptolDevMatrix(C.Structure):
     _fields_ = [("nrows", sptIndex), ("ncols", sptIndex), ("stride", sptIndex),
                 ("dtype", C.c_int), ("values", C.c_void_p)]
@@ -281,6 +287,23 @@ def _declare(lib):
         "sptCpdAls": (C.c_int, [P(sptSparseTensor), sptIndex, sptIndex, C.c_double, P(sptKruskalTensor)]),
         "sptolCpdAlsDevice": (C.c_int, [C.c_void_p, sptIndex, sptIndex, C.c_double, P(sptolDevMatrix),
                                         P(sptValue), P(sptolCpdStats), C.c_void_p]),
+        "sptolCpdAlsHicooDevice": (C.c_int, [C.c_void_p, sptIndex, sptIndex, C.c_double, P(sptolDevMatrix),
+                                             P(sptValue), P(sptolCpdStats), C.c_void_p]),
+        "sptCpdAlsHiCOO": (C.c_int, [P(sptSparseTensorHiCOO), sptIndex, sptIndex, C.c_double,
+                                     P(sptRankKruskalTensor)]),
+        "sptolSparseTensorMergeDuplicates": (C.c_int, [P(sptSparseTensor), C.c_int]),
+        "sptolSaveSparseTensorBinary": (C.c_int, [P(sptSparseTensor), C.c_char_p]),
+        "sptolLoadSparseTensorBinary": (C.c_int, [P(sptSparseTensor), C.c_char_p]),
+        "sptFreeSparseTensor": (None, [P(sptSparseTensor)]),
+        "sptolDevCooLoadBinary": (C.c_int, [P(C.c_void_p), C.c_char_p, C.c_void_p]),
+        "sptolDevCooNnz": (sptNnzIndex, [C.c_void_p]),
+        "sptolServerSubmitFile": (C.c_int, [C.c_void_p, C.c_char_p, sptIndex, sptIndex, C.c_double, C.c_uint64,
+                                            C.c_int, P(C.c_uint64)]),
+        "sptNewRankKruskalTensor": (C.c_int, [P(sptRankKruskalTensor), sptIndex, P(sptIndex), sptElementIndex]),
+        "sptFreeRankKruskalTensor": (None, [P(sptRankKruskalTensor)]),
+        "sptRandomizeRankMatrix": (C.c_int, [P(sptRankMatrix), sptIndex, sptElementIndex]),
+        "sptNewRankMatrix": (C.c_int, [P(sptRankMatrix), sptIndex, sptElementIndex]),
+        "sptFreeRankMatrix": (None, [P(sptRankMatrix)]),
         "sptolBatchCreate": (C.c_int, [P(C.c_void_p)]),
         "sptolBatchAdd": (C.c_int, [C.c_void_p, C.c_void_p, P(sptolDevMatrix), P(sptIndex), sptIndex,
                                     P(sptolDevMatrix)]),

@@ -17,7 +17,8 @@ import oracle
 from conftest import rel_fro
 from sptol_amd import gen
 from sptol_amd.parti import (CooTensor, Matrix, SPTOL_BF16, SPTOL_F32, check, mats_array, ptr,
-                             sptKruskalTensor, sptIndex, sptolCpdStats, sptolDevMatrix)
+                             sptKruskalTensor, sptIndex, sptolCpdStats, sptolDevMatrix, sptRankKruskalTensor,
+                             sptRankMatrix, to_hicoo)
 
 pytestmark = pytest.mark.gpu
 
@@ -176,3 +177,88 @@ def test_block_cache_reuse(gpu_lib):
         assert np.array_equal(x.view(np.uint32), y.view(np.uint32))
     lib.sptolHipReleaseCache()
     assert lib.sptolHipMemoryCached() == 0
+
+
+@pytest.mark.parametrize("kind", ["tiled", "seg"])
+def test_sptcpdalshicoo_dropin(gpu_lib, kind):
+    """sptCpdAlsHiCOO (hicoo/cpd.c:149) on the device, over both view kinds:
+    the same ALS as sptCpdAls on the tensor in HiCOO order, against the FP64
+    oracle from the same seeded initial factors."""
+    from gpu_util import VIEW_KIND
+    lib = gpu_lib
+    d, inds, vals = gen.uniform_tensor([60, 50, 40], 8000, seed=33)
+    R, iters = 8, 6
+    coo = CooTensor(d, [a.copy() for a in inds], vals.copy())
+    h, _ = to_hicoo(lib, coo, 3, 5)            # permutes coo into the HiCOO order
+    lib.sptolSetSeed(4321)
+    kt = sptRankKruskalTensor()
+    nd = np.asarray(d, np.uint32)
+    check(lib, lib.sptNewRankKruskalTensor(C.byref(kt), 3, ptr(nd, sptIndex), R), "new rank ktensor")
+    check(lib, lib.sptolSetHicooViewKind(VIEW_KIND[kind]), "view kind")
+    try:
+        check(lib, lib.sptCpdAlsHiCOO(C.byref(h), R, iters, 0.0, C.byref(kt)), "sptCpdAlsHiCOO")
+    finally:
+        lib.sptolSetHicooViewKind(0)
+    lib.sptolSetSeed(4321)
+    init = []
+    for m in range(3):
+        rm = sptRankMatrix()
+        check(lib, lib.sptNewRankMatrix(C.byref(rm), int(d[m]), R), "new rank matrix")
+        check(lib, lib.sptRandomizeRankMatrix(C.byref(rm), int(d[m]), R), "randomize")
+        init.append(np.ctypeslib.as_array(rm.values, shape=(int(d[m]), rm.stride))[:, :R].copy())
+        lib.sptFreeRankMatrix(C.byref(rm))
+    om, olam, ofits = oracle.cpd_als(d, coo.inds, coo.vals, init, iters)
+    lam = np.ctypeslib.as_array(kt.lambda_, shape=(R,)).copy()
+    _say(f"hicoo cpd ({kind}): fit {kt.fit:.9f} oracle {ofits[-1]:.9f}")
+    assert rel_fro(lam, olam) < 1e-4
+    for m in range(3):
+        f = kt.factors[m].contents
+        got = np.ctypeslib.as_array(f.values, shape=(f.nrows, f.stride))[:, :R]
+        assert rel_fro(got, om[m]) < 1e-4
+    assert abs(kt.fit - ofits[-1]) < 1e-6
+    lib.sptFreeRankKruskalTensor(C.byref(kt))
+    lib.sptFreeSparseTensorHiCOO(C.byref(h))
+
+
+def test_hicoo_device_cpd_matches_coo(gpu_lib):
+    """sptolCpdAlsHicooDevice and sptolCpdAlsDevice on the same 4-D tensor
+    (HiCOO order) and initial factors reach the same fits, and both match
+    the FP64 oracle."""
+    lib = gpu_lib
+    d, inds, vals, _ = gen.planted_tensor([300, 250, 200, 40], 80000, 16, 1.05, seed=41)
+    coo = CooTensor(d, [a.copy() for a in inds], vals.copy())
+    h, _ = to_hicoo(lib, coo, 4, 8)
+    R, iters = 16, 8
+    init = gen.uniform_factors(d, R, seed=42)
+    out = {}
+    for kind in ("coo", "hicoo"):
+        dm = (sptolDevMatrix * 4)()
+        hm = [Matrix(a) for a in init]
+        for m in range(4):
+            check(lib, lib.sptolDevMatrixCreate(C.byref(dm[m]), int(d[m]), R, SPTOL_F32), "create")
+            check(lib, lib.sptolDevMatrixUpload(C.byref(dm[m]), C.byref(hm[m].m)), "upload")
+        lam = np.zeros(R, np.float32)
+        st = sptolCpdStats()
+        t = C.c_void_p()
+        if kind == "coo":
+            check(lib, lib.sptolDevCooCreate(C.byref(t), coo.ref()), "DevCooCreate")
+            check(lib, lib.sptolCpdAlsDevice(t, R, iters, -1.0, dm, ptr(lam, C.c_float), C.byref(st), None), "cpd")
+            lib.sptolDevCooFree(t)
+        else:
+            check(lib, lib.sptolDevHicooCreate(C.byref(t), C.byref(h)), "DevHicooCreate")
+            check(lib, lib.sptolCpdAlsHicooDevice(t, R, iters, -1.0, dm, ptr(lam, C.c_float), C.byref(st), None),
+                  "cpd hicoo")
+            lib.sptolDevHicooFree(t)
+        A = []
+        for m in range(4):
+            check(lib, lib.sptolDevMatrixDownload(C.byref(hm[m].m), C.byref(dm[m])), "download")
+            A.append(hm[m].array().copy())
+            lib.sptolDevMatrixFree(C.byref(dm[m]))
+        out[kind] = (A, lam, st.fit, st.iterations)
+    _, olam, ofits = oracle.cpd_als(d, coo.inds, coo.vals, init, iters)
+    _say(f"fit coo {out['coo'][2]:.9f} hicoo {out['hicoo'][2]:.9f} oracle {ofits[-1]:.9f}")
+    assert out["coo"][3] == out["hicoo"][3] == iters
+    assert abs(out["coo"][2] - out["hicoo"][2]) < 1e-5
+    assert abs(out["hicoo"][2] - ofits[-1]) < 1e-5
+    assert rel_fro(out["hicoo"][1], olam) < 1e-3
+    lib.sptFreeSparseTensorHiCOO(C.byref(h))

@@ -187,3 +187,66 @@ def test_mttkrp_shape_mismatch_is_reported(lib):
     A = gen.uniform_factors([20, 10, 5], 8, seed=2)
     mats = [Matrix(a) for a in A[:2]] + [Matrix(gen.uniform_factors([5], 4, seed=3)[0])] + [Matrix(nrows=20, ncols=8)]
     assert lib.sptMTTKRP(coo.ref(), mats_array(mats), ptr(order, sptIndex), 0) == SPTERR_SHAPE_MISMATCH
+
+
+def _tensor_arrays(t, nmodes):
+    n = int(t.nnz)
+    d = np.ctypeslib.as_array(t.ndims, shape=(nmodes,)).copy()
+    inds = [np.ctypeslib.as_array(t.inds[m].data, shape=(n,)).copy() if n else np.zeros(0, np.uint32)
+            for m in range(nmodes)]
+    vals = np.ctypeslib.as_array(t.values.data, shape=(n,)).copy() if n else np.zeros(0, np.float32)
+    return d, inds, vals
+
+
+def test_merge_duplicates(lib):
+    """Duplicated coordinates merge into one nonzero whose value is the FP64
+    sum of the run (in input order), rounded once; the result is sorted
+    lexicographically.  Checked against a numpy restatement (unique rows of
+    the stacked coordinates, float64 bincount of the values)."""
+    rng = np.random.default_rng(5)
+    for dims, n in (([40, 30, 20], 30000), ([7, 5, 3, 4], 2000), ([2**31, 3], 500)):
+        inds = [rng.integers(0, d, size=n).astype(np.uint32) for d in dims]
+        vals = rng.standard_normal(n).astype(np.float32)
+        coo = CooTensor(dims, [a.copy() for a in inds], vals.copy())
+        assert lib.sptolSparseTensorMergeDuplicates(coo.ref(), 4) == 0
+        m = int(coo.st.nnz)
+        keys = np.stack(inds[::-1])          # lexsort: last key is primary
+        order = np.lexsort(keys)
+        coords = np.stack(inds, axis=1)[order]
+        first = np.ones(n, bool)
+        first[1:] = np.any(coords[1:] != coords[:-1], axis=1)
+        grp = np.cumsum(first) - 1
+        want = np.bincount(grp, weights=vals[order].astype(np.float64)).astype(np.float32)
+        assert m == int(first.sum())
+        assert m < n or dims[0] == 2**31      # the wide case draws no duplicates
+        for q in range(len(dims)):
+            assert np.array_equal(coo.inds[q][:m], coords[first][:, q])
+        assert np.array_equal(coo.vals[:m].view(np.uint32), want.view(np.uint32))
+
+
+def test_binary_roundtrip_and_rejects(lib, tmp_path):
+    d, inds, vals = gen.zipf_tensor([5000, 300, 70000, 12], 50000, 1.1, seed=9)
+    coo = CooTensor(d, inds, vals)
+    path = str(tmp_path / "t.bin").encode()
+    assert lib.sptolSaveSparseTensorBinary(coo.ref(), path) == 0
+    t = sptSparseTensor()
+    assert lib.sptolLoadSparseTensorBinary(C.byref(t), path) == 0
+    d2, i2, v2 = _tensor_arrays(t, 4)
+    lib.sptFreeSparseTensor(C.byref(t))
+    assert np.array_equal(d2, d)
+    for m in range(4):
+        assert np.array_equal(i2[m], coo.inds[m])
+    assert np.array_equal(v2.view(np.uint32), coo.vals.view(np.uint32))
+    raw = open(path, "rb").read()
+    hdr_bytes = (24 + 4 * 4 + 7) // 8 * 8
+    assert raw[:8] == b"SPTOLBT1" and len(raw) == hdr_bytes + 5 * 4 * len(vals)
+    # truncated file, bad magic, out-of-range index
+    bad = str(tmp_path / "bad.bin").encode()
+    open(bad, "wb").write(raw[:-4])
+    assert lib.sptolLoadSparseTensorBinary(C.byref(sptSparseTensor()), bad) == SPTERR_VALUE_ERROR
+    open(bad, "wb").write(b"XXXXXXXX" + raw[8:])
+    assert lib.sptolLoadSparseTensorBinary(C.byref(sptSparseTensor()), bad) == SPTERR_VALUE_ERROR
+    hdr = bytearray(raw)
+    hdr[hdr_bytes:hdr_bytes + 4] = np.uint32(d[0]).tobytes()    # inds[0][0] = ndims[0]
+    open(bad, "wb").write(bytes(hdr))
+    assert lib.sptolLoadSparseTensorBinary(C.byref(sptSparseTensor()), bad) == SPTERR_VALUE_ERROR

@@ -41,6 +41,7 @@ def lib() -> C.CDLL:
         L.oracle_mttkrp_coo_f32.argtypes = common + [f32p]
         L.oracle_mttkrp_coo_f64.argtypes = common + [P(C.c_double)]
         L.oracle_mttkrp_coo_omp_f32.argtypes = common + [f32p, C.c_int]
+        L.oracle_mttkrp_coo_omp_reduce_f32.argtypes = common + [f32p, C.c_int]
         L.oracle_sort_lex.argtypes = [C.c_uint32, C.c_uint64, P(u32p), f32p]
         L.oracle_morton_cmp.argtypes = [C.c_uint32, u32p, u32p]
         L.oracle_hicoo_convert.argtypes = [C.c_uint32, u32p, C.c_uint64, P(u32p), f32p, C.c_uint32,
@@ -50,6 +51,7 @@ def lib() -> C.CDLL:
         L.oracle_mttkrp_hicoo_f32.argtypes = hc + [f32p]
         L.oracle_mttkrp_hicoo_f64.argtypes = hc + [P(C.c_double)]
         L.oracle_mttkrp_hicoo_sched_omp_f32.argtypes = hc + [f32p, C.c_int]
+        L.oracle_mttkrp_hicoo_sched_reduce_omp_f32.argtypes = hc + [f32p, C.c_int]
         L.oracle_cpd_als_f64.argtypes = [C.c_uint32, u32p, C.c_uint64, P(u32p), f32p, C.c_uint32,
                                          C.c_uint32, C.c_double, P(P(C.c_double)), P(C.c_double),
                                          P(C.c_double)]
@@ -85,10 +87,11 @@ def _stride(R):
     return ((R - 1) // 8 + 1) * 8
 
 
-def mttkrp_coo(inds, vals, factors, order, mode, out_rows=None, dtype=np.float32, threads=0):
+def mttkrp_coo(inds, vals, factors, order, mode, out_rows=None, dtype=np.float32, threads=0, reduce=False):
     """Reference-order MTTKRP; returns (out_rows, stride) array (padding zero).
-    dtype float32 -> sequential FP32 restatement (threads=0) or the OpenMP
-    atomic restatement (threads>0); float64 -> FP64 oracle."""
+    dtype float32 -> sequential FP32 restatement (threads=0), the OpenMP
+    atomic restatement (threads>0) or, with reduce, the private-copy
+    restatement of sptOmpMTTKRP_Reduce; float64 -> FP64 oracle."""
     L = lib()
     N = len(inds)
     R = factors[0].shape[1]
@@ -103,6 +106,8 @@ def mttkrp_coo(inds, vals, factors, order, mode, out_rows=None, dtype=np.float32
             o.ctypes.data_as(P(C.c_uint32)), mode, rows]
     if dtype == np.float64:
         L.oracle_mttkrp_coo_f64(*args, out.ctypes.data_as(P(C.c_double)))
+    elif threads > 0 and reduce:
+        L.oracle_mttkrp_coo_omp_reduce_f32(*args, out.ctypes.data_as(P(C.c_float)), threads)
     elif threads > 0:
         L.oracle_mttkrp_coo_omp_f32(*args, out.ctypes.data_as(P(C.c_float)), threads)
     else:
@@ -169,7 +174,7 @@ class Hicoo:
         out["kschr"] = ks
         return out
 
-    def mttkrp(self, factors, order, mode, dtype=np.float32, threads=0):
+    def mttkrp(self, factors, order, mode, dtype=np.float32, threads=0, reduce=False):
         L = lib()
         R = factors[0].shape[1]
         S = _stride(R)
@@ -180,6 +185,8 @@ class Hicoo:
         args = [C.byref(self.h), _f32pp(pf), S, R, o.ctypes.data_as(P(C.c_uint32)), mode, rows]
         if dtype == np.float64:
             L.oracle_mttkrp_hicoo_f64(*args, out.ctypes.data_as(P(C.c_double)))
+        elif threads > 0 and reduce:
+            L.oracle_mttkrp_hicoo_sched_reduce_omp_f32(*args, out.ctypes.data_as(P(C.c_float)), threads)
         elif threads > 0:
             L.oracle_mttkrp_hicoo_sched_omp_f32(*args, out.ctypes.data_as(P(C.c_float)), threads)
         else:

@@ -96,6 +96,47 @@ void oracle_mttkrp_coo_omp_f32(uint32_t nmodes, uint64_t nnz, const uint32_t *co
     }
 }
 
+/* src/sptensor/mttkrp_omp.c:401-563 (sptOmpMTTKRP_Reduce and its 3-D form):
+ * the nonzeros split statically over tk threads, each adding into its own
+ * copy of the output, then every output row summed over the copies in
+ * thread order.  With the same OpenMP runtime and tk the partition is the
+ * reference's, so the result is bit-identical to it. */
+void oracle_mttkrp_coo_omp_reduce_f32(uint32_t nmodes, uint64_t nnz, const uint32_t *const *inds,
+                                      const float *vals, const float *const *mats, uint32_t stride,
+                                      uint32_t R, const uint32_t *mats_order, uint32_t mode,
+                                      uint32_t out_rows, float *out, int tk)
+{
+    const size_t per = (size_t)out_rows * stride;
+    float *copies = (float *)calloc((size_t)tk * per + 1, sizeof(float));
+    memset(out, 0, per * sizeof(float));
+    #pragma omp parallel num_threads(tk)
+    {
+        float *s = (float *)malloc((R ? R : 1) * sizeof(float));
+        float *mine = copies + (size_t)omp_get_thread_num() * per;
+        #pragma omp for schedule(static)
+        for (uint64_t x = 0; x < nnz; ++x) {
+            uint32_t m1 = mats_order[1];
+            const float *row = mats[m1] + (size_t)inds[m1][x] * stride;
+            float v = vals[x];
+            for (uint32_t r = 0; r < R; ++r) s[r] = v * row[r];
+            for (uint32_t i = 2; i < nmodes; ++i) {
+                uint32_t mi = mats_order[i];
+                const float *rw = mats[mi] + (size_t)inds[mi][x] * stride;
+                for (uint32_t r = 0; r < R; ++r) s[r] *= rw[r];
+            }
+            float *o = mine + (size_t)inds[mode][x] * stride;
+            for (uint32_t r = 0; r < R; ++r) o[r] += s[r];
+        }
+        free(s);
+    }
+    #pragma omp parallel for schedule(static) num_threads(tk)
+    for (uint32_t i = 0; i < out_rows; ++i)
+        for (int t = 0; t < tk; ++t)
+            for (uint32_t r = 0; r < R; ++r)
+                out[(size_t)i * stride + r] += copies[(size_t)t * per + (size_t)i * stride + r];
+    free(copies);
+}
+
 /* ------------------------------------------------------------------------ */
 /* Orders                                                                   */
 /* ------------------------------------------------------------------------ */
@@ -425,6 +466,62 @@ void oracle_mttkrp_hicoo_sched_omp_f32(const oracle_hicoo *h, const float *const
             free(s);
         }
     }
+}
+
+/* src/sptensor/hicoo/mttkrp_omp.c:1214-1503 (Scheduled_Reduce): iteration i
+ * of the kernel schedule is the parallel loop (dynamic, chunk 1); it walks
+ * the i-th kernel of every superblock row of the output mode, adding into
+ * the running thread's own copy of the output; the copies are then summed
+ * in thread order.  Which thread takes an iteration is up to the runtime,
+ * so results agree with the sequential kernel up to the order of additions. */
+void oracle_mttkrp_hicoo_sched_reduce_omp_f32(const oracle_hicoo *h, const float *const *mats,
+                                              uint32_t stride, uint32_t R, const uint32_t *mats_order,
+                                              uint32_t mode, uint32_t out_rows, float *out, int tk)
+{
+    uint32_t N = h->nmodes;
+    const size_t per = (size_t)out_rows * stride;
+    float *copies = (float *)calloc((size_t)tk * per + 1, sizeof(float));
+    memset(out, 0, per * sizeof(float));
+    uint64_t poff = 0;
+    for (uint32_t m = 0; m < mode; ++m) poff += h->nkd[m] + 1;
+    const uint64_t *kptr_m = h->kschr_ptr + poff;
+    const uint32_t *kidx_m = h->kschr_idx + (uint64_t)mode * h->nk;
+    uint32_t nkd = h->nkd[mode];
+    #pragma omp parallel num_threads(tk)
+    {
+        float *s = (float *)malloc((R ? R : 1) * sizeof(float));
+        float *mine = copies + (size_t)omp_get_thread_num() * per;
+        uint32_t c[8];
+        #pragma omp for schedule(dynamic, 1)
+        for (uint32_t it = 0; it < h->nkiters[mode]; ++it)
+            for (uint32_t kr = 0; kr < nkd; ++kr) {
+                if (kptr_m[kr] + it >= kptr_m[kr + 1]) continue;
+                uint64_t k = kidx_m[kptr_m[kr] + it];
+                for (uint64_t b = h->kptr[k]; b < h->kptr[k + 1]; ++b)
+                    for (uint64_t z = h->bptr[b]; z < h->bptr[b + 1]; ++z) {
+                        for (uint32_t m = 0; m < N; ++m)
+                            c[m] = (h->binds[(uint64_t)m * h->nb + b] << h->sb_bits) + h->einds[(uint64_t)m * h->nnz + z];
+                        uint32_t m1 = mats_order[1];
+                        const float *row = mats[m1] + (size_t)c[m1] * stride;
+                        float v = h->values[z];
+                        for (uint32_t r = 0; r < R; ++r) s[r] = v * row[r];
+                        for (uint32_t i = 2; i < N; ++i) {
+                            uint32_t mi = mats_order[i];
+                            const float *rw = mats[mi] + (size_t)c[mi] * stride;
+                            for (uint32_t r = 0; r < R; ++r) s[r] *= rw[r];
+                        }
+                        float *o = mine + (size_t)c[mode] * stride;
+                        for (uint32_t r = 0; r < R; ++r) o[r] += s[r];
+                    }
+            }
+        free(s);
+    }
+    #pragma omp parallel for schedule(static) num_threads(tk)
+    for (uint32_t i = 0; i < out_rows; ++i)
+        for (int t = 0; t < tk; ++t)
+            for (uint32_t r = 0; r < R; ++r)
+                out[(size_t)i * stride + r] += copies[(size_t)t * per + (size_t)i * stride + r];
+    free(copies);
 }
 
 /* ------------------------------------------------------------------------ */

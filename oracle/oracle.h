@@ -24,6 +24,9 @@
  *                             LAPACK, tolerance); exact low-rank recovery
  *                             (fit -> 1); fit closed form on a dense tensor
  *   oracle_*_omp_*            equal (tolerance) to the sequential functions
+ *   oracle_mttkrp_coo_omp_reduce_f32
+ *                             bit-exact vs the compiled reference
+ *                             sptOmpMTTKRP_Reduce with the same thread count
  */
 #ifndef SPTOL_ORACLE_H
 #define SPTOL_ORACLE_H
@@ -48,6 +51,12 @@ void oracle_mttkrp_coo_omp_f32(uint32_t nmodes, uint64_t nnz, const uint32_t *co
                                const float *vals, const float *const *mats, uint32_t stride,
                                uint32_t R, const uint32_t *mats_order, uint32_t mode,
                                uint32_t out_rows, float *out, int tk);
+/* OpenMP restatement of sptOmpMTTKRP_Reduce (static partition, private
+ * copies summed in thread order: bit-identical to the reference per tk). */
+void oracle_mttkrp_coo_omp_reduce_f32(uint32_t nmodes, uint64_t nnz, const uint32_t *const *inds,
+                                      const float *vals, const float *const *mats, uint32_t stride,
+                                      uint32_t R, const uint32_t *mats_order, uint32_t mode,
+                                      uint32_t out_rows, float *out, int tk);
 
 /* Lexicographic COO sort (sptSparseTensorSortIndex), in place. */
 void oracle_sort_lex(uint32_t nmodes, uint64_t nnz, uint32_t **inds, float *vals);
@@ -88,6 +97,10 @@ void oracle_mttkrp_hicoo_f64(const oracle_hicoo *h, const float *const *mats, ui
 void oracle_mttkrp_hicoo_sched_omp_f32(const oracle_hicoo *h, const float *const *mats,
                                        uint32_t stride, uint32_t R, const uint32_t *mats_order,
                                        uint32_t mode, uint32_t out_rows, float *out, int tk);
+/* Scheduled_Reduce restatement (thread-private copies, ordered sum). */
+void oracle_mttkrp_hicoo_sched_reduce_omp_f32(const oracle_hicoo *h, const float *const *mats,
+                                              uint32_t stride, uint32_t R, const uint32_t *mats_order,
+                                              uint32_t mode, uint32_t out_rows, float *out, int tk);
 
 /* CPD-ALS in double precision with the reference's update rules and a
  * correct fit.  mats[m] (ndims[m] x R, row stride R) hold the initial factors

@@ -48,6 +48,11 @@ def golden_hicoo():
 
 
 @pytest.fixture(scope="session")
+def golden_omp_reduce():
+    return Golden(os.path.join(GOLDEN, "coo_omp_reduce.npz"))
+
+
+@pytest.fixture(scope="session")
 def golden_cpd():
     return Golden(os.path.join(GOLDEN, "cpd.npz"))
 

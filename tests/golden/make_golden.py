@@ -13,6 +13,10 @@ Fixtures (numpy .npz, no pickles) hold inputs and the reference's outputs:
                   outputs on every mode
   cpd.npz         CpdAlsStep (src/sptensor/cpd.c:32) factors and lambda after
                   a fixed number of iterations from committed initial factors
+  coo_omp_reduce.npz
+                  sptOmpMTTKRP_Reduce (src/sptensor/mttkrp_omp.c:401) with 4
+                  threads on the COO cases, every mode (its static partition
+                  makes the result a function of the thread count)
   *.dump.txt      sptDumpSparseTensorHiCOO text for two small cases
   tns_*.npz       tensors read by sptLoadSparseTensor from the reference's
                   tensors/*.tns fixtures
@@ -125,6 +129,36 @@ def write_coo(cases):
     np.savez_compressed(os.path.join(HERE, "coo_mttkrp.npz"), **out)
 
 
+OMP_TK = 4
+
+
+def write_omp_reduce(cases):
+    P = C.POINTER
+    ref.sptOmpMTTKRP_Reduce.argtypes = [P(sptSparseTensor), P(P(sptMatrix)), P(P(sptMatrix)), P(sptIndex),
+                                        sptIndex, C.c_int]
+    out = {"tk": np.int64(OMP_TK)}
+    for name, (d, inds, vals, R) in cases.items():
+        N = len(d)
+        factors = gen.uniform_factors(d, R, seed=300 + len(name))
+        out[f"{name}/ndims"] = d
+        out[f"{name}/vals"] = vals
+        out[f"{name}/R"] = np.int64(R)
+        for m in range(N):
+            out[f"{name}/ind{m}"] = inds[m]
+            out[f"{name}/A{m}"] = factors[m]
+        for mode in range(N):
+            o = np.asarray(orders(N, mode)[0], dtype=np.uint32)
+            coo = CooTensor(d, inds, vals)
+            mats = [Matrix(f) for f in factors]
+            res = Matrix(nrows=int(d[mode]), ncols=R, cap=int(max(d)))
+            copies = [Matrix(nrows=int(d[mode]), ncols=R) for _ in range(OMP_TK)]
+            assert ref.sptOmpMTTKRP_Reduce(coo.ref(), mats_array(mats + [res]), mats_array(copies),
+                                           ptr(o, sptIndex), mode, OMP_TK) == 0
+            out[f"{name}/order{mode}"] = o
+            out[f"{name}/out{mode}"] = res.buf[: int(d[mode])].copy()
+    np.savez_compressed(os.path.join(HERE, "coo_omp_reduce.npz"), **out)
+
+
 def ref_hicoo(lib, d, inds, vals, sb, sk, dump_path=None):
     coo = CooTensor(d, [a.copy() for a in inds], vals.copy())
     h = sptSparseTensorHiCOO()
@@ -220,10 +254,18 @@ def write_tns():
 
 
 if __name__ == "__main__":
+    # optional arguments name the fixtures to write (default: all)
+    want = set(sys.argv[1:]) or {"tns", "coo", "hicoo", "cpd", "omp_reduce"}
     ref = ref_lib()
     cases = coo_cases()
-    write_tns()
-    write_coo(cases)
-    write_hicoo(cases)
-    write_cpd(cases)
+    if "tns" in want:
+        write_tns()
+    if "coo" in want:
+        write_coo(cases)
+    if "hicoo" in want:
+        write_hicoo(cases)
+    if "cpd" in want:
+        write_cpd(cases)
+    if "omp_reduce" in want:
+        write_omp_reduce(cases)
     print("golden fixtures written to", HERE)

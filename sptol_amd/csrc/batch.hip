@@ -169,11 +169,9 @@ __global__ __launch_bounds__(kBatchWPB * 64) void k_batch_seg(const SegArgs *__r
 {
     __shared__ SegShared<LPN, NM, kBatchWPB> sh;
     const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;
-    for (;;) {
-        uint32_t wq = 0;
-        if (lane == 0) wq = atomicAdd(counter, 1u);
-        const uint64_t t = __builtin_amdgcn_readfirstlane(wq);
-        if (t >= total) break;
+    // first range by wave index, then W, W+1, ... from the class counter
+    const uint64_t W = (uint64_t)gridDim.x * kBatchWPB;
+    for (uint64_t t = (uint64_t)blockIdx.x * kBatchWPB + wv; t < total;) {
         uint32_t lo = 0, hi = n;   // prefix[lo] <= t < prefix[hi]
         while (hi - lo > 1) {
             const uint32_t mid = (lo + hi) >> 1;
@@ -181,6 +179,10 @@ __global__ __launch_bounds__(kBatchWPB * 64) void k_batch_seg(const SegArgs *__r
         }
         const SegArgs a = items[lo];
         seg_range<LPN, NM, FT, kBatchWPB>(a, t - prefix[lo], sh, wv, lane);
+        if (W >= total) break;   // every range was taken by index
+        uint32_t wq = 0;
+        if (lane == 0) wq = atomicAdd(counter, 1u);
+        t = W + __builtin_amdgcn_readfirstlane(wq);
     }
 }
 

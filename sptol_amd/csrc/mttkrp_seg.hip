@@ -1,9 +1,11 @@
 // Segmented MTTKRP of one tensor: the mode-sorted record view, its build,
 // and the launches (range body and carry pass in seg_kernel.hpp).
 //
-// The main kernel is persistent: each wave takes ranges of nonzeros from a
-// work counter until they run out, so ranges of uneven cost balance; the
-// carry kernel that always follows resets the counter for the next launch.
+// The main kernel is persistent: each wave takes its first range of
+// nonzeros by its own index and the following ones from a work counter
+// until they run out, so ranges of uneven cost balance without a burst of
+// atomics at the start; the carry kernel that always follows resets the
+// counter for the next launch.
 // Calls on one view must be ordered (one stream): they share the view's
 // counter and carry buffers.
 #include "seg_kernel.hpp"
@@ -39,12 +41,17 @@ void k_mttkrp_seg(const SegArgs a, uint32_t *__restrict__ counter)
 {
     __shared__ SegShared<LPN, NM, kSegWPB> sh;
     const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;
-    for (;;) {
+    // Wave w of the W in the grid takes range w first, then pulls ranges
+    // W, W+1, ... from the counter: no atomics while every wave starts, none
+    // at all in a launch of at most W ranges, balancing after that.
+    const uint64_t W = (uint64_t)gridDim.x * kSegWPB;
+    uint64_t wave = (uint64_t)blockIdx.x * kSegWPB + wv;
+    while (wave < a.nwaves) {
+        seg_range<LPN, NM, FT, kSegWPB>(a, wave, sh, wv, lane);
+        if (W >= a.nwaves) break;   // every range was taken by index
         uint32_t wq = 0;
         if (lane == 0) wq = atomicAdd(counter, 1u);
-        const uint64_t wave = __builtin_amdgcn_readfirstlane(wq);
-        if (wave >= a.nwaves) break;
-        seg_range<LPN, NM, FT, kSegWPB>(a, wave, sh, wv, lane);
+        wave = W + __builtin_amdgcn_readfirstlane(wq);
     }
 }
 
@@ -181,7 +188,7 @@ int seg_build(SegView &v, uint32_t nmodes, uint32_t mode, uint32_t nrows, uint64
 }
 
 // Nonzeros per lane group for a tensor of nnz nonzeros: about 32 ranges per
-// compute unit, between 64 and 256 (SPTOL_SEG_CHG overrides, for tuning).
+// compute unit, between 16 and 256 (SPTOL_SEG_CHG overrides, for tuning).
 uint32_t seg_chg(uint64_t nnz, uint32_t G)
 {
     static const int env = env_int("SPTOL_SEG_CHG");
@@ -191,7 +198,7 @@ uint32_t seg_chg(uint64_t nnz, uint32_t G)
     } else {
         const uint64_t target = (uint64_t)num_cus() * 32;
         c = (nnz + G * target - 1) / (G * target);
-        if (c < 64) c = 64;
+        if (c < 16) c = 16;
         if (c > 256) c = 256;
     }
     c = (c + 7) / 8 * 8;

@@ -33,7 +33,7 @@ sys.path.insert(0, ROOT)
 
 from sptol_amd import gen  # noqa: E402
 from sptol_amd.parti import (CooTensor, Matrix, SPTOL_BF16, SPTOL_F32, check, load_library, ptr,  # noqa: E402
-                             sptIndex, sptolDevMatrix, to_hicoo)
+                             sptIndex, sptSparseTensor, sptolDevMatrix, to_hicoo)
 
 HBM_PEAK = 8.0e12
 DIMS = [12_000_000, 9_000_000, 28_000_000]
@@ -379,30 +379,108 @@ def run_coo(args, lib):
                        "algorithmic_bytes_per_launch": B, "carry_kernel_avg_ms": round(t["mttkrp_seg_carry_ms"], 5),
                        "fill_zero_avg_ms": round(t["fill_zero_ms"], 5)}
     if not args.no_cpu:
-        res["cpu_baseline"] = cpu_baseline_coo(inds, vals, factors, order)
+        res["cpu_baseline"] = cpu_baseline_coo(dims, inds, vals, factors, order)
     lib.sptolDevMatrixFree(C.byref(out))
     free_factors(lib, dm)
     lib.sptolDevCooFree(h)
     return res
 
 
-def cpu_baseline_coo(inds, vals, factors, order):
+def ref_library():
+    """The reference library compiled from its sources into oracle/_ref, or
+    None where it was not built."""
+    try:
+        return C.CDLL(os.path.join(ROOT, "oracle", "_ref", "libparti_ref.so"))
+    except OSError:
+        return None
+
+
+class quiet_stdout:
+    """Send what C code prints to stdout to /dev/null (the reference's
+    drivers print per call; this program's stdout is its one JSON line)."""
+
+    def __enter__(self):
+        sys.stdout.flush()
+        self.libc = C.CDLL(None)
+        self.libc.fflush(None)
+        self.saved = os.dup(1)
+        nul = os.open(os.devnull, os.O_WRONLY)
+        os.dup2(nul, 1)
+        os.close(nul)
+        return self
+
+    def __exit__(self, *exc):
+        self.libc.fflush(None)
+        os.dup2(self.saved, 1)
+        os.close(self.saved)
+        return False
+
+
+def host_threads():
+    ncpu = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    return max(1, min(16, ncpu))
+
+
+def cpu_baseline_coo(dims, inds, vals, factors, order):
     """The reference's OpenMP COO MTTKRP sptOmpMTTKRP (mttkrp_omp.c:125) from
     oracle/_ref, one warm-up and the mean of 5 calls (examples/mttkrp.c:240-283);
     the oracle's restatement when oracle/_ref is absent."""
-    ncpu = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
-    tk = max(1, min(16, ncpu))
-    import oracle
-    ts = []
-    for k in range(6):
-        t0 = time.perf_counter()
-        oracle.mttkrp_coo(inds, vals, factors, order, 0, threads=tk)
-        if k:
-            ts.append(time.perf_counter() - t0)
+    tk = host_threads()
+    ref = ref_library()
     nnz = len(vals)
-    return {"value": nnz / float(np.mean(ts)) / 1e9, "unit": "Gnnz/s", "cores": tk, "kind": "port",
-            "sample": f"oracle restatement of sptOmpMTTKRP (atomic, mttkrp_omp.c:125), tk={tk}, the full "
-                      f"{nnz}-nonzero tensor, mean of 5 calls after one warm-up"}
+    ts = []
+    if ref is not None:
+        from sptol_amd.parti import mats_array
+        coo = CooTensor(dims, inds, vals)
+        mats = [Matrix(f) for f in factors] + [Matrix(nrows=int(dims[0]), ncols=factors[0].shape[1],
+                                                      cap=int(max(dims)))]
+        arr = mats_array(mats)
+        o = np.ascontiguousarray(order, dtype=np.uint32)
+        ref.sptOmpMTTKRP.argtypes = [C.POINTER(sptSparseTensor), C.c_void_p, C.POINTER(sptIndex), sptIndex, C.c_int]
+        with quiet_stdout():
+            for k in range(6):
+                t0 = time.perf_counter()
+                rc = ref.sptOmpMTTKRP(coo.ref(), C.cast(arr, C.c_void_p), ptr(o, sptIndex), 0, tk)
+                if k:
+                    ts.append(time.perf_counter() - t0)
+                assert rc == 0
+        what, kind = "the compiled reference's sptOmpMTTKRP (atomic, mttkrp_omp.c:125)", "reference"
+    else:
+        import oracle
+        for k in range(6):
+            t0 = time.perf_counter()
+            oracle.mttkrp_coo(inds, vals, factors, order, 0, threads=tk)
+            if k:
+                ts.append(time.perf_counter() - t0)
+        what, kind = "oracle restatement of sptOmpMTTKRP (atomic, mttkrp_omp.c:125)", "port"
+    return {"value": nnz / float(np.mean(ts)) / 1e9, "unit": "Gnnz/s", "cores": tk, "kind": kind,
+            "sample": f"{what}, tk={tk}, the full {nnz}-nonzero tensor, mean of 5 calls after one warm-up"}
+
+
+def cpu_baseline_cpd(dims, inds, vals, init, R):
+    """One ALS iteration of the reference's OpenMP CPD (OmpCpdAlsStep,
+    cpd_omp.c:32, atomic MTTKRP, FP32 LAPACK solve) from oracle/_ref on the
+    full tensor, from the same initial factors; None without oracle/_ref."""
+    ref = ref_library()
+    if ref is None:
+        return None
+    from sptol_amd.parti import mats_array, sptValue
+    tk = host_threads()
+    coo = CooTensor(dims, inds, vals)
+    mats = [Matrix(a) for a in init] + [Matrix(nrows=int(max(dims)), ncols=R)]
+    lam = np.zeros(R, np.float32)
+    ref.OmpCpdAlsStep.argtypes = [C.POINTER(sptSparseTensor), sptIndex, sptIndex, C.c_double, C.c_int, C.c_int,
+                                  C.c_void_p, C.c_void_p, C.POINTER(sptValue)]
+    ref.OmpCpdAlsStep.restype = C.c_double
+    iters = 1
+    with quiet_stdout():
+        t0 = time.perf_counter()
+        ref.OmpCpdAlsStep(coo.ref(), R, iters, 0.0, tk, 0, C.cast(mats_array(mats), C.c_void_p), None,
+                          ptr(lam, C.c_float))
+        dt = time.perf_counter() - t0
+    return {"value": dt / iters, "unit": "s/iter", "cores": tk, "kind": "reference",
+            "sample": f"OmpCpdAlsStep (cpd_omp.c:32), tk={tk}, use_reduce=0, {iters} iteration on the full "
+                      f"{len(vals)}-nonzero tensor from the same initial factors, {dt:.1f} s of CPU time"}
 
 
 def run_cpd(args, lib):
@@ -446,6 +524,9 @@ def run_cpd(args, lib):
                        "zipf_s": 1.05, "planted_rank": R, "noise": 0.01}, higher=False)
     res["variants"] = out
     res["bf16_minus_f32_fit"] = out["bf16"]["fit"] - out["f32"]["fit"]
+    res["cpu_baseline"] = None if args.no_cpu else cpu_baseline_cpd(d, inds, vals, init, R)
+    if res["cpu_baseline"]:
+        log(f"cpd CPU baseline: {res['cpu_baseline']}")
     return res
 
 

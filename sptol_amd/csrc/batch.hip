@@ -162,8 +162,15 @@ __global__ __launch_bounds__(256) void k_batch_mttkrp(const BatchDesc *__restric
 // ---------------------------------------------------------------------------
 constexpr int kBatchWPB = 4;
 
+// register budget of the batched range kernels, in waves per SIMD: 4 (128
+// VGPRs) measured faster than the 3 the compiler would choose for 4-mode
+// tensors, spills included
+#ifndef SPTOL_BATCH_WAVES
+#define SPTOL_BATCH_WAVES 4
+#endif
+
 template <int LPN, int NM, typename FT>
-__global__ __launch_bounds__(kBatchWPB * 64) void k_batch_seg(const SegArgs *__restrict__ items,
+__global__ __launch_bounds__(kBatchWPB * 64) __attribute__((amdgpu_waves_per_eu(SPTOL_BATCH_WAVES, 8))) void k_batch_seg(const SegArgs *__restrict__ items,
                                                               const uint64_t *__restrict__ prefix, uint32_t n,
                                                               uint64_t total, uint32_t *__restrict__ counter)
 {

@@ -34,9 +34,12 @@ constexpr int kSegWPB = 4;
 #ifndef SPTOL_SEG_WAVES3
 #define SPTOL_SEG_WAVES3 4
 #endif
+#ifndef SPTOL_SEG_WAVES4
+#define SPTOL_SEG_WAVES4 4
+#endif
 
 template <int LPN, int NM, typename FT>
-__global__ __launch_bounds__(kSegWPB * 64) __attribute__((amdgpu_waves_per_eu(NM == 3 ? SPTOL_SEG_WAVES3 : 4, 8)))
+__global__ __launch_bounds__(kSegWPB * 64) __attribute__((amdgpu_waves_per_eu(NM == 3 ? SPTOL_SEG_WAVES3 : SPTOL_SEG_WAVES4, 8)))
 void k_mttkrp_seg(const SegArgs a, uint32_t *__restrict__ counter)
 {
     __shared__ SegShared<LPN, NM, kSegWPB> sh;

@@ -410,6 +410,9 @@ int sptolHipStreamCreate(void **stream)
 
 int sptolHipStreamDestroy(void *stream)
 {
+    // drain first: cached blocks tagged with this stream must not meet a
+    // new stream at the same address while the old work is still pending
+    if (stream) SPTOL_HIP(hipStreamSynchronize((hipStream_t)stream), "HipWrap StreamDestroy");
     if (stream) SPTOL_HIP(hipStreamDestroy((hipStream_t)stream), "HipWrap StreamDestroy");
     return 0;
 }

@@ -100,7 +100,7 @@ struct SegView {
     float *carry = nullptr;        // per wave: partial row carried into an earlier wave's row
     uint32_t *carry_row = nullptr; // per wave: that row, or kNoRow
     uint64_t carry_waves = 0;      // capacity in waves (64 floats each)
-    uint32_t *counter = nullptr;   // work counter of the persistent kernel (0 between launches)
+    uint32_t *counter = nullptr;   // work counter of the persistent kernel (0 between launches), after touched
     bool ready = false;
     void release();
 };
@@ -190,7 +190,7 @@ void prof_stop(int token, hipStream_t s);
 // Allocate + copy on stream s (synchronises s).
 int dup_on(void **dst, const void *src, size_t bytes, hipStream_t s, const char *module);
 // Stream-aware forms of sptolDevCooCreate / sptolDevCooPrepareMode.
-int coo_create(sptolDevCoo *h, const sptSparseTensor *X, hipStream_t s);
+int coo_create(sptolDevCoo *h, const sptSparseTensor *X, hipStream_t s, bool check_vals = false);
 int coo_prepare(sptolDevCoo h, uint32_t mode, hipStream_t s);
 // Every mode (3- and 4-mode tensors pack their records once and gather the
 // views from them).
@@ -201,11 +201,9 @@ int hicoo_mttkrp(sptolDevHicoo X, const sptolDevMatrix *mats, const uint32_t *or
                  sptolDevMatrix *out, hipStream_t s);
 int hicoo_prepare(sptolDevHicoo h, uint32_t mode, hipStream_t s);
 // Device COO tensor straight from a binary tensor file (the layout of
-// sptolSaveSparseTensorBinary), streamed through pinned chunks; indices are
-// checked on the device.  Synchronises s.
-int coo_load_binary(sptolDevCoo *h, const char *path, hipStream_t s);
-// *finite = no NaN or infinity among vals[0..n) (synchronises s).
-int check_finite(const float *vals, uint64_t n, hipStream_t s, bool *finite);
+// sptolSaveSparseTensorBinary), streamed through pinned chunks; indices (and
+// with check_vals, values) are checked on the device.  Synchronises s.
+int coo_load_binary(sptolDevCoo *h, const char *path, hipStream_t s, bool check_vals = false);
 // Device MTTKRP on stream s (sptolDevMTTKRP without the argument checks).
 int coo_mttkrp(sptolDevCoo X, const sptolDevMatrix *mats, const uint32_t *order, uint32_t mode,
                sptolDevMatrix *out, hipStream_t s);

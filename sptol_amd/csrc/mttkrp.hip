@@ -476,66 +476,56 @@ int dup_on(void **dst, const void *src, size_t bytes, hipStream_t s, const char 
     return 0;
 }
 
-// bad[0] |= 1 when some index of mode m is >= ndims[m]
-__global__ void k_check_inds(const uint32_t *__restrict__ idx, uint64_t n, uint32_t lim, uint32_t *__restrict__ bad)
+
+struct CheckArgs {
+    const uint32_t *inds[kMaxModes];
+    uint32_t ndims[kMaxModes];
+    const float *vals;     // null: values not checked
+    uint64_t nnz;
+    uint32_t nmodes;
+};
+
+// bad[0] |= 1 for an index >= ndims, |= 2 for a NaN or infinite value: one
+// pass over every array of the tensor.
+__global__ void k_check_tensor(const CheckArgs a, uint32_t *__restrict__ bad)
 {
     uint32_t b = 0;
-    for (uint64_t z = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; z < n; z += (uint64_t)gridDim.x * blockDim.x)
-        b |= idx[z] >= lim ? 1u : 0u;
-    if (b) atomicOr(bad, 1u);
-}
-
-// bad[0] |= 1 when some value is NaN or infinite
-__global__ void k_check_finite(const float *__restrict__ v, uint64_t n, uint32_t *__restrict__ bad)
-{
-    uint32_t b = 0;
-    for (uint64_t z = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; z < n; z += (uint64_t)gridDim.x * blockDim.x)
-        b |= (__float_as_uint(v[z]) & 0x7F800000u) == 0x7F800000u ? 1u : 0u;
-    if (b) atomicOr(bad, 1u);
-}
-
-int check_finite(const float *vals, uint64_t n, hipStream_t s, bool *finite)
-{
-    *finite = true;
-    if (!n) return 0;
-    StreamScope scope(s);
-    uint32_t *bad = nullptr;
-    SPTOL_TRY(dmalloc((void **)&bad, 4, "Check finite"));
-    uint32_t hb = 0;
-    hipError_t e = hipMemsetAsync(bad, 0, 4, s);
-    if (e == hipSuccess) {
-        hipLaunchKernelGGL(k_check_finite, dim3(grid_for(n)), dim3(256), 0, s, vals, n, bad);
-        e = hipGetLastError();
+    for (uint64_t z = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; z < a.nnz; z += (uint64_t)gridDim.x * blockDim.x) {
+        for (uint32_t m = 0; m < a.nmodes; ++m) b |= a.inds[m][z] >= a.ndims[m] ? 1u : 0u;
+        if (a.vals) b |= (__float_as_uint(a.vals[z]) & 0x7F800000u) == 0x7F800000u ? 2u : 0u;
     }
-    if (e == hipSuccess) e = hipMemcpyAsync(&hb, bad, 4, hipMemcpyDeviceToHost, s);
-    if (e == hipSuccess) e = hipStreamSynchronize(s);
-    dfree(bad);
-    SPTOL_HIP(e, "Check finite");
-    *finite = hb == 0;
-    return 0;
+    if (b) atomicOr(bad, b);
 }
 
-// Reject any index >= ndims on the device (t's arrays are on s).
-static int check_inds_dev(sptolDevCoo t, hipStream_t s, const char *module)
+// Reject any index >= ndims (and, with check_vals, any non-finite value)
+// on the device (t's arrays are on s).
+static int check_inds_dev(sptolDevCoo t, hipStream_t s, const char *module, bool check_vals = false)
 {
     if (!t->nnz) return 0;
+    CheckArgs a;
+    std::memset(&a, 0, sizeof(a));
+    for (uint32_t m = 0; m < t->nmodes; ++m) { a.inds[m] = t->inds[m]; a.ndims[m] = t->ndims[m]; }
+    a.vals = check_vals ? t->vals : nullptr;
+    a.nnz = t->nnz;
+    a.nmodes = t->nmodes;
     uint32_t *bad = nullptr;
     SPTOL_TRY(dmalloc((void **)&bad, 4, module));
     uint32_t hb = 0;
     hipError_t e = hipMemsetAsync(bad, 0, 4, s);
-    for (uint32_t m = 0; m < t->nmodes && e == hipSuccess; ++m) {
-        hipLaunchKernelGGL(k_check_inds, dim3(grid_for(t->nnz)), dim3(256), 0, s, t->inds[m], t->nnz, t->ndims[m], bad);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_check_tensor, dim3(grid_for(t->nnz)), dim3(256), 0, s, a, bad);
         e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipMemcpyAsync(&hb, bad, 4, hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     dfree(bad);
     SPTOL_HIP(e, module);
-    if (hb) SPTOL_CHECK(SPTERR_VALUE_ERROR, module, "index >= ndims");
+    if (hb & 1u) SPTOL_CHECK(SPTERR_VALUE_ERROR, module, "index >= ndims");
+    if (hb & 2u) SPTOL_CHECK(SPTERR_VALUE_ERROR, module, "non-finite value");
     return 0;
 }
 
-int coo_load_binary(sptolDevCoo *h, const char *path, hipStream_t s)
+int coo_load_binary(sptolDevCoo *h, const char *path, hipStream_t s, bool check_vals)
 {
     *h = nullptr;
     StreamScope scope(s);   // everything freed here was used on s only
@@ -595,13 +585,13 @@ int coo_load_binary(sptolDevCoo *h, const char *path, hipStream_t s)
         pinned_chunk_free(buf[q]);
         if (ev[q]) (void)hipEventDestroy(ev[q]);
     }
-    if (!r) r = check_inds_dev(t, s, "DevCoo Load");
+    if (!r) r = check_inds_dev(t, s, "DevCoo Load", check_vals);
     if (r) { sptolDevCooFree(t); return r; }
     *h = t;
     return 0;
 }
 
-int coo_create(sptolDevCoo *h, const sptSparseTensor *X, hipStream_t s)
+int coo_create(sptolDevCoo *h, const sptSparseTensor *X, hipStream_t s, bool check_vals)
 {
     *h = nullptr;
     StreamScope scope(s);   // everything freed here was used on s only
@@ -619,14 +609,17 @@ int coo_create(sptolDevCoo *h, const sptSparseTensor *X, hipStream_t s)
     // The kernels index factor rows with these values, so reject any index
     // >= ndims before any kernel uses them (the reference trusts its input;
     // here an out-of-range index would be a device fault).
-    if ((r = check_inds_dev(t, s, "DevCoo New"))) { sptolDevCooFree(t); return r; }
+    if ((r = check_inds_dev(t, s, "DevCoo New", check_vals))) { sptolDevCooFree(t); return r; }
     *h = t;
     return 0;
 }
 
-static int coo_prepare_from(sptolDevCoo h, uint32_t mode, hipStream_t s, const uint4 *rec0);
+static int coo_prepare_from(sptolDevCoo h, uint32_t mode, hipStream_t s, const uint4 *rec0, const uint32_t *iota);
 
-int coo_prepare(sptolDevCoo h, uint32_t mode, hipStream_t s) { return coo_prepare_from(h, mode, s, nullptr); }
+int coo_prepare(sptolDevCoo h, uint32_t mode, hipStream_t s)
+{
+    return coo_prepare_from(h, mode, s, nullptr, nullptr);
+}
 
 int coo_prepare_all(sptolDevCoo h, hipStream_t s)
 {
@@ -637,17 +630,22 @@ int coo_prepare_all(sptolDevCoo h, hipStream_t s)
         for (uint32_t m = 0; m < h->nmodes; ++m) SPTOL_TRY(coo_prepare(h, m, s));
         return 0;
     }
-    StreamScope scope(s);   // the packed records are used on s only
+    StreamScope scope(s);   // the packed records and 0..nnz-1 are used on s only
     uint4 *rec0 = nullptr;
-    SPTOL_TRY(dmalloc((void **)&rec0, h->nnz * sizeof(uint4), "DevCoo Prepare"));
-    int r = seg_pack_unsorted(rec0, h->nmodes, h->nnz, h->inds, h->vals, s);
-    for (uint32_t m = 0; m < h->nmodes && !r; ++m) r = coo_prepare_from(h, m, s, rec0);
+    uint32_t *iota = nullptr;
+    int r = dmalloc((void **)&rec0, h->nnz * sizeof(uint4), "DevCoo Prepare");
+    if (!r) r = dmalloc((void **)&iota, h->nnz * 4, "DevCoo Prepare");
+    if (!r) r = seg_pack_unsorted(rec0, h->nmodes, h->nnz, h->inds, h->vals, s);
+    if (!r) hipLaunchKernelGGL(k_iota, dim3(grid_for(h->nnz)), dim3(256), 0, s, iota, h->nnz);
+    for (uint32_t m = 0; m < h->nmodes && !r; ++m) r = coo_prepare_from(h, m, s, rec0, iota);
     (void)hipStreamSynchronize(s);
     dfree(rec0);
+    dfree(iota);
     return r;
 }
 
-static int coo_prepare_from(sptolDevCoo h, uint32_t mode, hipStream_t s, const uint4 *rec0)
+// iota: 0..nnz-1 on the device (the sort's input values), or null to make it
+static int coo_prepare_from(sptolDevCoo h, uint32_t mode, hipStream_t s, const uint4 *rec0, const uint32_t *iota)
 {
     if (mode >= h->nmodes) SPTOL_CHECK(SPTERR_VALUE_ERROR, "DevCoo Prepare", "mode out of range");
     SortedCopy &sc = h->sorted[mode];
@@ -660,17 +658,18 @@ static int coo_prepare_from(sptolDevCoo h, uint32_t mode, hipStream_t s, const u
     size_t tmp_bytes = 0;
     int r = 0;
     auto cleanup = [&]() { dfree(perm); dfree(perm2); dfree(keys2); dfree(tmp); };
-    if ((r = dmalloc((void **)&perm, n * 4, "DevCoo Prepare")) ||
+    if ((!iota && (r = dmalloc((void **)&perm, n * 4, "DevCoo Prepare"))) ||
         (r = dmalloc((void **)&perm2, n * 4, "DevCoo Prepare")) ||
         (r = dmalloc((void **)&keys2, n * 4, "DevCoo Prepare"))) { cleanup(); return r; }
-    hipLaunchKernelGGL(k_iota, dim3(grid_for(n)), dim3(256), 0, s, perm, n);
+    if (!iota) hipLaunchKernelGGL(k_iota, dim3(grid_for(n)), dim3(256), 0, s, perm, n);
+    const uint32_t *vin = iota ? iota : perm;
     int endbit = 1;
     while (endbit < 32 && (1ull << endbit) < h->ndims[mode]) ++endbit;
-    hipError_t e = hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, h->inds[mode], keys2, perm, perm2,
+    hipError_t e = hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, h->inds[mode], keys2, vin, perm2,
                                                       (int)n, 0, endbit, s);
     if (e == hipSuccess) {
         if ((r = dmalloc(&tmp, tmp_bytes ? tmp_bytes : 4, "DevCoo Prepare"))) { cleanup(); return r; }
-        e = hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, h->inds[mode], keys2, perm, perm2, (int)n, 0,
+        e = hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, h->inds[mode], keys2, vin, perm2, (int)n, 0,
                                                endbit, s);
     }
     if (e != hipSuccess) { cleanup(); SPTOL_HIP(e, "DevCoo Prepare"); }

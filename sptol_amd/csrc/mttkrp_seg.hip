@@ -21,11 +21,11 @@ void SegView::release()
     if (own_vals) dfree(vals);
     vals = nullptr;
     own_vals = false;
-    dfree(touched); touched = nullptr;
+    dfree(touched); touched = nullptr;   // the counter lives at its end
     dfree(carry); carry = nullptr;
     dfree(carry_row); carry_row = nullptr;
     carry_waves = 0;
-    dfree(counter); counter = nullptr;
+    counter = nullptr;
     ready = false;
 }
 
@@ -153,14 +153,14 @@ int seg_build(SegView &v, uint32_t nmodes, uint32_t mode, uint32_t nrows, uint64
     v.nnz = nnz;
     for (uint32_t m = 0; m < nmodes; ++m) v.ndims[m] = ndims[m];
     int r = 0;
+    // the row bitmap and, after it, the work counter: one allocation, one clear
     const uint64_t words = ((uint64_t)nrows + 31) / 32;
-    if ((r = dmalloc((void **)&v.touched, (words ? words : 1) * 4, "Seg view")) ||
-        (r = dmalloc((void **)&v.counter, 4, "Seg view"))) {
+    if ((r = dmalloc((void **)&v.touched, (words + 1) * 4, "Seg view"))) {
         v.release();
         return r;
     }
-    hipError_t e = hipMemsetAsync(v.touched, 0, (words ? words : 1) * 4, s);
-    if (e == hipSuccess) e = hipMemsetAsync(v.counter, 0, 4, s);
+    v.counter = v.touched + words;
+    hipError_t e = hipMemsetAsync(v.touched, 0, (words + 1) * 4, s);
     if (e != hipSuccess) { v.release(); SPTOL_HIP(e, "Seg view"); }
     if (nnz) {
         if ((r = dmalloc((void **)&v.rec, nnz * sizeof(uint4), "Seg view"))) { v.release(); return r; }

@@ -164,7 +164,7 @@ static int prepare_loaded(sptolServer srv, Request *r, hipStream_t s);
 static int prepare_request(sptolServer srv, Request *r, hipStream_t s)
 {
     if (!r->path.empty()) {
-        SPTOL_TRY(coo_load_binary(&r->X, r->path.c_str(), s));
+        SPTOL_TRY(coo_load_binary(&r->X, r->path.c_str(), s, true));
         bool same = r->X->nmodes == r->nmodes && r->X->nnz == r->nnz;
         for (uint32_t m = 0; m < r->nmodes && same; ++m) same = r->X->ndims[m] == r->ndims[m];
         if (!same) SPTOL_CHECK(SPTERR_VALUE_ERROR, "Serve", "tensor file changed after submission");
@@ -187,17 +187,14 @@ static int prepare_request(sptolServer srv, Request *r, hipStream_t s)
     X.inds = iv;
     X.values.len = X.values.cap = r->nnz;
     X.values.data = const_cast<float *>(r->src_vals);
-    SPTOL_TRY(coo_create(&r->X, &X, s));
+    SPTOL_TRY(coo_create(&r->X, &X, s, true));
     return prepare_loaded(srv, r, s);
 }
 
-// The request's tensor is on the device: check the values, prepare every
-// mode, create the factors and set up the job.
+// The request's tensor is on the device (indices and values checked):
+// prepare every mode, create the factors and set up the job.
 static int prepare_loaded(sptolServer srv, Request *r, hipStream_t s)
 {
-    bool finite = true;
-    SPTOL_TRY(check_finite(r->X->vals, r->nnz, s, &finite));
-    if (!finite) SPTOL_CHECK(SPTERR_VALUE_ERROR, "Serve", "non-finite value");
     SPTOL_TRY(coo_prepare_all(r->X, s));
     uint64_t st = r->seed;
     for (uint32_t m = 0; m < r->nmodes; ++m) {

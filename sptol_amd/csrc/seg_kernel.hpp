@@ -151,6 +151,10 @@ __device__ __forceinline__ void seg_range(const SegArgs &a, uint64_t wave, SegSh
         }
     };
     if (nonempty) load_recs(zs);
+    // this group's staged records, word by word
+    const uint32_t *stg = reinterpret_cast<const uint32_t *>(&sh.stage[wv][g][0]);
+    const uint32_t *stg_row = stg + mode, *stg_o1 = stg + o1, *stg_o2 = stg + o2;
+    const uint32_t *stg_o3 = stg + (NM == 4 ? o3 : 0u), *stg_w = stg + 3;
 
     for (uint64_t z = zs; z < ze; z += U) {
 #pragma unroll
@@ -162,22 +166,27 @@ __device__ __forceinline__ void seg_range(const SegArgs &a, uint64_t wave, SegSh
             }
         }
         seg_wave_sync();
-        uint4 rr[U];
+        // each word is read from LDS at its own (wave-uniform) position in
+        // the record: no per-record select over the mode order
+        uint32_t rw[U], j1[U], j2[U], j3[U];
         float vv[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            rr[u] = sh.stage[wv][g][u];
-            vv[u] = NM == 4 ? sh.stagev[wv][g][u] : __uint_as_float(rr[u].w);
+            rw[u] = stg_row[4 * u];
+            j1[u] = stg_o1[4 * u];
+            j2[u] = stg_o2[4 * u];
+            j3[u] = NM == 4 ? stg_o3[4 * u] : 0u;
+            vv[u] = NM == 4 ? sh.stagev[wv][g][u] : __uint_as_float(stg_w[4 * u]);
         }
         seg_wave_sync();
         float4 p[U], q[U], t[NM == 4 ? U : 1];
 #pragma unroll
-        for (int u = 0; u < U; ++u) p[u] = seg_ld(A1 + (size_t)seg_comp(rr[u], o1) * S + cc);
+        for (int u = 0; u < U; ++u) p[u] = seg_ld(A1 + (size_t)j1[u] * S + cc);
 #pragma unroll
-        for (int u = 0; u < U; ++u) q[u] = seg_ld(A2 + (size_t)seg_comp(rr[u], o2) * S + cc);
+        for (int u = 0; u < U; ++u) q[u] = seg_ld(A2 + (size_t)j2[u] * S + cc);
         if (NM == 4) {
 #pragma unroll
-            for (int u = 0; u < U; ++u) t[u] = seg_ld(A3 + (size_t)seg_comp(rr[u], o3) * S + cc);
+            for (int u = 0; u < U; ++u) t[u] = seg_ld(A3 + (size_t)j3[u] * S + cc);
         }
         if (z + U < ze) load_recs(z + U);
 #pragma unroll
@@ -198,7 +207,7 @@ __device__ __forceinline__ void seg_range(const SegArgs &a, uint64_t wave, SegSh
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             if (z + u < ze) {
-                const uint32_t row = seg_comp(rr[u], mode);
+                const uint32_t row = rw[u];
                 if (row != cur) {
                     if (cur != kNoRow) {
                         if (in_first && first_cont) facc = acc;

@@ -198,28 +198,29 @@ __device__ __forceinline__ void seg_range(const SegArgs &a, uint64_t wave, SegSh
             if (NM == 4) {
                 p[u].x *= t[u].x; p[u].y *= t[u].y; p[u].z *= t[u].z; p[u].w *= t[u].w;
             }
-            // padding columns (>= R) hold unspecified factor values: keep them 0
-            p[u].x = m0 ? p[u].x : 0.f;
-            p[u].y = m1 ? p[u].y : 0.f;
-            p[u].z = m2 ? p[u].z : 0.f;
-            p[u].w = m3 ? p[u].w : 0.f;
+            // padding columns (>= R) hold unspecified factor values, and slots
+            // past the range repeat its last nonzero: both contribute 0
+            const bool ok = z + u < ze;
+            p[u].x = (ok && m0) ? p[u].x : 0.f;
+            p[u].y = (ok && m1) ? p[u].y : 0.f;
+            p[u].z = (ok && m2) ? p[u].z : 0.f;
+            p[u].w = (ok && m3) ? p[u].w : 0.f;
         }
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            if (z + u < ze) {
-                const uint32_t row = rw[u];
-                if (row != cur) {
-                    if (cur != kNoRow) {
-                        if (in_first && first_cont) facc = acc;
-                        else if (col_ok) seg_st(out + (size_t)cur * S + c0, acc);
-                        in_first = false;
-                    }
-                    cur = row;
-                    acc = p[u];
-                } else {
-                    seg_add(acc, p[u]);
+            // a repeated slot carries the last nonzero's row, so it only
+            // adds its 0 to the open run
+            const uint32_t row = rw[u];
+            if (row != cur) {
+                if (cur != kNoRow) {
+                    if (in_first && first_cont) facc = acc;
+                    else if (col_ok) seg_st(out + (size_t)cur * S + c0, acc);
+                    in_first = false;
                 }
+                cur = row;
+                acc = zero;   // the row's sum starts from 0, as the reference's output does
             }
+            seg_add(acc, p[u]);
         }
     }
 

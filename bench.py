@@ -276,6 +276,9 @@ def run_hicoo(args, lib):
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 4),
                      "traffic": profile_traffic(TRAFFIC_PROFILE) if full else None,
                      "traffic_source": f"profiles/{TRAFFIC_PROFILE}.md" if full else None,
+                     # measured bytes (FETCH + WRITE) over the same duration: the counter-based fraction
+                     "traffic_frac": (round(profile_traffic(TRAFFIC_PROFILE) / (k_ms * 1e-3) / HBM_PEAK, 4)
+                                      if full and profile_traffic(TRAFFIC_PROFILE) else None),
                      "kernel": KERNEL_SYMBOL[kname], "kernel_avg_ms": round(k_ms, 4),
                      "algorithmic_bytes_per_launch": int(B_avg),
                      "carry_kernel_avg_ms": round(t["mttkrp_seg_carry_ms"], 4),

@@ -174,7 +174,7 @@ __global__ __launch_bounds__(kBatchWPB * 64) __attribute__((amdgpu_waves_per_eu(
                                                               const uint64_t *__restrict__ prefix, uint32_t n,
                                                               uint64_t total, uint32_t *__restrict__ counter)
 {
-    __shared__ SegShared<LPN, NM, kBatchWPB> sh;
+    __shared__ SegSharedOf<LPN, NM, FT, kBatchWPB> sh;
     const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;
     // first range by wave index, then W, W+1, ... from the class counter
     const uint64_t W = (uint64_t)gridDim.x * kBatchWPB;
@@ -219,10 +219,9 @@ SegBatchDev::~SegBatchDev()
     dfree(arena); dfree(carry); dfree(carry_row); dfree(counters);
 }
 
-static int seg_class(uint32_t lpn, uint32_t nmodes)
+static int seg_class(uint32_t glpn, uint32_t nmodes, int dtype)
 {
-    const int li = lpn <= 2 ? 0 : (lpn == 4 ? 1 : (lpn == 8 ? 2 : 3));
-    return li * 2 + (nmodes == 4 ? 1 : 0);
+    return seg_lane_class(glpn, dtype) * 2 + (nmodes == 4 ? 1 : 0);
 }
 
 // Nonzeros per lane group of a batch: 256 scaled by the tuned factor.
@@ -236,16 +235,17 @@ static uint32_t seg_tile_chg(double scale)
 
 // Plan and upload the segmented items (a stream-ordered copy from pinned
 // staging).
-static int seg_batch_prepare(const std::vector<const BatchDesc *> &items, double scale, SegBatchDev &dev, hipStream_t s)
+static int seg_batch_prepare(const std::vector<const BatchDesc *> &items, int dtype, double scale, SegBatchDev &dev,
+                             hipStream_t s)
 {
     std::vector<SegArgs> cls[8];
     const uint32_t chg = seg_tile_chg(scale);
     uint64_t ranges = 0;
     for (const BatchDesc *d : items) {
         SegArgs a;
-        SPTOL_TRY(seg_fill_args(*d->seg, d->mats, d->order, d->R, d->stride, d->out, chg, a));
+        SPTOL_TRY(seg_fill_args(*d->seg, d->mats, d->order, d->R, d->stride, dtype, d->out, chg, a));
         ranges += a.nwaves;
-        cls[seg_class(a.lpn, a.nmodes)].push_back(a);
+        cls[seg_class(a.glpn, a.nmodes, dtype)].push_back(a);
     }
     if (dev.carry_cap < ranges) {
         dfree(dev.carry); dev.carry = nullptr;
@@ -324,15 +324,17 @@ static void seg_batch_kernels_ft(const SegBatchDev &dev, int bpc, hipStream_t s)
 {
     for (int c = 0; c < 8; ++c) {
         if (!dev.n[c]) continue;
+        // class = lane class * 2 + (4 modes)
+        constexpr int L1 = sizeof(FT) == 2 ? 1 : 2;
         switch (c) {
-        case 0: seg_batch_class<2, 3, FT>(dev, c, bpc, s); break;
-        case 1: seg_batch_class<2, 4, FT>(dev, c, bpc, s); break;
-        case 2: seg_batch_class<4, 3, FT>(dev, c, bpc, s); break;
-        case 3: seg_batch_class<4, 4, FT>(dev, c, bpc, s); break;
-        case 4: seg_batch_class<8, 3, FT>(dev, c, bpc, s); break;
-        case 5: seg_batch_class<8, 4, FT>(dev, c, bpc, s); break;
-        case 6: seg_batch_class<16, 3, FT>(dev, c, bpc, s); break;
-        default: seg_batch_class<16, 4, FT>(dev, c, bpc, s); break;
+        case 0: seg_batch_class<L1, 3, FT>(dev, c, bpc, s); break;
+        case 1: seg_batch_class<L1, 4, FT>(dev, c, bpc, s); break;
+        case 2: seg_batch_class<L1 * 2, 3, FT>(dev, c, bpc, s); break;
+        case 3: seg_batch_class<L1 * 2, 4, FT>(dev, c, bpc, s); break;
+        case 4: seg_batch_class<L1 * 4, 3, FT>(dev, c, bpc, s); break;
+        case 5: seg_batch_class<L1 * 4, 4, FT>(dev, c, bpc, s); break;
+        case 6: seg_batch_class<L1 * 8, 3, FT>(dev, c, bpc, s); break;
+        default: seg_batch_class<L1 * 8, 4, FT>(dev, c, bpc, s); break;
         }
     }
 }
@@ -428,7 +430,7 @@ static int build_plan(sptolBatch b)
     split_descs(b->descs, segs, leg);
     if (b->exec) { (void)hipGraphExecDestroy(b->exec); b->exec = nullptr; }
     if (b->graph) { (void)hipGraphDestroy(b->graph); b->graph = nullptr; }
-    SPTOL_TRY(seg_batch_prepare(segs, b->cfg.tile_scale, b->seg, nullptr));
+    SPTOL_TRY(seg_batch_prepare(segs, b->dtype, b->cfg.tile_scale, b->seg, nullptr));
     const size_t n = leg.size();
     b->nlegacy = (uint32_t)n;
     dfree(b->d_desc); dfree(b->d_tile_prefix); dfree(b->d_fill_prefix);
@@ -520,7 +522,7 @@ int batch_launch(std::vector<BatchDesc> &descs, int dtype, double tile_scale, in
     std::vector<BatchDesc> leg;
     split_descs(descs, segs, leg);
     if (!segs.empty()) {
-        SPTOL_TRY(seg_batch_prepare(segs, tile_scale, bb.seg, s));
+        SPTOL_TRY(seg_batch_prepare(segs, dtype, tile_scale, bb.seg, s));
         SPTOL_TRY(seg_batch_kernels(bb.seg, dtype, blocks_per_cu, s));
     }
     const size_t n = leg.size();

@@ -119,6 +119,11 @@ int seg_pack_unsorted(uint4 *rec0, uint32_t nmodes, uint64_t nnz, uint32_t *cons
 bool seg_ok(uint32_t nmodes, uint32_t stride);
 // Lanes per nonzero (one float4 of columns each) for a row stride.
 uint32_t seg_lpn(uint32_t stride);
+// Lanes per lane group of the range kernels for factors of type dtype (4
+// FP32 or 8 BF16 columns a lane), and its class 0..3 among that type's
+// kernels ({2, 4, 8, 16} lanes for FP32, {1, 2, 4, 8} for BF16).
+uint32_t seg_glpn(uint32_t stride, int dtype);
+int seg_lane_class(uint32_t glpn, int dtype);
 // Nonzeros per lane group for a tensor of nnz nonzeros and G groups per wave.
 uint32_t seg_chg(uint64_t nnz, uint32_t G);
 // out[0 .. nrows*stride) = MTTKRP of the view (no fill needed beforehand).
@@ -228,7 +233,7 @@ void make_desc(sptolDevCoo X, const sptolDevMatrix *mats, const uint32_t *mats_o
                const sptolDevMatrix *out, BatchDesc *d);
 
 // Device side of the segmented part of a batch (batch.hip): per class of
-// (lanes per nonzero, modes) the items and their range prefix; all items in
+// (lanes per group, modes) the items and their range prefix; all items in
 // class order with their carry-block prefix; per-batch carry space; one
 // work counter per class (zero between launches).
 struct SegBatchDev {

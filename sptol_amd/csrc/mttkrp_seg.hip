@@ -42,7 +42,7 @@ template <int LPN, int NM, typename FT>
 __global__ __launch_bounds__(kSegWPB * 64) __attribute__((amdgpu_waves_per_eu(NM == 3 ? SPTOL_SEG_WAVES3 : SPTOL_SEG_WAVES4, 8)))
 void k_mttkrp_seg(const SegArgs a, uint32_t *__restrict__ counter)
 {
-    __shared__ SegShared<LPN, NM, kSegWPB> sh;
+    __shared__ SegSharedOf<LPN, NM, FT, kSegWPB> sh;
     const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;
     // Wave w of the W in the grid takes range w first, then pulls ranges
     // W, W+1, ... from the counter: no atomics while every wave starts, none
@@ -118,7 +118,7 @@ static int env_int(const char *name)
 
 bool seg_ok(uint32_t nmodes, uint32_t stride)
 {
-    return (nmodes == 3 || nmodes == 4) && stride >= 4 && stride <= 64;
+    return (nmodes == 3 || nmodes == 4) && stride >= 8 && stride <= 64 && stride % 8 == 0;
 }
 
 uint32_t seg_lpn(uint32_t stride)
@@ -126,6 +126,21 @@ uint32_t seg_lpn(uint32_t stride)
     uint32_t lpn = 2;
     while (lpn * 4 < stride) lpn <<= 1;
     return lpn;
+}
+
+uint32_t seg_glpn(uint32_t stride, int dtype)
+{
+    const uint32_t cols = dtype == SPTOL_BF16 ? 8u : 4u;
+    uint32_t lpn = dtype == SPTOL_BF16 ? 1u : 2u;
+    while (lpn * cols < stride) lpn <<= 1;
+    return lpn;
+}
+
+int seg_lane_class(uint32_t glpn, int dtype)
+{
+    int i = 0;
+    for (uint32_t l = dtype == SPTOL_BF16 ? 1u : 2u; l < glpn; l <<= 1) ++i;
+    return i;
 }
 
 // Records of every nonzero in the tensor's own order (the layout of
@@ -209,12 +224,12 @@ uint32_t seg_chg(uint64_t nnz, uint32_t G)
 }
 
 int seg_fill_args(const SegView &v, const void *const *mats, const uint32_t *order, uint32_t R, uint32_t stride,
-                  float *out, uint32_t chg, SegArgs &a)
+                  int dtype, float *out, uint32_t chg, SegArgs &a)
 {
     if (!v.ready) SPTOL_CHECK(SPTERR_VALUE_ERROR, "Seg MTTKRP", "view not built");
     if (!seg_ok(v.nmodes, stride)) SPTOL_CHECK(SPTERR_VALUE_ERROR, "Seg MTTKRP", "unsupported shape");
-    const uint32_t lpn = seg_lpn(stride);
-    const uint32_t G = 64 / lpn;
+    const uint32_t glpn = seg_glpn(stride, dtype);
+    const uint32_t G = 64 / glpn;
     const uint64_t per_wave = (uint64_t)G * chg;
     uint64_t nwaves = (v.nnz + per_wave - 1) / per_wave;
     const uint64_t by_rows = ((uint64_t)v.nrows + 4095) / 4096;
@@ -237,7 +252,8 @@ int seg_fill_args(const SegView &v, const void *const *mats, const uint32_t *ord
     a.S = stride;
     a.nrows = v.nrows;
     a.chg = chg;
-    a.lpn = lpn;
+    a.lpn = seg_lpn(stride);
+    a.glpn = glpn;
     a.rows_per_wave = (uint32_t)(((uint64_t)v.nrows + nwaves - 1) / nwaves);
     a.nwaves = nwaves;
     a.out = out;
@@ -247,9 +263,9 @@ int seg_fill_args(const SegView &v, const void *const *mats, const uint32_t *ord
 
 // Arguments of a single MTTKRP on the view, with the view's own carry space.
 static int seg_args(SegView &v, const void *const *mats, const uint32_t *order, uint32_t R, uint32_t stride,
-                    float *out, uint32_t chg, SegArgs &a)
+                    int dtype, float *out, uint32_t chg, SegArgs &a)
 {
-    SPTOL_TRY(seg_fill_args(v, mats, order, R, stride, out, chg, a));
+    SPTOL_TRY(seg_fill_args(v, mats, order, R, stride, dtype, out, chg, a));
     if (v.carry_waves < a.nwaves) {
         dfree(v.carry); v.carry = nullptr;
         dfree(v.carry_row); v.carry_row = nullptr;
@@ -276,15 +292,27 @@ static void seg_launch_main(const SegArgs &a, uint32_t *counter, hipStream_t s)
     hipLaunchKernelGGL((k_mttkrp_seg<LPN, NM, FT>), dim3((unsigned)blocks), dim3(kSegWPB * 64), 0, s, a, counter);
 }
 
+// lanes per group of the range kernels: FP32 {2, 4, 8, 16}, BF16 {1, 2, 4, 8}
+template <typename FT, int I>
+constexpr int seg_lanes() { return (sizeof(FT) == 2 ? 1 : 2) << I; }
+
 template <typename FT>
 static void seg_launch_ft(const SegArgs &a, uint32_t *counter, hipStream_t s)
 {
     const bool n3 = a.nmodes == 3;
-    switch (a.lpn) {
-    case 2: n3 ? seg_launch_main<2, 3, FT>(a, counter, s) : seg_launch_main<2, 4, FT>(a, counter, s); break;
-    case 4: n3 ? seg_launch_main<4, 3, FT>(a, counter, s) : seg_launch_main<4, 4, FT>(a, counter, s); break;
-    case 8: n3 ? seg_launch_main<8, 3, FT>(a, counter, s) : seg_launch_main<8, 4, FT>(a, counter, s); break;
-    default: n3 ? seg_launch_main<16, 3, FT>(a, counter, s) : seg_launch_main<16, 4, FT>(a, counter, s); break;
+    switch (seg_lane_class(a.glpn, sizeof(FT) == 2 ? SPTOL_BF16 : SPTOL_F32)) {
+    case 0:
+        n3 ? seg_launch_main<seg_lanes<FT, 0>(), 3, FT>(a, counter, s) : seg_launch_main<seg_lanes<FT, 0>(), 4, FT>(a, counter, s);
+        break;
+    case 1:
+        n3 ? seg_launch_main<seg_lanes<FT, 1>(), 3, FT>(a, counter, s) : seg_launch_main<seg_lanes<FT, 1>(), 4, FT>(a, counter, s);
+        break;
+    case 2:
+        n3 ? seg_launch_main<seg_lanes<FT, 2>(), 3, FT>(a, counter, s) : seg_launch_main<seg_lanes<FT, 2>(), 4, FT>(a, counter, s);
+        break;
+    default:
+        n3 ? seg_launch_main<seg_lanes<FT, 3>(), 3, FT>(a, counter, s) : seg_launch_main<seg_lanes<FT, 3>(), 4, FT>(a, counter, s);
+        break;
     }
 }
 
@@ -292,7 +320,7 @@ int seg_mttkrp(SegView &v, const void *const *mats, const uint32_t *order, uint3
                float *out, hipStream_t s)
 {
     SegArgs a;
-    SPTOL_TRY(seg_args(v, mats, order, R, stride, out, seg_chg(v.nnz, 64 / seg_lpn(stride)), a));
+    SPTOL_TRY(seg_args(v, mats, order, R, stride, dtype, out, seg_chg(v.nnz, 64 / seg_glpn(stride, dtype)), a));
     int tok = prof_start("mttkrp_seg", s);
     if (dtype == SPTOL_BF16) seg_launch_ft<uint16_t>(a, v.counter, s);
     else seg_launch_ft<float>(a, v.counter, s);

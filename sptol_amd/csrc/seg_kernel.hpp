@@ -15,15 +15,17 @@
 //
 // One "range" (seg_range) is the unit of work of a wave:
 //  * The wave owns G*chg consecutive nonzeros; each of its G lane groups
-//    (LPN = stride/4 lanes, 4 columns per lane) owns chg of them.
+//    owns chg of them.  A group has LPN lanes of C columns each: 4 FP32
+//    columns (one 16-byte gather) or 8 BF16 columns (also one 16-byte
+//    gather, so BF16 factors halve the gathers per nonzero).
 //  * A group loads U records with one coalesced access (lane l loads record
 //    z+l), passes them through LDS and reads them back as broadcasts, then
 //    issues all U*(N-1) factor-row gathers before using any; the next
 //    step's records are loaded behind the gathers.
-//  * Products are summed in registers while the output row repeats.  A row
-//    that starts and ends inside the group is stored with one plain 16-byte
-//    store per lane.  The group's first row, when it began in the previous
-//    group, and its last row stay in registers.
+//  * Products are summed in FP32 registers while the output row repeats.  A
+//    row that starts and ends inside the group is stored with plain 16-byte
+//    stores.  The group's first row, when it began in the previous group,
+//    and its last row stay in registers.
 //  * The wave then merges its groups' boundary rows through LDS in a fixed
 //    order: the group holding a row's first nonzero stores the row.  A row
 //    that began in an earlier range is written as this range's carry.
@@ -50,12 +52,14 @@ struct SegArgs {
     const uint32_t *touched;
     float *carry;            // nwaves x kSegCarryStride floats
     uint32_t *carry_row;     // nwaves
-    uint32_t nmodes, pad;
+    uint32_t nmodes;
+    uint32_t glpn;           // lanes per group of the range kernel (lpn: of the carry kernel, 4 columns a lane)
 };
 
-// SegArgs of one MTTKRP on view v (carry space left unset; mttkrp_seg.hip).
+// SegArgs of one MTTKRP on view v with factors of type dtype (carry space
+// left unset; mttkrp_seg.hip).
 int seg_fill_args(const SegView &v, const void *const *mats, const uint32_t *order, uint32_t R, uint32_t stride,
-                  float *out, uint32_t chg, SegArgs &a);
+                  int dtype, float *out, uint32_t chg, SegArgs &a);
 
 constexpr uint32_t kSegCarryStride = 64;
 constexpr int kSegU = 4;                 // records per lane-group step
@@ -67,13 +71,6 @@ __device__ __forceinline__ uint32_t seg_comp(const uint4 r, uint32_t k)
 }
 
 __device__ __forceinline__ float4 seg_ld(const float *p) { return *reinterpret_cast<const float4 *>(p); }
-
-__device__ __forceinline__ float4 seg_ld(const uint16_t *p)
-{
-    const uint2 u = *reinterpret_cast<const uint2 *>(p);
-    return make_float4(__uint_as_float(u.x << 16), __uint_as_float(u.x & 0xFFFF0000u),
-                       __uint_as_float(u.y << 16), __uint_as_float(u.y & 0xFFFF0000u));
-}
 
 __device__ __forceinline__ void seg_st(float *p, const float4 v) { *reinterpret_cast<float4 *>(p) = v; }
 
@@ -92,42 +89,117 @@ __device__ __forceinline__ void seg_wave_sync()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Per-wave LDS of the range body, for WPB waves.
-template <int LPN, int NM, int WPB>
+// Columns of one factor row held by a lane: C values of type FT, loaded
+// with one 16-byte access and widened to FP32 one column at a time where
+// they are used (a BF16 value is the upper half of its FP32 value).
+template <typename FT, int C>
+struct SegCols;
+
+template <>
+struct SegCols<float, 4> {
+    using Raw = float4;
+    static __device__ __forceinline__ Raw load(const float *p) { return *reinterpret_cast<const float4 *>(p); }
+    static __device__ __forceinline__ float at(const Raw &r, int k)
+    {
+        return k == 0 ? r.x : (k == 1 ? r.y : (k == 2 ? r.z : r.w));
+    }
+};
+
+template <>
+struct SegCols<uint16_t, 8> {
+    using Raw = uint4;
+    static __device__ __forceinline__ Raw load(const uint16_t *p) { return *reinterpret_cast<const uint4 *>(p); }
+    static __device__ __forceinline__ float at(const Raw &r, int k)
+    {
+        const int h = k >> 1;
+        const uint32_t w = h == 0 ? r.x : (h == 1 ? r.y : (h == 2 ? r.z : r.w));
+        return __uint_as_float((k & 1) ? (w & 0xFFFF0000u) : (w << 16));
+    }
+};
+
+// Columns per lane of the range kernel for factor type FT.
+template <typename FT>
+constexpr int seg_cols() { return sizeof(FT) == 2 ? 8 : 4; }
+
+// FP32 sums of C columns (registers once the loops are unrolled).
+template <int C>
+struct SegAcc {
+    float v[C];
+    __device__ __forceinline__ void zero()
+    {
+#pragma unroll
+        for (int k = 0; k < C; ++k) v[k] = 0.f;
+    }
+    __device__ __forceinline__ void add(const SegAcc &b)
+    {
+#pragma clang fp contract(off)
+#pragma unroll
+        for (int k = 0; k < C; ++k) v[k] += b.v[k];
+    }
+    __device__ __forceinline__ void store(float *p) const
+    {
+#pragma unroll
+        for (int j = 0; j < C / 4; ++j) seg_st(p + 4 * j, make_float4(v[4 * j], v[4 * j + 1], v[4 * j + 2], v[4 * j + 3]));
+    }
+    __device__ __forceinline__ void to_lds(float4 *p) const
+    {
+#pragma unroll
+        for (int j = 0; j < C / 4; ++j) p[j] = make_float4(v[4 * j], v[4 * j + 1], v[4 * j + 2], v[4 * j + 3]);
+    }
+    __device__ __forceinline__ void add_lds(const float4 *p)
+    {
+#pragma clang fp contract(off)
+#pragma unroll
+        for (int j = 0; j < C / 4; ++j) {
+            const float4 t = p[j];
+            v[4 * j] += t.x; v[4 * j + 1] += t.y; v[4 * j + 2] += t.z; v[4 * j + 3] += t.w;
+        }
+    }
+};
+
+// Per-wave LDS of the range body, for WPB waves of C columns a lane.
+template <int LPN, int NM, int WPB, int C>
 struct SegShared {
     static constexpr uint32_t G = 64 / LPN;
     static constexpr uint32_t SL = kSegU + 1;   // staging slots per group (+1 spreads banks)
     uint4 stage[WPB][G][SL];
     float stagev[WPB][G][NM == 4 ? SL : 1];
-    float4 piece[WPB][64];
+    float4 piece[WPB][64][C / 4];
     uint32_t info[WPB][G][2];
 };
 
 template <int LPN, int NM, typename FT, int WPB>
-__device__ __forceinline__ void seg_range(const SegArgs &a, uint64_t wave, SegShared<LPN, NM, WPB> &sh, uint32_t wv,
-                                          uint32_t lane)
+using SegSharedOf = SegShared<LPN, NM, WPB, seg_cols<FT>()>;
+
+template <int LPN, int NM, typename FT, int WPB>
+__device__ __forceinline__ void seg_range(const SegArgs &a, uint64_t wave, SegSharedOf<LPN, NM, FT, WPB> &sh,
+                                          uint32_t wv, uint32_t lane)
 {
 #pragma clang fp contract(off)
+    constexpr int C = seg_cols<FT>();
+    using Cols = SegCols<FT, C>;
+    using Raw = typename Cols::Raw;
     constexpr int U = kSegU;
     constexpr uint32_t KL = (U + LPN - 1) / LPN;   // records each lane stages per step
     constexpr uint32_t G = 64 / LPN;
     const uint32_t g = lane / LPN, l = lane % LPN;
-    const uint32_t S = a.S, R = a.R, c0 = l * 4u;
-    const bool col_ok = c0 < S;
-    const uint32_t cc = col_ok ? c0 : 0u;     // idle lanes gather column 0
-    const bool m0 = c0 < R, m1 = c0 + 1 < R, m2 = c0 + 2 < R, m3 = c0 + 3 < R;
+    const uint32_t S = a.S, R = a.R, c0 = l * (uint32_t)C;
+    const bool col_ok = c0 < S;                  // S is a multiple of 8: a lane's columns are all in the row
+    const uint32_t cc = col_ok ? c0 : 0u;        // idle lanes gather column 0
+    const uint32_t ncol = R > c0 ? R - c0 : 0u;  // this lane's columns below R
     const FT *__restrict__ A1 = (const FT *)a.A1;
     const FT *__restrict__ A2 = (const FT *)a.A2;
     const FT *__restrict__ A3 = (const FT *)a.A3;
     const uint32_t mode = a.mode, o1 = a.o1, o2 = a.o2, o3 = a.o3;
     const uint4 *__restrict__ rec = a.rec;
     float *__restrict__ out = a.out;
-    const float4 zero = make_float4(0.f, 0.f, 0.f, 0.f);
 
     const uint64_t zs = (wave * G + g) * (uint64_t)a.chg;
     const uint64_t ze = zs < a.nnz ? min<uint64_t>(zs + a.chg, a.nnz) : zs;
     const bool nonempty = zs < ze;
-    float4 acc = zero, facc = zero;
+    SegAcc<C> acc, facc;
+    acc.zero();
+    facc.zero();
     uint32_t cur = kNoRow, frow = kNoRow;
     bool first_cont = false, in_first = true;
     if (nonempty) {
@@ -179,33 +251,16 @@ __device__ __forceinline__ void seg_range(const SegArgs &a, uint64_t wave, SegSh
             vv[u] = NM == 4 ? sh.stagev[wv][g][u] : __uint_as_float(stg_w[4 * u]);
         }
         seg_wave_sync();
-        float4 p[U], q[U], t[NM == 4 ? U : 1];
+        Raw p[U], q[U], t[NM == 4 ? U : 1];
 #pragma unroll
-        for (int u = 0; u < U; ++u) p[u] = seg_ld(A1 + (size_t)j1[u] * S + cc);
+        for (int u = 0; u < U; ++u) p[u] = Cols::load(A1 + (size_t)j1[u] * S + cc);
 #pragma unroll
-        for (int u = 0; u < U; ++u) q[u] = seg_ld(A2 + (size_t)j2[u] * S + cc);
+        for (int u = 0; u < U; ++u) q[u] = Cols::load(A2 + (size_t)j2[u] * S + cc);
         if (NM == 4) {
 #pragma unroll
-            for (int u = 0; u < U; ++u) t[u] = seg_ld(A3 + (size_t)j3[u] * S + cc);
+            for (int u = 0; u < U; ++u) t[u] = Cols::load(A3 + (size_t)j3[u] * S + cc);
         }
         if (z + U < ze) load_recs(z + U);
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            p[u].x = (vv[u] * p[u].x) * q[u].x;
-            p[u].y = (vv[u] * p[u].y) * q[u].y;
-            p[u].z = (vv[u] * p[u].z) * q[u].z;
-            p[u].w = (vv[u] * p[u].w) * q[u].w;
-            if (NM == 4) {
-                p[u].x *= t[u].x; p[u].y *= t[u].y; p[u].z *= t[u].z; p[u].w *= t[u].w;
-            }
-            // padding columns (>= R) hold unspecified factor values, and slots
-            // past the range repeat its last nonzero: both contribute 0
-            const bool ok = z + u < ze;
-            p[u].x = (ok && m0) ? p[u].x : 0.f;
-            p[u].y = (ok && m1) ? p[u].y : 0.f;
-            p[u].z = (ok && m2) ? p[u].z : 0.f;
-            p[u].w = (ok && m3) ? p[u].w : 0.f;
-        }
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             // a repeated slot carries the last nonzero's row, so it only
@@ -214,13 +269,22 @@ __device__ __forceinline__ void seg_range(const SegArgs &a, uint64_t wave, SegSh
             if (row != cur) {
                 if (cur != kNoRow) {
                     if (in_first && first_cont) facc = acc;
-                    else if (col_ok) seg_st(out + (size_t)cur * S + c0, acc);
+                    else if (col_ok) acc.store(out + (size_t)cur * S + c0);
                     in_first = false;
                 }
                 cur = row;
-                acc = zero;   // the row's sum starts from 0, as the reference's output does
+                acc.zero();   // the row's sum starts from 0, as the reference's output does
             }
-            seg_add(acc, p[u]);
+            // ((v * A_o1) * A_o2) * A_o3 per column; padding columns (>= R)
+            // hold unspecified factor values, and slots past the range
+            // repeat its last nonzero: both contribute 0
+            const bool ok = z + u < ze;
+#pragma unroll
+            for (int k = 0; k < C; ++k) {
+                float x = (vv[u] * Cols::at(p[u], k)) * Cols::at(q[u], k);
+                if (NM == 4) x *= Cols::at(t[u], k);
+                acc.v[k] += (ok && (uint32_t)k < ncol) ? x : 0.f;
+            }
         }
     }
 
@@ -230,45 +294,47 @@ __device__ __forceinline__ void seg_range(const SegArgs &a, uint64_t wave, SegSh
     // else its first row's sum.
     const bool single = in_first;
     const uint32_t flags = (nonempty ? 1u : 0u) | (first_cont ? 2u : 0u) | (single ? 4u : 0u);
-    const float4 piece = single ? acc : facc;
-    sh.piece[wv][lane] = piece;
+    const SegAcc<C> piece = single ? acc : facc;
+    piece.to_lds(sh.piece[wv][lane]);
     if (l == 0) {
         sh.info[wv][g][0] = frow;
         sh.info[wv][g][1] = flags;
     }
     seg_wave_sync();
     // the chain of pieces that extends a row past group `from - 1`
-    auto chain = [&](float4 tot, uint32_t from) {
+    auto chain = [&](SegAcc<C> tot, uint32_t from) {
         for (uint32_t k = from; k < G; ++k) {
             const uint32_t f = sh.info[wv][k][1];
             if ((f & 3u) != 3u) break;
-            seg_add(tot, sh.piece[wv][k * LPN + l]);
+            tot.add_lds(sh.piece[wv][k * LPN + l]);
             if (!(f & 4u)) break;
         }
         return tot;
     };
     if (nonempty && !(single && first_cont)) {
         // this group holds the first nonzero of its last row
-        const float4 tot = chain(acc, g + 1);
-        if (col_ok) seg_st(out + (size_t)cur * S + c0, tot);
+        const SegAcc<C> tot = chain(acc, g + 1);
+        if (col_ok) tot.store(out + (size_t)cur * S + c0);
     }
     if (g == 0) {
         uint32_t crow = kNoRow;
         if (nonempty && first_cont) {
-            const float4 tot = single ? chain(piece, 1) : piece;
+            const SegAcc<C> tot = single ? chain(piece, 1) : piece;
             crow = frow;
-            if (col_ok) seg_st(a.carry + wave * kSegCarryStride + c0, tot);
+            if (col_ok) tot.store(a.carry + wave * kSegCarryStride + c0);
         }
         if (l == 0) a.carry_row[wave] = crow;
     }
     seg_wave_sync();   // piece / info are rewritten by the next range
 
     // zero the untouched rows of this range's slice
+    SegAcc<C> zero;
+    zero.zero();
     const uint64_t r0 = wave * (uint64_t)a.rows_per_wave;
     const uint64_t r1 = min<uint64_t>(r0 + a.rows_per_wave, a.nrows);
     for (uint64_t r = r0 + g; r < r1; r += G) {
         const uint32_t w = a.touched[r >> 5];
-        if (!((w >> (r & 31u)) & 1u) && col_ok) seg_st(out + r * S + c0, zero);
+        if (!((w >> (r & 31u)) & 1u) && col_ok) zero.store(out + r * S + c0);
     }
 }
 

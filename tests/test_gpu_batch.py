@@ -11,9 +11,9 @@ import pytest
 
 import oracle
 from conftest import rel_fro
-from gpu_util import DevFactors, DevOut, dev_coo
+from gpu_util import DevFactors, DevOut, bf16_round, dev_coo
 from sptol_amd import gen
-from sptol_amd.parti import SPTOL_F32, check, ptr, sptIndex
+from sptol_amd.parti import SPTOL_BF16, SPTOL_F32, check, ptr, sptIndex
 
 pytestmark = pytest.mark.gpu
 
@@ -22,7 +22,7 @@ def _say(*a):
     print(*a, flush=True)
 
 
-def _make_batch(lib, specs, prepare):
+def _make_batch(lib, specs, prepare, dtype=SPTOL_F32):
     items = []
     for k, (dims, nnz, R, mode, seed) in enumerate(specs):
         d, inds, vals = gen.uniform_tensor(dims, nnz, seed=seed)
@@ -32,9 +32,10 @@ def _make_batch(lib, specs, prepare):
         h = dev_coo(lib, d, inds, vals)
         if prepare is True or (prepare == "mixed" and k % 2 == 0):
             check(lib, lib.sptolDevCooPrepareMode(h, mode), "prepare")
-        F = DevFactors(lib, A)
+        F = DevFactors(lib, A, dtype)
         out = DevOut(lib, int(d[mode]), R)
-        want = oracle.mttkrp_coo(inds, vals, A, order, mode, dtype=np.float64)
+        Au = [bf16_round(a) for a in A] if dtype == SPTOL_BF16 else A
+        want = oracle.mttkrp_coo(inds, vals, Au, order, mode, dtype=np.float64)
         items.append(dict(h=h, F=F, out=out, order=order, mode=mode, want=want, R=R))
     b = C.c_void_p()
     check(lib, lib.sptolBatchCreate(C.byref(b)), "BatchCreate")
@@ -87,6 +88,20 @@ def test_batch_plain_and_autotuned(gpu_lib, prepare):
     sc, oc = C.c_double(), C.c_int()
     lib.sptolBatchGetConfig(b, C.byref(sc), C.byref(oc))
     _say(f"  tuned tile_scale={sc.value} blocks_per_cu={oc.value}")
+    check(lib, lib.sptolBatchRun(b, None), "BatchRun")
+    check(lib, lib.sptolHipStreamSynchronize(None), "sync")
+    _verify(items)
+    _free(lib, b, items)
+
+
+@pytest.mark.parametrize("prepare", [True, "mixed"])
+def test_batch_bf16_factors(gpu_lib, prepare):
+    """BF16-stored factors (8 columns a lane on the segmented path, R from 5
+    to 64): outputs match the FP64 oracle run on the same BF16-rounded
+    factors."""
+    lib = gpu_lib
+    b, items = _make_batch(lib, SPECS, prepare, SPTOL_BF16)
+    _say("batch bf16", prepare)
     check(lib, lib.sptolBatchRun(b, None), "BatchRun")
     check(lib, lib.sptolHipStreamSynchronize(None), "sync")
     _verify(items)

@@ -180,7 +180,8 @@ def test_dropin_hicoo_vs_reference_golden(gpu_lib, golden_hicoo):
                                             (32, 0, "tiled"), (16, 0, "tiled"), (32, 1, "tiled"), (8, 0, "tiled"),
                                             (5, 0, "tiled"), (64, 0, "tiled"), (100, 0, "tiled"),
                                             (32, 0, "seg"), (16, 0, "seg"), (32, 1, "seg"), (8, 0, "seg"),
-                                            (5, 0, "seg"), (64, 0, "seg"), (100, 0, "seg"), (24, 1, "seg")])
+                                            (5, 0, "seg"), (64, 0, "seg"), (100, 0, "seg"), (24, 1, "seg"),
+                                            (8, 1, "seg"), (5, 1, "seg"), (16, 1, "seg"), (64, 1, "seg")])
 def test_device_hicoo_zipf(gpu_lib, R, dtype, prepare):
     d, inds, vals = gen.zipf_tensor([120000, 90000, 280000], 300000, 1.1, seed=R + dtype)
     coo = CooTensor(d, [a.copy() for a in inds], vals.copy())

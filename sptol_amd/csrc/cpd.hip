@@ -635,6 +635,7 @@ struct AlsStepDesc {
     double *lambda, *norms;
     const double *normx;
     double *fit;
+    double *host_fit;          // the job's pinned slot: the fit is also stored there
     uint32_t I, R, S, N, m, norm_kind, end_iter, pad;
     uint32_t nb_col, nb_scale, nb_gram, nb_inner;
     uint32_t blk[kPhases];     // first block in each phase's grid (solve: within the rank class)
@@ -707,6 +708,9 @@ __global__ __launch_bounds__(256) void k_fit_b(const AlsStepDesc *__restrict__ d
     const AlsStepDesc &q = d[blockIdx.x];
     if (!q.end_iter) return;   // whole block
     fit_body(q.ata, q.N, q.R, q.lambda, q.partial, q.nb_inner, 0, q.normx, q.fit);
+    // thread 0 wrote the fit; the host reads the slot once the stream has
+    // passed this kernel (no copy is queued per job)
+    if (threadIdx.x == 0) *q.host_fit = *q.fit;
 }
 
 // ---------------------------------------------------------------------------
@@ -789,7 +793,7 @@ int als_step_batch(const std::vector<CpdJob *> &jobs, AlsBatchBuffers &bb, hipSt
             q.partial = j->w.partial;
             q.fac = j->w.fac; q.piv = j->w.piv; q.kind = j->w.kind;
             q.lambda = j->w.lambda; q.norms = j->w.norms;
-            q.normx = j->w.normx; q.fit = j->w.fit;
+            q.normx = j->w.normx; q.fit = j->w.fit; q.host_fit = j->h_fit;
             q.I = I; q.R = j->R; q.S = j->S; q.N = j->N; q.m = m;
             q.norm_kind = j->it == 0 ? 0 : 1;
             q.end_iter = m + 1 == j->N ? 1 : 0;
@@ -859,7 +863,7 @@ int als_step_batch(const std::vector<CpdJob *> &jobs, AlsBatchBuffers &bb, hipSt
     }
     prof_stop(tok, s);
     SPTOL_HIP(hipGetLastError(), "CPD batch");
-    for (CpdJob *j : jobs) SPTOL_TRY(j->advance(s));
+    for (CpdJob *j : jobs) j->advance();
     return 0;
 }
 
@@ -992,12 +996,10 @@ int CpdJob::after_mttkrp(hipStream_t s)
     return 0;
 }
 
-int CpdJob::advance(hipStream_t s)
+void CpdJob::advance()
 {
-    if (++m < N) return 0;
-    SPTOL_HIP(hipMemcpyAsync(h_fit, w.fit, sizeof(double), hipMemcpyDeviceToHost, s), "CPD fit");
+    if (++m < N) return;
     fit_pending = true;
-    return 0;
 }
 
 void CpdJob::fit_ready()

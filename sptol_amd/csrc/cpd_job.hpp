@@ -74,9 +74,10 @@ struct CpdJob {
     // fit_pending (call fit_ready() after the stream synchronises).
     int after_mttkrp(hipStream_t s);
     // The tail of after_mttkrp once the step's kernels are queued (by
-    // als_step_batch): advance the mode and, at the end of an iteration,
-    // queue the fit's copy to the host and set fit_pending.
-    int advance(hipStream_t s);
+    // als_step_batch, whose fit kernel also stores the fit in the pinned
+    // slot): advance the mode and, at the end of an iteration, set
+    // fit_pending.
+    void advance();
     void fit_ready();
     // Final weights (and the exact fit when exact_fit): finish_enqueue
     // queues the kernels and the copies to the pinned slot; finish_read

@@ -85,13 +85,18 @@ __global__ void k_seg_pack(uint4 *__restrict__ rec, float *__restrict__ vout, co
 
 // rec[z] = rec0[perm[z]] (and vout[z] = vals[perm[z]]): one 16-byte gather
 // per nonzero from records packed in the tensor's own order.
+// Records through the sort permutation, and the touched-row bitmap from the
+// sorted rows (a row's first nonzero sets its bit).
 __global__ void k_seg_gather_rec(uint4 *__restrict__ rec, float *__restrict__ vout, const uint4 *__restrict__ rec0,
-                                 const float *__restrict__ vals, const uint32_t *__restrict__ perm, uint64_t nnz)
+                                 const float *__restrict__ vals, const uint32_t *__restrict__ perm,
+                                 const uint32_t *__restrict__ rows, uint32_t *__restrict__ touched, uint64_t nnz)
 {
     for (uint64_t z = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; z < nnz; z += (uint64_t)gridDim.x * blockDim.x) {
         const uint32_t p = perm[z];
         rec[z] = rec0[p];
         if (vout) vout[z] = vals[p];
+        const uint32_t r = rows[z];
+        if (z == 0 || rows[z - 1] != r) atomicOr(&touched[r >> 5], 1u << (r & 31u));
     }
 }
 
@@ -189,14 +194,15 @@ int seg_build(SegView &v, uint32_t nmodes, uint32_t mode, uint32_t nrows, uint64
                 v.vals = const_cast<float *>(vals);
             }
         }
-        if (perm && rec0)
+        if (perm && rec0) {
             hipLaunchKernelGGL(k_seg_gather_rec, dim3(seg_grid(nnz)), dim3(256), 0, s, v.rec, vout, rec0, vals, perm,
-                               nnz);
-        else
+                               sorted_rows, v.touched, nnz);
+        } else {
             hipLaunchKernelGGL(k_seg_pack, dim3(seg_grid(nnz)), dim3(256), 0, s, v.rec, vout, inds[0], inds[1],
                                inds[2], nmodes == 4 ? inds[3] : inds[0], vals, perm, nnz, nmodes);
-        hipLaunchKernelGGL(k_seg_touch, dim3(seg_grid(nnz)), dim3(256), 0, s, v.touched,
-                           perm ? sorted_rows : inds[mode], nnz);
+            hipLaunchKernelGGL(k_seg_touch, dim3(seg_grid(nnz)), dim3(256), 0, s, v.touched,
+                               perm ? sorted_rows : inds[mode], nnz);
+        }
     }
     e = hipGetLastError();
     if (e == hipSuccess) e = hipStreamSynchronize(s);

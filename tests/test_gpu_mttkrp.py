@@ -247,9 +247,9 @@ def test_segmented_kernel_deterministic_and_rowwise_exact(gpu_lib):
 
 @pytest.mark.parametrize("R", [16, 32])
 def test_segmented_hot_rows_short_modes(gpu_lib, R):
-    """Segmented kernel with hot factor rows in LDS (>= 65536 nonzeros) on
-    4-D and 3-D Zipf tensors with modes shorter than the 256 hot slots, every
-    output mode, FP32 and BF16 factors."""
+    """Segmented kernel on 4-D and 3-D Zipf tensors with very short modes
+    (40-60 rows), so output rows are long and span many lane groups and
+    ranges, every output mode, FP32 and BF16 factors."""
     for dims in ([500, 400, 300, 60], [3000, 40, 2000]):
         d, inds, vals = gen.zipf_tensor(dims, 150000, 1.05, seed=R + len(dims))
         A = gen.uniform_factors(d, R, seed=4)

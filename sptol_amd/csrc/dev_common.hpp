@@ -109,12 +109,18 @@ struct SegView {
 // With perm, inds and vals are in any order and nonzero perm[z] becomes
 // record z; sorted_rows[z] is then its mode index, and copy_vals must be set.
 // With rec0 as well (records of the tensor in its own order, from
-// seg_pack_unsorted), each record is one 16-byte gather from rec0.
+// seg_pack_unsorted), each record is one 16-byte gather from rec0, and the
+// sorted rows may come as 16-bit keys (sorted_rows16, at most 2^16 rows)
+// instead of sorted_rows.
 int seg_build(SegView &v, uint32_t nmodes, uint32_t mode, uint32_t nrows, uint64_t nnz,
               uint32_t *const *inds, const float *vals, bool copy_vals, hipStream_t s, const uint32_t *ndims,
-              const uint32_t *perm = nullptr, const uint32_t *sorted_rows = nullptr, const uint4 *rec0 = nullptr);
+              const uint32_t *perm = nullptr, const uint32_t *sorted_rows = nullptr, const uint4 *rec0 = nullptr,
+              const uint16_t *sorted_rows16 = nullptr);
+// Records in the tensor's own order into rec0 and, for each mode m with
+// keys16[m] non-null (keys16 itself may be null), the mode's indices as
+// 16-bit sort keys.
 int seg_pack_unsorted(uint4 *rec0, uint32_t nmodes, uint64_t nnz, uint32_t *const *inds, const float *vals,
-                      hipStream_t s);
+                      hipStream_t s, uint16_t *const *keys16 = nullptr);
 // Whether the segmented kernel handles this shape.
 bool seg_ok(uint32_t nmodes, uint32_t stride);
 // Lanes per nonzero (one float4 of columns each) for a row stride.

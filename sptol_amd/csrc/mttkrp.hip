@@ -614,11 +614,12 @@ int coo_create(sptolDevCoo *h, const sptSparseTensor *X, hipStream_t s, bool che
     return 0;
 }
 
-static int coo_prepare_from(sptolDevCoo h, uint32_t mode, hipStream_t s, const uint4 *rec0, const uint32_t *iota);
+static int coo_prepare_from(sptolDevCoo h, uint32_t mode, hipStream_t s, const uint4 *rec0, const uint32_t *iota,
+                            const uint16_t *keys16);
 
 int coo_prepare(sptolDevCoo h, uint32_t mode, hipStream_t s)
 {
-    return coo_prepare_from(h, mode, s, nullptr, nullptr);
+    return coo_prepare_from(h, mode, s, nullptr, nullptr, nullptr);
 }
 
 int coo_prepare_all(sptolDevCoo h, hipStream_t s)
@@ -630,22 +631,33 @@ int coo_prepare_all(sptolDevCoo h, hipStream_t s)
         for (uint32_t m = 0; m < h->nmodes; ++m) SPTOL_TRY(coo_prepare(h, m, s));
         return 0;
     }
-    StreamScope scope(s);   // the packed records and 0..nnz-1 are used on s only
+    StreamScope scope(s);   // the packed records, 0..nnz-1 and the 16-bit keys are used on s only
     uint4 *rec0 = nullptr;
     uint32_t *iota = nullptr;
+    // modes of at most 2^16 rows sort 16-bit keys (6 bytes a pass instead of 8)
+    uint16_t *k16[kMaxModes] = {}, *k16_all = nullptr;
+    uint32_t n16 = 0;
+    for (uint32_t m = 0; m < h->nmodes; ++m) n16 += (!h->sorted[m].ready && h->ndims[m] <= 65536u) ? 1u : 0u;
     int r = dmalloc((void **)&rec0, h->nnz * sizeof(uint4), "DevCoo Prepare");
     if (!r) r = dmalloc((void **)&iota, h->nnz * 4, "DevCoo Prepare");
-    if (!r) r = seg_pack_unsorted(rec0, h->nmodes, h->nnz, h->inds, h->vals, s);
+    if (!r && n16) r = dmalloc((void **)&k16_all, h->nnz * 2 * n16, "DevCoo Prepare");
+    for (uint32_t m = 0, k = 0; m < h->nmodes && !r && n16; ++m)
+        if (!h->sorted[m].ready && h->ndims[m] <= 65536u) k16[m] = k16_all + (size_t)(k++) * h->nnz;
+    if (!r) r = seg_pack_unsorted(rec0, h->nmodes, h->nnz, h->inds, h->vals, s, k16);
     if (!r) hipLaunchKernelGGL(k_iota, dim3(grid_for(h->nnz)), dim3(256), 0, s, iota, h->nnz);
-    for (uint32_t m = 0; m < h->nmodes && !r; ++m) r = coo_prepare_from(h, m, s, rec0, iota);
+    for (uint32_t m = 0; m < h->nmodes && !r; ++m) r = coo_prepare_from(h, m, s, rec0, iota, k16[m]);
     (void)hipStreamSynchronize(s);
     dfree(rec0);
     dfree(iota);
+    dfree(k16_all);
     return r;
 }
 
-// iota: 0..nnz-1 on the device (the sort's input values), or null to make it
-static int coo_prepare_from(sptolDevCoo h, uint32_t mode, hipStream_t s, const uint4 *rec0, const uint32_t *iota)
+// iota: 0..nnz-1 on the device (the sort's input values), or null to make it;
+// keys16: the mode's indices as 16-bit keys (with rec0; at most 2^16 rows),
+// or null to sort the 32-bit indices
+static int coo_prepare_from(sptolDevCoo h, uint32_t mode, hipStream_t s, const uint4 *rec0, const uint32_t *iota,
+                            const uint16_t *keys16)
 {
     if (mode >= h->nmodes) SPTOL_CHECK(SPTERR_VALUE_ERROR, "DevCoo Prepare", "mode out of range");
     SortedCopy &sc = h->sorted[mode];
@@ -657,28 +669,35 @@ static int coo_prepare_from(sptolDevCoo h, uint32_t mode, hipStream_t s, const u
     void *tmp = nullptr;
     size_t tmp_bytes = 0;
     int r = 0;
+    // 16-bit keys need the packed records (the view is built by gathering them)
+    const bool k16 = keys16 && rec0 && (h->nmodes == 3 || h->nmodes == 4) && h->ndims[mode] <= 65536u;
     auto cleanup = [&]() { dfree(perm); dfree(perm2); dfree(keys2); dfree(tmp); };
     if ((!iota && (r = dmalloc((void **)&perm, n * 4, "DevCoo Prepare"))) ||
         (r = dmalloc((void **)&perm2, n * 4, "DevCoo Prepare")) ||
-        (r = dmalloc((void **)&keys2, n * 4, "DevCoo Prepare"))) { cleanup(); return r; }
+        (r = dmalloc((void **)&keys2, n * (k16 ? 2 : 4), "DevCoo Prepare"))) { cleanup(); return r; }
     if (!iota) hipLaunchKernelGGL(k_iota, dim3(grid_for(n)), dim3(256), 0, s, perm, n);
     const uint32_t *vin = iota ? iota : perm;
+    uint16_t *keys2_16 = reinterpret_cast<uint16_t *>(keys2);
     int endbit = 1;
     while (endbit < 32 && (1ull << endbit) < h->ndims[mode]) ++endbit;
-    hipError_t e = hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, h->inds[mode], keys2, vin, perm2,
-                                                      (int)n, 0, endbit, s);
+    // stable LSD radix sort of (row, position): the two calls size, then run
+    auto sort = [&](void *t, size_t &bytes) {
+        return k16 ? hipcub::DeviceRadixSort::SortPairs(t, bytes, keys16, keys2_16, vin, perm2, (int)n, 0, endbit, s)
+                   : hipcub::DeviceRadixSort::SortPairs(t, bytes, h->inds[mode], keys2, vin, perm2, (int)n, 0,
+                                                        endbit, s);
+    };
+    hipError_t e = sort(nullptr, tmp_bytes);
     if (e == hipSuccess) {
         if ((r = dmalloc(&tmp, tmp_bytes ? tmp_bytes : 4, "DevCoo Prepare"))) { cleanup(); return r; }
-        e = hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, h->inds[mode], keys2, vin, perm2, (int)n, 0,
-                                               endbit, s);
+        e = sort(tmp, tmp_bytes);
     }
     if (e != hipSuccess) { cleanup(); SPTOL_HIP(e, "DevCoo Prepare"); }
     if (h->nmodes == 3 || h->nmodes == 4) {
         // 3- and 4-mode tensors: the packed records are gathered straight
         // through the permutation; no sorted SoA copy is kept (the atomic
         // fallback kernel reads the tensor in its own order)
-        r = seg_build(sc.seg, h->nmodes, mode, h->ndims[mode], n, h->inds, h->vals, true, s, h->ndims, perm2, keys2,
-                      rec0);
+        r = seg_build(sc.seg, h->nmodes, mode, h->ndims[mode], n, h->inds, h->vals, true, s, h->ndims, perm2,
+                      k16 ? nullptr : keys2, rec0, k16 ? keys2_16 : nullptr);
         cleanup();
         if (r) return r;
         sc.ready = true;

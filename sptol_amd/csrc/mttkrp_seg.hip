@@ -83,20 +83,43 @@ __global__ void k_seg_pack(uint4 *__restrict__ rec, float *__restrict__ vout, co
     }
 }
 
+// Records in the tensor's own order, and 16-bit copies of the indices of
+// the modes with at most 2^16 rows (k16.k[m], null for the others: the sort
+// keys of those modes).
+struct Keys16 {
+    uint16_t *k[4];
+};
+
+__global__ void k_seg_pack_keys(uint4 *__restrict__ rec, const uint32_t *__restrict__ i0,
+                                const uint32_t *__restrict__ i1, const uint32_t *__restrict__ i2,
+                                const uint32_t *__restrict__ i3, const float *__restrict__ vals, uint64_t nnz,
+                                uint32_t nm, Keys16 k16)
+{
+    for (uint64_t z = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; z < nnz; z += (uint64_t)gridDim.x * blockDim.x) {
+        const uint4 r = make_uint4(i0[z], i1[z], i2[z], nm == 3 ? __float_as_uint(vals[z]) : i3[z]);
+        rec[z] = r;
+        if (k16.k[0]) k16.k[0][z] = (uint16_t)r.x;
+        if (k16.k[1]) k16.k[1][z] = (uint16_t)r.y;
+        if (k16.k[2]) k16.k[2][z] = (uint16_t)r.z;
+        if (nm == 4 && k16.k[3]) k16.k[3][z] = (uint16_t)r.w;
+    }
+}
+
 // rec[z] = rec0[perm[z]] (and vout[z] = vals[perm[z]]): one 16-byte gather
-// per nonzero from records packed in the tensor's own order.
-// Records through the sort permutation, and the touched-row bitmap from the
-// sorted rows (a row's first nonzero sets its bit).
+// per nonzero from records packed in the tensor's own order.  Also the
+// touched-row bitmap from the sorted rows (a row's first nonzero sets its
+// bit); K is the type of the sorted row keys.
+template <typename K>
 __global__ void k_seg_gather_rec(uint4 *__restrict__ rec, float *__restrict__ vout, const uint4 *__restrict__ rec0,
                                  const float *__restrict__ vals, const uint32_t *__restrict__ perm,
-                                 const uint32_t *__restrict__ rows, uint32_t *__restrict__ touched, uint64_t nnz)
+                                 const K *__restrict__ rows, uint32_t *__restrict__ touched, uint64_t nnz)
 {
     for (uint64_t z = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; z < nnz; z += (uint64_t)gridDim.x * blockDim.x) {
         const uint32_t p = perm[z];
         rec[z] = rec0[p];
         if (vout) vout[z] = vals[p];
         const uint32_t r = rows[z];
-        if (z == 0 || rows[z - 1] != r) atomicOr(&touched[r >> 5], 1u << (r & 31u));
+        if (z == 0 || (uint32_t)rows[z - 1] != r) atomicOr(&touched[r >> 5], 1u << (r & 31u));
     }
 }
 
@@ -151,21 +174,26 @@ int seg_lane_class(uint32_t glpn, int dtype)
 // Records of every nonzero in the tensor's own order (the layout of
 // SegView::rec), for seg_build's gather through a sort permutation.
 int seg_pack_unsorted(uint4 *rec0, uint32_t nmodes, uint64_t nnz, uint32_t *const *inds, const float *vals,
-                      hipStream_t s)
+                      hipStream_t s, uint16_t *const *keys16)
 {
     if (!nnz) return 0;
-    hipLaunchKernelGGL(k_seg_pack, dim3(seg_grid(nnz)), dim3(256), 0, s, rec0, (float *)nullptr, inds[0], inds[1],
-                       inds[2], nmodes == 4 ? inds[3] : inds[0], vals, (const uint32_t *)nullptr, nnz, nmodes);
+    Keys16 k16 = {};
+    for (uint32_t m = 0; m < nmodes && keys16; ++m) k16.k[m] = keys16[m];
+    hipLaunchKernelGGL(k_seg_pack_keys, dim3(seg_grid(nnz)), dim3(256), 0, s, rec0, inds[0], inds[1], inds[2],
+                       nmodes == 4 ? inds[3] : inds[0], vals, nnz, nmodes, k16);
     SPTOL_HIP(hipGetLastError(), "Seg view");
     return 0;
 }
 
 int seg_build(SegView &v, uint32_t nmodes, uint32_t mode, uint32_t nrows, uint64_t nnz, uint32_t *const *inds,
               const float *vals, bool copy_vals, hipStream_t s, const uint32_t *ndims, const uint32_t *perm,
-              const uint32_t *sorted_rows, const uint4 *rec0)
+              const uint32_t *sorted_rows, const uint4 *rec0, const uint16_t *sorted_rows16)
 {
     v.release();
-    if (perm && (!sorted_rows || !copy_vals)) SPTOL_CHECK(SPTERR_VALUE_ERROR, "Seg view", "perm needs sorted rows and copied values");
+    if (perm && ((!sorted_rows && !sorted_rows16) || !copy_vals))
+        SPTOL_CHECK(SPTERR_VALUE_ERROR, "Seg view", "perm needs sorted rows and copied values");
+    if (sorted_rows16 && (!perm || !rec0 || nrows > 65536u))
+        SPTOL_CHECK(SPTERR_VALUE_ERROR, "Seg view", "16-bit rows need a permutation, packed records and <= 2^16 rows");
     if (nmodes != 3 && nmodes != 4) SPTOL_CHECK(SPTERR_VALUE_ERROR, "Seg view", "nmodes must be 3 or 4");
     v.nmodes = nmodes;
     v.mode = mode;
@@ -194,9 +222,12 @@ int seg_build(SegView &v, uint32_t nmodes, uint32_t mode, uint32_t nrows, uint64
                 v.vals = const_cast<float *>(vals);
             }
         }
-        if (perm && rec0) {
-            hipLaunchKernelGGL(k_seg_gather_rec, dim3(seg_grid(nnz)), dim3(256), 0, s, v.rec, vout, rec0, vals, perm,
-                               sorted_rows, v.touched, nnz);
+        if (perm && rec0 && sorted_rows16) {
+            hipLaunchKernelGGL(k_seg_gather_rec<uint16_t>, dim3(seg_grid(nnz)), dim3(256), 0, s, v.rec, vout, rec0,
+                               vals, perm, sorted_rows16, v.touched, nnz);
+        } else if (perm && rec0) {
+            hipLaunchKernelGGL(k_seg_gather_rec<uint32_t>, dim3(seg_grid(nnz)), dim3(256), 0, s, v.rec, vout, rec0,
+                               vals, perm, sorted_rows, v.touched, nnz);
         } else {
             hipLaunchKernelGGL(k_seg_pack, dim3(seg_grid(nnz)), dim3(256), 0, s, v.rec, vout, inds[0], inds[1],
                                inds[2], nmodes == 4 ? inds[3] : inds[0], vals, perm, nnz, nmodes);

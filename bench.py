@@ -180,19 +180,23 @@ def alg_bytes(kname, nnz, dims, U, mode, R, nb=0, nk=0, factor_bytes=4):
 
 def timed_steps(lib, step, kname, steps, warmup, extra=("fill_zero", "mttkrp_seg_carry")):
     """Warm up, then time exactly `steps` calls of step() between device
-    synchronisations; kernel durations come from hipEvents recorded on the
-    launch stream (sptolProfile*)."""
+    synchronisations (the throughput; no instrumentation).  A second pass of
+    the same `steps` calls records hipEvents around every launch on its
+    stream (sptolProfile*) for the kernel durations."""
     for _ in range(warmup):
         step()
-    sync(lib)
-    lib.sptolProfileReset()
-    lib.sptolProfileEnable(1)
     sync(lib)
     t0 = time.perf_counter()
     for _ in range(steps):
         step()
     sync(lib)
     dt = time.perf_counter() - t0
+    lib.sptolProfileReset()
+    lib.sptolProfileEnable(1)
+    sync(lib)
+    for _ in range(steps):
+        step()
+    sync(lib)
     lib.sptolProfileEnable(0)
     kms, kn = prof(lib, kname)
     if kn == 0:   # the segmented kernel ran instead
@@ -242,10 +246,11 @@ def run_hicoo(args, lib):
         check(lib, lib.sptolDevMatrixCreate(C.byref(outs[m]), int(dims[m]), R, SPTOL_F32), "DevMatrixCreate")
     orders = [np.array([m, (m + 1) % 3, (m + 2) % 3], dtype=np.uint32) for m in range(3)]
 
+    hargs = [(dh, dm, ptr(orders[m], sptIndex), m, C.byref(outs[m]), None) for m in range(3)]
+
     def step_hicoo():
-        for m in range(3):
-            check(lib, lib.sptolDevMTTKRPHiCOO(dh, dm, ptr(orders[m], sptIndex), m, C.byref(outs[m]), None),
-                  "DevMTTKRPHiCOO")
+        for a in hargs:
+            check(lib, lib.sptolDevMTTKRPHiCOO(*a), "DevMTTKRPHiCOO")
 
     t = timed_steps(lib, step_hicoo, "mttkrp_hicoo", args.steps, args.warmup)
     dt, k_ms, kname = t["dt"], t["kernel_ms"], t["kernel"]
@@ -292,10 +297,11 @@ def run_hicoo(args, lib):
         for m in range(3):
             check(lib, lib.sptolDevCooPrepareMode(dc, m), "PrepareMode")
 
+        cargs = [(dc, dm, ptr(orders[m], sptIndex), m, C.byref(outs[m]), None) for m in range(3)]
+
         def step_coo():
-            for m in range(3):
-                check(lib, lib.sptolDevMTTKRP(dc, dm, ptr(orders[m], sptIndex), m, C.byref(outs[m]), None),
-                      "DevMTTKRP")
+            for a in cargs:
+                check(lib, lib.sptolDevMTTKRP(*a), "DevMTTKRP")
 
         tc = timed_steps(lib, step_coo, "mttkrp_coo", args.steps, args.warmup)
         dt_c, kc_ms = tc["dt"], tc["kernel_ms"]
@@ -365,8 +371,10 @@ def run_coo(args, lib):
     order = np.array([0, 1, 2], np.uint32)
     steps = max(args.steps, 200)
 
+    sargs = (h, dm, ptr(order, sptIndex), 0, C.byref(out), None)
+
     def step():
-        check(lib, lib.sptolDevMTTKRP(h, dm, ptr(order, sptIndex), 0, C.byref(out), None), "mttkrp")
+        check(lib, lib.sptolDevMTTKRP(*sargs), "mttkrp")
 
     t = timed_steps(lib, step, "mttkrp_coo", steps, args.warmup)
     U = [distinct_count(inds[m], d[m]) for m in range(3)]
@@ -563,9 +571,11 @@ def run_batch(args, lib):
         total += len(vals)
     log(f"batch: {ntens} tensors, {total} nnz prepared in {time.time() - t0:.1f}s")
 
+    pargs = [(h, dm, ptr(order, sptIndex), mode, C.byref(out), None) for h, dm, out, order, mode, _ in items]
+
     def plain():
-        for h, dm, out, order, mode, _ in items:
-            check(lib, lib.sptolDevMTTKRP(h, dm, ptr(order, sptIndex), mode, C.byref(out), None), "mttkrp")
+        for a in pargs:
+            check(lib, lib.sptolDevMTTKRP(*a), "mttkrp")
 
     tp = timed_steps(lib, plain, "mttkrp_coo", args.steps, args.warmup)
     b = C.c_void_p()

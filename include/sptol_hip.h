@@ -213,7 +213,7 @@ typedef struct {
     int    exact_fit;       /* 1 = FP64 pass for <X,K> (see sptolCpdStats.exact_fit)         */
     int    blocks_per_cu;   /* persistent-kernel residency (0 = default 8)                  */
     double tile_scale;      /* batched tile length multiplier (0 = default 1.0)             */
-    int    admit_threads;   /* parallel admission workers, each with its own stream (0 = 4) */
+    int    admit_threads;   /* parallel admission workers, each with its own stream: 1..4 (0 = 4) */
     int    borrow_inputs;   /* 1 = Submit keeps pointers to the caller's index and value
                                arrays, which must stay valid and unchanged until the
                                request has been waited for; 0 = Submit copies them     */

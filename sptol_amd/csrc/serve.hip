@@ -9,13 +9,15 @@
 //     all slots are full, rejecting requests that could never fit), upload
 //     and prepare them, and hand them to the compute thread -- so uploads
 //     and preparation overlap with each other and with compute;
-//   * a compute thread advances every active request by one ALS mode-step
-//     per tick: the MTTKRPs of all of them run as one persistent batched
-//     launch, and their FP64 solve / normalise / Gram (and fit) steps as
-//     one launch per phase over all of them; at the end of an iteration the
-//     request's fit is copied back
-//     behind an event, and the request sits out ticks until the event has
-//     completed, so the host never blocks the device to read a fit;
+//   * a compute thread advances the active requests by one ALS mode-step
+//     per tick.  The requests form two cohorts (balanced by nonzeros), each
+//     on its own compute stream, and ticks alternate between them.  Within
+//     a tick the MTTKRPs of a cohort run as one persistent batched launch,
+//     and their FP64 solve / normalise / Gram (and fit) steps as one launch
+//     per phase.  At the end of an iteration the fit kernel stores the
+//     request's fit in its pinned slot, and the request sits out ticks until
+//     the event behind it has completed, so the host never blocks the
+//     device to read a fit;
 //   * a finished request's final kernels are queued behind an event and the
 //     request goes to a completion thread, which copies its factors to host
 //     memory, returns its device memory to the block cache (no device-wide
@@ -26,7 +28,9 @@
 // owns its tensor, factors and workspace, and the batched kernel writes only
 // each request's own output.
 #include "cpd_job.hpp"
+#include <algorithm>
 #include <atomic>
+#include <cstdlib>
 #include <condition_variable>
 #include <deque>
 #include <mutex>
@@ -74,6 +78,7 @@ struct Request {
     sptolDevMatrix mats[kMaxModes] = {};
     CpdJob *job = nullptr;
     hipEvent_t fit_event = nullptr;
+    int cohort = 0;                // compute stream that runs its iterations
     // results
     std::vector<float> lambda;
     std::vector<std::vector<float>> factors;   // per mode, ndims[m] x stride
@@ -90,6 +95,11 @@ struct Request {
         if (fit_event) { (void)hipEventDestroy(fit_event); fit_event = nullptr; }
     }
 };
+
+// Compute streams (cohorts of active requests; 3 or 4 measured slower than 2)
+// and the most admission workers a server runs.
+constexpr int kCohorts = 2;
+constexpr int kMaxAdmitThreads = 4;
 
 static size_t estimate_bytes(const Request &r)
 {
@@ -139,7 +149,7 @@ struct sptolServer_s {
     std::thread compute_thr, release_thr;
     bool stop_release = false;
     std::vector<hipStream_t> admit_stream;
-    hipStream_t compute_stream = nullptr, release_stream = nullptr;
+    hipStream_t compute_stream[kCohorts] = {}, release_stream = nullptr;
     int device = 0;
     std::atomic<uint64_t> ticks{0};
 };
@@ -276,7 +286,7 @@ static void admit_loop(sptolServer srv, int worker)
 // it to the completion thread.
 static void complete(sptolServer srv, Request *r)
 {
-    hipStream_t s = srv->compute_stream;
+    hipStream_t s = srv->compute_stream[r->cohort];
     r->lambda.resize(r->R);
     int err = r->job->finish_enqueue(s);
     if (!err) {
@@ -336,27 +346,42 @@ static void release_loop(sptolServer srv)
     }
 }
 
+// The compute thread.  Active requests are split into two cohorts, each with
+// its own compute stream and batch buffers, and ticks alternate between
+// them: one cohort's solve and Gram kernels and the tail of its MTTKRP
+// launches overlap the other cohort's MTTKRP.  A request's arithmetic does
+// not depend on which requests share its batch, so results do not depend
+// on the cohorts.
 static void compute_loop(sptolServer srv)
 {
     (void)hipSetDevice(srv->device);
-    hipStream_t s = srv->compute_stream;
     std::vector<Request *> active;
-    BatchBuffers bb;
-    AlsBatchBuffers ab;
+    BatchBuffers bb[kCohorts];
+    AlsBatchBuffers ab[kCohorts];
     std::vector<BatchDesc> descs[2];
     std::vector<Request *> stepped;
     std::vector<CpdJob *> jobs;
+    const int nc = kCohorts;
+    uint64_t load[kCohorts] = {};   // nonzeros of each cohort's active requests
+    int turn = 0;
     for (;;) {
         {
             std::unique_lock<std::mutex> lk(srv->mu);
             if (active.empty())
                 srv->cv_compute.wait(lk, [&] { return srv->stop || !srv->ready.empty(); });
             if (srv->stop && srv->ready.empty() && active.empty()) return;
-            for (auto *r : srv->ready) active.push_back(r);
+            for (auto *r : srv->ready) {
+                int best = 0;
+                for (int k = 1; k < nc; ++k)
+                    if (load[k] < load[best]) best = k;
+                r->cohort = best;
+                load[r->cohort] += r->nnz;
+                active.push_back(r);
+            }
             srv->ready.clear();
         }
         // collect fits that have arrived; finish converged requests
-        bool any_ready = false;
+        bool ready_in[kCohorts] = {};
         for (size_t i = 0; i < active.size();) {
             Request *r = active[i];
             if (r->job->fit_pending) {
@@ -364,27 +389,32 @@ static void compute_loop(sptolServer srv)
                 r->job->fit_ready();
             }
             if (r->job->done) {
+                load[r->cohort] -= r->nnz;
                 complete(srv, r);
                 active[i] = active.back();
                 active.pop_back();
                 continue;
             }
-            any_ready = true;
+            ready_in[r->cohort] = true;
             ++i;
         }
         if (active.empty()) continue;
-        if (!any_ready) {
+        if (std::none_of(ready_in, ready_in + kCohorts, [](bool b) { return b; })) {
             // every active request waits on its fit: block on the oldest
             for (auto *r : active)
                 if (r->job->fit_pending) { (void)hipEventSynchronize(r->fit_event); break; }
             continue;
         }
-        // one tick: the current MTTKRP of every ready request
+        // one tick of one cohort: the current MTTKRP of its ready requests
+        int c = turn;
+        while (!ready_in[c]) c = (c + 1) % nc;
+        turn = (c + 1) % nc;
+        hipStream_t s = srv->compute_stream[c];
         descs[0].clear();
         descs[1].clear();
         stepped.clear();
         for (auto *r : active) {
-            if (r->job->fit_pending || r->job->done) continue;
+            if (r->cohort != c || r->job->fit_pending || r->job->done) continue;
             CpdJob &j = *r->job;
             uint32_t o[kMaxModes];
             j.order(o);
@@ -403,10 +433,11 @@ static void compute_loop(sptolServer srv)
             // as one set of launches
             StreamScope scope(s);   // batch buffers that grow were used on s only
             for (int k = 0; k < 2 && !err; ++k)
-                err = batch_launch(descs[k], k ? SPTOL_BF16 : SPTOL_F32, srv->cfg.tile_scale, srv->cfg.blocks_per_cu, bb, s);
+                err = batch_launch(descs[k], k ? SPTOL_BF16 : SPTOL_F32, srv->cfg.tile_scale, srv->cfg.blocks_per_cu,
+                                   bb[c], s);
             jobs.clear();
             for (auto *r : stepped) jobs.push_back(r->job);
-            if (!err) err = als_step_batch(jobs, ab, s);
+            if (!err) err = als_step_batch(jobs, ab[c], s);
         }
         for (auto *r : stepped) {
             int e = err;
@@ -419,6 +450,7 @@ static void compute_loop(sptolServer srv)
             if (e) {
                 (void)hipStreamSynchronize(s);
                 r->job->done = true;
+                load[r->cohort] -= r->nnz;
                 StreamScope scope(s);
                 r->free_device();
                 finish_request(srv, r, e);
@@ -438,14 +470,20 @@ int sptolServerCreate(sptolServer *out, const sptolServerConfig *cfg)
     *out = nullptr;
     if (!cfg || cfg->hbm_budget == 0 || cfg->max_active < 1)
         SPTOL_CHECK(SPTERR_VALUE_ERROR, "Serve", "hbm_budget > 0 and max_active >= 1 required");
+    // More than 4 admission workers is refused: a stream with 8 workers and
+    // 96 slots failed view preparation with runtime errors and, once,
+    // faulted the device, for a reason not yet found.
+    if (cfg->admit_threads > kMaxAdmitThreads)
+        SPTOL_CHECK(SPTERR_VALUE_ERROR, "Serve", "admit_threads must be at most 4");
     sptolServer s = new sptolServer_s();
     s->cfg = *cfg;
     if (!(s->cfg.tile_scale > 0)) s->cfg.tile_scale = 1.0;
     if (s->cfg.blocks_per_cu < 1) s->cfg.blocks_per_cu = 8;
-    if (s->cfg.admit_threads < 1) s->cfg.admit_threads = 4;
-    if (s->cfg.admit_threads > 16) s->cfg.admit_threads = 16;
+    if (s->cfg.admit_threads < 1) s->cfg.admit_threads = kMaxAdmitThreads;
     (void)hipGetDevice(&s->device);
-    hipError_t e = hipStreamCreateWithFlags(&s->compute_stream, hipStreamNonBlocking);
+    hipError_t e = hipSuccess;
+    for (int k = 0; k < kCohorts && e == hipSuccess; ++k)
+        e = hipStreamCreateWithFlags(&s->compute_stream[k], hipStreamNonBlocking);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->release_stream, hipStreamNonBlocking);
     for (int w = 0; w < s->cfg.admit_threads && e == hipSuccess; ++w) {
         hipStream_t as = nullptr;
@@ -454,7 +492,8 @@ int sptolServerCreate(sptolServer *out, const sptolServerConfig *cfg)
     }
     if (e != hipSuccess) {
         for (auto as : s->admit_stream) (void)hipStreamDestroy(as);
-        if (s->compute_stream) (void)hipStreamDestroy(s->compute_stream);
+        for (auto cs : s->compute_stream)
+            if (cs) (void)hipStreamDestroy(cs);
         if (s->release_stream) (void)hipStreamDestroy(s->release_stream);
         delete s;
         SPTOL_HIP(e, "Serve");
@@ -643,7 +682,8 @@ void sptolServerDestroy(sptolServer srv)
         delete kv.second;
     }
     for (auto as : srv->admit_stream) (void)hipStreamDestroy(as);
-    (void)hipStreamDestroy(srv->compute_stream);
+    for (auto cs : srv->compute_stream)
+        if (cs) (void)hipStreamDestroy(cs);
     (void)hipStreamDestroy(srv->release_stream);
     delete srv;
 }

@@ -250,3 +250,16 @@ def test_binary_roundtrip_and_rejects(lib, tmp_path):
     hdr[hdr_bytes:hdr_bytes + 4] = np.uint32(d[0]).tobytes()    # inds[0][0] = ndims[0]
     open(bad, "wb").write(bytes(hdr))
     assert lib.sptolLoadSparseTensorBinary(C.byref(sptSparseTensor()), bad) == SPTERR_VALUE_ERROR
+
+
+def test_server_rejects_more_than_four_admission_threads(lib):
+    """sptolServerCreate refuses admit_threads above 4 before touching the
+    device (no stream or thread is created)."""
+    from sptol_amd.parti import SPTERR_VALUE_ERROR, sptolServerConfig
+    cfg = sptolServerConfig()
+    cfg.hbm_budget = 1 << 30
+    cfg.max_active = 4
+    cfg.admit_threads = 8
+    srv = C.c_void_p()
+    assert lib.sptolServerCreate(C.byref(srv), C.byref(cfg)) == SPTERR_VALUE_ERROR
+    assert not srv.value

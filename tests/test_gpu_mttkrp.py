@@ -88,6 +88,27 @@ def test_ranks_and_padding(gpu_lib, R):
             _check(got, ref32, ref64, R)
 
 
+@pytest.mark.parametrize("R", [1, 5, 8, 16, 24, 32, 64, 100])
+def test_ranks_and_padding_bf16(gpu_lib, R):
+    """BF16-stored factors at every lane shape of the segmented kernel (1, 2,
+    4 and 8 lanes of 8 columns, idle lanes at R=24; R above 64 takes the
+    atomic kernel), 3-D and 4-D, every mode, sorted view and not, against
+    the oracle run on the same BF16-rounded factors."""
+    for dims in ([400, 300, 500], [60, 50, 40, 30]):
+        d, inds, vals = gen.zipf_tensor(dims, 30000, 1.1, seed=R + len(dims))
+        A = gen.uniform_factors(d, R, seed=R + 2)
+        Au = [bf16_round(a) for a in A]
+        N = len(d)
+        for mode in range(N):
+            _say("rank bf16", R, dims, mode)
+            order = [mode] + [(mode + i) % N for i in range(1, N)]
+            ref32 = oracle.mttkrp_coo(inds, vals, Au, order, mode)
+            ref64 = oracle.mttkrp_coo(inds, vals, Au, order, mode, dtype=np.float64)
+            for prepare in (False, True):
+                got = dev_mttkrp_coo(gpu_lib, d, inds, vals, A, order, mode, prepare=prepare, dtype=SPTOL_BF16)
+                _check(got, ref32, ref64, R)
+
+
 def test_higher_order_and_edge_cases(gpu_lib):
     # 5-D tensor (generic-order kernel path)
     d, inds, vals = gen.uniform_tensor([20, 15, 10, 12, 9], 20000, seed=3)

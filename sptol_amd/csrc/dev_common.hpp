@@ -130,8 +130,9 @@ uint32_t seg_lpn(uint32_t stride);
 // kernels ({2, 4, 8, 16} lanes for FP32, {1, 2, 4, 8} for BF16).
 uint32_t seg_glpn(uint32_t stride, int dtype);
 int seg_lane_class(uint32_t glpn, int dtype);
-// Nonzeros per lane group for a tensor of nnz nonzeros and G groups per wave.
-uint32_t seg_chg(uint64_t nnz, uint32_t G);
+// Nonzeros per lane group for a tensor of nnz nonzeros and G groups per
+// wave; l2_factors: the gathered factor rows total at most 4 MiB.
+uint32_t seg_chg(uint64_t nnz, uint32_t G, bool l2_factors);
 // out[0 .. nrows*stride) = MTTKRP of the view (no fill needed beforehand).
 int seg_mttkrp(SegView &v, const void *const *mats, const uint32_t *order, uint32_t R, uint32_t stride,
                int dtype, float *out, hipStream_t s);

@@ -243,18 +243,25 @@ int seg_build(SegView &v, uint32_t nmodes, uint32_t mode, uint32_t nrows, uint64
 }
 
 // Nonzeros per lane group for a tensor of nnz nonzeros: about 32 ranges per
-// compute unit, between 16 and 256 (SPTOL_SEG_CHG overrides, for tuning).
-uint32_t seg_chg(uint64_t nnz, uint32_t G)
+// compute unit, between 16 and 256; when the gathered factors fit in one
+// XCD's L2, about 16 ranges per compute unit (one per resident wave) and up
+// to 768 (SPTOL_SEG_CHG overrides, for tuning).  With factors in HBM,
+// shorter ranges balance the waves better (the 12M x 9M x 28M HiCOO
+// headline: 42.1 Gnnz/s at 256, 41.0 at 512); with factors in L2 the
+// per-range work (merge, carry, zero fill) counts more (the 4-D CPD at R=32,
+// per iteration: FP32 4.24 ms at 256 and 4.12 at 768, BF16 3.28 and 3.22).
+uint32_t seg_chg(uint64_t nnz, uint32_t G, bool l2_factors)
 {
     static const int env = env_int("SPTOL_SEG_CHG");
     uint64_t c;
     if (env > 0) {
         c = (uint64_t)env;
     } else {
-        const uint64_t target = (uint64_t)num_cus() * 32;
+        const uint64_t target = (uint64_t)num_cus() * (l2_factors ? 16 : 32);
         c = (nnz + G * target - 1) / (G * target);
+        const uint64_t cap = l2_factors ? 768 : 256;
         if (c < 16) c = 16;
-        if (c > 256) c = 256;
+        if (c > cap) c = cap;
     }
     c = (c + 7) / 8 * 8;
     return (uint32_t)c;
@@ -357,7 +364,12 @@ int seg_mttkrp(SegView &v, const void *const *mats, const uint32_t *order, uint3
                float *out, hipStream_t s)
 {
     SegArgs a;
-    SPTOL_TRY(seg_args(v, mats, order, R, stride, dtype, out, seg_chg(v.nnz, 64 / seg_glpn(stride, dtype)), a));
+    // bytes of the factor rows the launch gathers from
+    uint64_t fbytes = 0;
+    for (uint32_t k = 1; k < v.nmodes; ++k)
+        fbytes += (uint64_t)v.ndims[order[k]] * stride * (dtype == SPTOL_BF16 ? 2u : 4u);
+    const bool l2 = fbytes <= ((uint64_t)4 << 20);
+    SPTOL_TRY(seg_args(v, mats, order, R, stride, dtype, out, seg_chg(v.nnz, 64 / seg_glpn(stride, dtype), l2), a));
     int tok = prof_start("mttkrp_seg", s);
     if (dtype == SPTOL_BF16) seg_launch_ft<uint16_t>(a, v.counter, s);
     else seg_launch_ft<float>(a, v.counter, s);

@@ -162,6 +162,25 @@ def test_c1_full_size(gpu_lib):
              f"reference FP32 vs FP64 {rel_fro(ref32, ref64):.2e}")
 
 
+def test_long_ranges_small_factors(gpu_lib):
+    """20M nonzeros, R=32, factors small enough for L2 (at most 384 KB
+    gathered per mode): the view's lane groups take about 616 (FP32) or 312
+    (BF16) nonzeros each, above the 256 used when factors live in HBM. Every
+    mode, FP32 and BF16."""
+    d, inds, vals = gen.uniform_tensor([3000, 2000, 1000], 20_000_000, seed=77)
+    A = gen.uniform_factors(d, 32, seed=78)
+    h = dev_coo(gpu_lib, d, inds, vals)
+    for mode in range(3):
+        order = [mode, (mode + 1) % 3, (mode + 2) % 3]
+        for dtype in (0, SPTOL_BF16):
+            _say("long ranges", mode, dtype)
+            Au = [bf16_round(a) for a in A] if dtype else A
+            got = dev_mttkrp_coo(gpu_lib, d, inds, vals, A, order, mode, prepare=True, dtype=dtype, h=h)
+            _check(got, oracle.mttkrp_coo(inds, vals, Au, order, mode),
+                   oracle.mttkrp_coo(inds, vals, Au, order, mode, dtype=np.float64), 32)
+    gpu_lib.sptolDevCooFree(h)
+
+
 def test_bf16_factors(gpu_lib):
     d, inds, vals = gen.zipf_tensor([3000, 2000, 4000], 100000, 1.1, seed=9)
     A = gen.uniform_factors(d, 32, seed=10)

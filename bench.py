@@ -33,7 +33,7 @@ sys.path.insert(0, ROOT)
 
 from sptol_amd import gen  # noqa: E402
 from sptol_amd.parti import (CooTensor, Matrix, SPTOL_BF16, SPTOL_F32, check, load_library, ptr,  # noqa: E402
-                             sptIndex, sptSparseTensor, sptolDevMatrix, to_hicoo)
+                             sptIndex, sptSparseTensor, sptolDevMatrix, to_hicoo, to_hicoo_device)
 
 HBM_PEAK = 8.0e12
 DIMS = [12_000_000, 9_000_000, 28_000_000]
@@ -222,12 +222,23 @@ def run_hicoo(args, lib):
     factors = gen.uniform_factors(dims, R, seed=7)
     coo_keep = [a.copy() for a in inds]
 
-    t0 = time.time()
     hcoo = CooTensor(dims, inds, vals.copy())
+    t0 = time.perf_counter()
     h, max_nnzb = to_hicoo(lib, hcoo, 7, 14)
+    t_host = time.perf_counter() - t0
     nb = int(h.bptr.len) - 1
     nk = int(h.kptr.len) - 1
-    log(f"HiCOO: nb={nb} nk={nk} max_nnzb={max_nnzb} in {time.time() - t0:.1f}s")
+    log(f"HiCOO: nb={nb} nk={nk} max_nnzb={max_nnzb} in {t_host:.1f}s")
+    # the GPU converter on a copy of the same input: COO upload, sorts and
+    # block arrays on the device, HiCOO arrays (and the permuted COO) back
+    dcoo = CooTensor(dims, [a.copy() for a in coo_keep], vals.copy())
+    sync(lib)
+    t0 = time.perf_counter()
+    hd, _ = to_hicoo_device(lib, dcoo, 7, 14)
+    t_dev = time.perf_counter() - t0
+    lib.sptFreeSparseTensorHiCOO(C.byref(hd))
+    del dcoo
+    log(f"HiCOO on the GPU in {t_dev:.2f}s")
 
     dh = C.c_void_p()
     check(lib, lib.sptolDevHicooCreate(C.byref(dh), C.byref(h)), "DevHicooCreate")
@@ -288,6 +299,10 @@ def run_hicoo(args, lib):
                      "algorithmic_bytes_per_launch": int(B_avg),
                      "carry_kernel_avg_ms": round(t["mttkrp_seg_carry_ms"], 4),
                      "fill_zero_avg_ms": round(t["fill_zero_ms"], 4)},
+        # sptSparseTensorToHiCOO (convert.c:300) on the host and
+        # sptolSparseTensorToHiCOODevice, host arrays in and out, same layout
+        "hicoo_conversion": {"host_s": round(t_host, 3), "device_s": round(t_dev, 3),
+                             "host_threads": host_threads()},
     }
 
     if not args.no_coo:

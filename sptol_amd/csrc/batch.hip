@@ -235,17 +235,18 @@ static uint32_t seg_tile_chg(double scale)
 
 // Plan and upload the segmented items (a stream-ordered copy from pinned
 // staging).
-static int seg_batch_prepare(const std::vector<const BatchDesc *> &items, int dtype, double scale, SegBatchDev &dev,
-                             hipStream_t s)
+// scale[c]: the tile factor of class c.
+static int seg_batch_prepare(const std::vector<const BatchDesc *> &items, int dtype, const double *scale,
+                             SegBatchDev &dev, hipStream_t s)
 {
     std::vector<SegArgs> cls[8];
-    const uint32_t chg = seg_tile_chg(scale);
     uint64_t ranges = 0;
     for (const BatchDesc *d : items) {
+        const int c = seg_class(seg_glpn(d->stride, dtype), d->nmodes, dtype);
         SegArgs a;
-        SPTOL_TRY(seg_fill_args(*d->seg, d->mats, d->order, d->R, d->stride, dtype, d->out, chg, a));
+        SPTOL_TRY(seg_fill_args(*d->seg, d->mats, d->order, d->R, d->stride, dtype, d->out, seg_tile_chg(scale[c]), a));
         ranges += a.nwaves;
-        cls[seg_class(a.glpn, a.nmodes, dtype)].push_back(a);
+        cls[c].push_back(a);
     }
     if (dev.carry_cap < ranges) {
         dfree(dev.carry); dev.carry = nullptr;
@@ -320,10 +321,11 @@ static void seg_batch_class(const SegBatchDev &dev, int c, int blocks_per_cu, hi
 }
 
 template <typename FT>
-static void seg_batch_kernels_ft(const SegBatchDev &dev, int bpc, hipStream_t s)
+static void seg_batch_kernels_ft(const SegBatchDev &dev, const int *bpc_of, hipStream_t s)
 {
     for (int c = 0; c < 8; ++c) {
         if (!dev.n[c]) continue;
+        const int bpc = bpc_of[c];
         // class = lane class * 2 + (4 modes)
         constexpr int L1 = sizeof(FT) == 2 ? 1 : 2;
         switch (c) {
@@ -339,8 +341,9 @@ static void seg_batch_kernels_ft(const SegBatchDev &dev, int bpc, hipStream_t s)
     }
 }
 
-// Class kernels then the carry kernel (which resets the class counters).
-static int seg_batch_kernels(const SegBatchDev &dev, int dtype, int blocks_per_cu, hipStream_t s)
+// Class kernels (blocks_per_cu[c] for class c) then the carry kernel (which
+// resets the class counters).
+static int seg_batch_kernels(const SegBatchDev &dev, int dtype, const int *blocks_per_cu, hipStream_t s)
 {
     if (!dev.n_all) return 0;
     int tok = prof_start("batch_seg", s);
@@ -357,9 +360,31 @@ static int seg_batch_kernels(const SegBatchDev &dev, int dtype, int blocks_per_c
     return 0;
 }
 
+// Launch configuration of a batch: tile_scale multiplies the base tile
+// length and blocks_per_cu caps the persistent grid.  The pair applies to
+// the atomic path and, unless per_class is set, to every segmented class;
+// the autotuner may then refine each class (cls_*), since the classes are
+// separate launches.
 struct TuneCfg {
-    double tile_scale = 1.0;     // multiplies the base tile length
+    double tile_scale = 1.0;
     int blocks_per_cu = 8;
+    bool per_class = false;
+    double cls_scale[8] = {};
+    int cls_bpc[8] = {};
+    void seg_scales(double *sc) const
+    {
+        for (int c = 0; c < 8; ++c) sc[c] = per_class ? cls_scale[c] : tile_scale;
+    }
+    void seg_bpcs(int *b) const
+    {
+        for (int c = 0; c < 8; ++c) b[c] = per_class ? cls_bpc[c] : blocks_per_cu;
+    }
+    void make_per_class()
+    {
+        if (per_class) return;
+        for (int c = 0; c < 8; ++c) { cls_scale[c] = tile_scale; cls_bpc[c] = blocks_per_cu; }
+        per_class = true;
+    }
 };
 
 static std::map<std::string, TuneCfg> g_tune_cache;
@@ -430,7 +455,9 @@ static int build_plan(sptolBatch b)
     split_descs(b->descs, segs, leg);
     if (b->exec) { (void)hipGraphExecDestroy(b->exec); b->exec = nullptr; }
     if (b->graph) { (void)hipGraphDestroy(b->graph); b->graph = nullptr; }
-    SPTOL_TRY(seg_batch_prepare(segs, b->dtype, b->cfg.tile_scale, b->seg, nullptr));
+    double sc[8];
+    b->cfg.seg_scales(sc);
+    SPTOL_TRY(seg_batch_prepare(segs, b->dtype, sc, b->seg, nullptr));
     const size_t n = leg.size();
     b->nlegacy = (uint32_t)n;
     dfree(b->d_desc); dfree(b->d_tile_prefix); dfree(b->d_fill_prefix);
@@ -484,7 +511,9 @@ static int launch_batch(sptolBatch b, hipStream_t s)
 {
     SPTOL_TRY(launch_legacy(b->d_desc, b->d_tile_prefix, b->d_fill_prefix, b->d_counter, b->nlegacy, b->ntiles,
                             b->nchunks, b->dtype, b->cfg.blocks_per_cu, s));
-    return seg_batch_kernels(b->seg, b->dtype, b->cfg.blocks_per_cu, s);
+    int bpc[8];
+    b->cfg.seg_bpcs(bpc);
+    return seg_batch_kernels(b->seg, b->dtype, bpc, s);
 }
 
 void make_desc(sptolDevCoo X, const sptolDevMatrix *mats, const uint32_t *mats_order, uint32_t mode,
@@ -522,8 +551,11 @@ int batch_launch(std::vector<BatchDesc> &descs, int dtype, double tile_scale, in
     std::vector<BatchDesc> leg;
     split_descs(descs, segs, leg);
     if (!segs.empty()) {
-        SPTOL_TRY(seg_batch_prepare(segs, dtype, tile_scale, bb.seg, s));
-        SPTOL_TRY(seg_batch_kernels(bb.seg, dtype, blocks_per_cu, s));
+        double sc[8];
+        int bpc[8];
+        for (int c = 0; c < 8; ++c) { sc[c] = tile_scale; bpc[c] = blocks_per_cu; }
+        SPTOL_TRY(seg_batch_prepare(segs, dtype, sc, bb.seg, s));
+        SPTOL_TRY(seg_batch_kernels(bb.seg, dtype, bpc, s));
     }
     const size_t n = leg.size();
     if (n == 0) return 0;
@@ -606,6 +638,7 @@ int sptolBatchSetConfig(sptolBatch b, double tile_scale, int blocks_per_cu)
         SPTOL_CHECK(SPTERR_VALUE_ERROR, "Batch Config", "tile_scale > 0, blocks_per_cu in [1, 32]");
     b->cfg.tile_scale = tile_scale;
     b->cfg.blocks_per_cu = blocks_per_cu;
+    b->cfg.per_class = false;
     return build_plan(b);
 }
 
@@ -651,9 +684,11 @@ int sptolBatchCapture(sptolBatch b, void *stream)
 }
 
 // Autotuner: time every (tile scale, blocks per CU) candidate on the batch
-// (median of `reps` runs after one warm-up), keep the fastest, and cache it
-// under the batch's signature (factor dtype, rank classes, log2 nonzeros per
-// class).  A cached signature is reused without timing.
+// (median of `reps` runs after one warm-up) and keep the fastest; then, for
+// each segmented class in turn, every candidate for that class alone with
+// the others held, keeping any that shortens the whole launch.  The result
+// is cached under the batch's signature (factor dtype, rank classes, log2
+// nonzeros per class); a cached signature is reused without timing.
 int sptolBatchAutotune(sptolBatch b, int reps, void *stream)
 {
     if (b->descs.empty()) return 0;
@@ -666,36 +701,65 @@ int sptolBatchAutotune(sptolBatch b, int reps, void *stream)
     hipStream_t s = stream_of(stream);
     const double scales[] = {0.25, 0.5, 1.0, 2.0, 4.0};
     const int occs[] = {2, 4, 8, 16};
-    double best = 1e300;
-    TuneCfg bestc = b->cfg;
     hipEvent_t e0, e1;
     SPTOL_HIP(hipEventCreate(&e0), "Autotune");
     SPTOL_HIP(hipEventCreate(&e1), "Autotune");
     if (reps < 1) reps = 1;
+    // median time of one batch launch under b->cfg (plan rebuilt for it)
+    auto time_cfg = [&](double &ms_out) {
+        int r = build_plan(b);
+        if (!r) r = launch_batch(b, s);   // warm-up
+        std::vector<float> t;
+        for (int k = 0; k < reps && !r; ++k) {
+            (void)hipEventRecord(e0, s);
+            r = launch_batch(b, s);
+            (void)hipEventRecord(e1, s);
+            (void)hipEventSynchronize(e1);
+            float ms = 0;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            t.push_back(ms);
+        }
+        if (r) return r;
+        std::sort(t.begin(), t.end());
+        ms_out = t[t.size() / 2];
+        return 0;
+    };
+    int r = 0;
+    double best = 1e300;
+    TuneCfg bestc = b->cfg;
+    // one pair for every class (and the atomic path)
     for (double sc : scales)
         for (int oc : occs) {
+            if (r) break;
+            b->cfg = TuneCfg();
             b->cfg.tile_scale = sc;
             b->cfg.blocks_per_cu = oc;
-            int r = build_plan(b);
-            if (r) { (void)hipEventDestroy(e0); (void)hipEventDestroy(e1); return r; }
-            r = launch_batch(b, s);   // warm-up
-            std::vector<float> t;
-            for (int k = 0; k < reps && !r; ++k) {
-                (void)hipEventRecord(e0, s);
-                r = launch_batch(b, s);
-                (void)hipEventRecord(e1, s);
-                (void)hipEventSynchronize(e1);
-                float ms = 0;
-                (void)hipEventElapsedTime(&ms, e0, e1);
-                t.push_back(ms);
-            }
-            if (r) { (void)hipEventDestroy(e0); (void)hipEventDestroy(e1); return r; }
-            std::sort(t.begin(), t.end());
-            const double med = t[t.size() / 2];
-            if (med < best) { best = med; bestc = b->cfg; }
+            double ms = 0;
+            r = time_cfg(ms);
+            if (!r && ms < best) { best = ms; bestc = b->cfg; }
         }
+    // then each segmented class on its own, the others held at their best
+    for (int c = 0; c < 8 && !r; ++c) {
+        if (!b->seg.n[c]) continue;
+        b->cfg = bestc;
+        double ms = 0;
+        r = time_cfg(ms);   // the current best, timed again beside the candidates
+        if (r) break;
+        best = ms;
+        for (double sc : scales)
+            for (int oc : occs) {
+                if (r) break;
+                b->cfg = bestc;
+                b->cfg.make_per_class();
+                b->cfg.cls_scale[c] = sc;
+                b->cfg.cls_bpc[c] = oc;
+                r = time_cfg(ms);
+                if (!r && ms < best) { best = ms; bestc = b->cfg; }
+            }
+    }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
+    if (r) return r;
     b->cfg = bestc;
     g_tune_cache[key] = bestc;
     return build_plan(b);

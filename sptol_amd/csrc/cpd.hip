@@ -242,15 +242,18 @@ __device__ __forceinline__ void factor_body(const double *__restrict__ ata, uint
     __shared__ double V[RM * (RM + 1)];   // LU fallback, rows padded against bank conflicts
     const uint32_t RR = R * R, t = threadIdx.x;
     const bool mine = t < R;
+    // row t of the Hadamard of the other modes' Grams, multiplied in mode
+    // order; each Gram's row is loaded as one batch of independent loads
     double row[RM];
 #pragma unroll
-    for (int k = 0; k < RM; ++k) {
-        double v = 0.0;
-        if (mine && k < (int)R) {
-            v = 1.0;
-            for (uint32_t q = 1; q < N; ++q) v *= ata[(size_t)((m + q) % N) * RR + t * R + k];
-        }
-        row[k] = v;
+    for (int k = 0; k < RM; ++k) row[k] = (mine && k < (int)R) ? 1.0 : 0.0;
+    for (uint32_t q = 1; q < N; ++q) {
+        const double *__restrict__ g = ata + (size_t)((m + q) % N) * RR + (size_t)(mine ? t : 0) * R;
+        double gv[RM];
+#pragma unroll
+        for (int k = 0; k < RM; ++k) gv[k] = (mine && k < (int)R) ? g[k] : 1.0;
+#pragma unroll
+        for (int k = 0; k < RM; ++k) row[k] *= gv[k];
     }
     bool spd = true;                                    // wave-uniform
 #pragma unroll

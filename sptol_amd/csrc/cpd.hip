@@ -1005,6 +1005,18 @@ void CpdJob::advance()
     fit_pending = true;
 }
 
+void CpdJob::fit_skipped()
+{
+    fit_pending = false;
+    iters = (int)it + 1;
+    if (it + 1 >= niters) {
+        done = true;
+    } else {
+        ++it;
+        m = 0;
+    }
+}
+
 void CpdJob::fit_ready()
 {
     fit = *h_fit;
@@ -1085,15 +1097,25 @@ namespace sptol {
 static int run_job(CpdJob &job, sptValue *lambda_out, sptolCpdStats *stats, hipStream_t s,
                    std::chrono::steady_clock::time_point t0)
 {
+    // Without a positive tolerance (and without per-iteration prints) only
+    // the iteration count ends the loop: every iteration is queued without
+    // waiting for its fit, and the last fit is read after the final
+    // synchronisation.
+    const bool wait_fits = job.tol > 0.0 || verbose();
     while (!job.done) {
         SPTOL_TRY(job.mttkrp(s));
         SPTOL_TRY(job.after_mttkrp(s));
         if (job.fit_pending) {
-            SPTOL_HIP(hipStreamSynchronize(s), "CPD");
-            job.fit_ready();
+            if (wait_fits) {
+                SPTOL_HIP(hipStreamSynchronize(s), "CPD");
+                job.fit_ready();
+            } else {
+                job.fit_skipped();
+            }
         }
     }
     SPTOL_TRY(job.finish(s, lambda_out));
+    if (!wait_fits) job.fit = job.h_fit[0];   // the last iteration's fit, or the exact one
     if (stats) {
         stats->iterations = job.iters;
         stats->fit = job.fit;

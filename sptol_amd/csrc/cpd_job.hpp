@@ -79,6 +79,9 @@ struct CpdJob {
     // fit_pending.
     void advance();
     void fit_ready();
+    // End of an iteration whose fit is not read (no positive tolerance):
+    // advance on the iteration count alone.
+    void fit_skipped();
     // Final weights (and the exact fit when exact_fit): finish_enqueue
     // queues the kernels and the copies to the pinned slot; finish_read
     // takes the results once the stream has passed them.  finish does both.

@@ -82,6 +82,8 @@ hipStream_t stream_of(void *s);
 int verbose();
 
 constexpr int kMaxModes = 8;
+// Most items one device sort, scan or select takes (their counts are int).
+constexpr uint64_t kMaxSortItems = 0x7FFFFFFFull;
 constexpr uint32_t kNoRow = 0xFFFFFFFFu;
 
 // Compute units of the current device (cached).

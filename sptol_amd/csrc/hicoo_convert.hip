@@ -258,7 +258,7 @@ int sptolSparseTensorToHiCOODevice(sptSparseTensorHiCOO *hitsr, sptNnzIndex *max
     const uint32_t N = tsr->nmodes;
     const uint64_t nnz = tsr->nnz;
     if (N < 2 || N > 8) SPTOL_CHECK(SPTERR_VALUE_ERROR, mod, "nmodes must be in [2, 8]");
-    if (nnz >= 0x7FFFFFFFull) SPTOL_CHECK(SPTERR_VALUE_ERROR, mod, "nnz >= 2^31");
+    if (nnz > kMaxSortItems) SPTOL_CHECK(SPTERR_VALUE_ERROR, mod, "nnz > 2^31 - 1 (the device sort counts items in 32-bit ints)");
     hipStream_t s = nullptr;
     HcBufs bufs;
     int r = 0;

@@ -288,7 +288,8 @@ bool hicoo_tiled_ok(uint32_t sb_bits, uint32_t stride)
 int build_hicoo_view(sptolDevHicoo X, uint32_t mode, HicooView &v, hipStream_t s)
 {
     const uint64_t nb = X->nb, nnz = X->nnz;
-    if (nb >= 0xFFFFFFFFull) SPTOL_CHECK(SPTERR_VALUE_ERROR, "HiCOO view", "nb >= 2^32");
+    if (nb > kMaxSortItems)
+        SPTOL_CHECK(SPTERR_VALUE_ERROR, "HiCOO view", "nb > 2^31 - 1 (the device sort counts items in 32-bit ints)");
     uint32_t *perm = nullptr, *perm2 = nullptr, *key2 = nullptr, *start_block = nullptr, *tcoord = nullptr;
     uint64_t *sizes = nullptr, *tz0 = nullptr, *tz1 = nullptr;
     uint8_t *flags = nullptr;
@@ -425,7 +426,8 @@ int build_hicoo_seg(sptolDevHicoo X, uint32_t mode, HicooView &v, hipStream_t s)
 {
     const uint64_t nb = X->nb, nnz = X->nnz;
     const uint32_t N = X->nmodes;
-    if (nnz >= 0xFFFFFFFFull) SPTOL_CHECK(SPTERR_VALUE_ERROR, "HiCOO view", "nnz >= 2^32");
+    if (nnz > kMaxSortItems)
+        SPTOL_CHECK(SPTERR_VALUE_ERROR, "HiCOO view", "nnz > 2^31 - 1 (the device sort counts items in 32-bit ints)");
     uint32_t *coords[kMaxModes] = {}, *sorted[kMaxModes] = {};
     uint32_t *perm = nullptr, *perm2 = nullptr, *key2 = nullptr, **d_coords = nullptr;
     float *svals = nullptr;

@@ -627,7 +627,7 @@ int coo_prepare_all(sptolDevCoo h, hipStream_t s)
     bool todo = false;
     for (uint32_t m = 0; m < h->nmodes; ++m) todo = todo || !h->sorted[m].ready;
     if (!todo) return 0;
-    if ((h->nmodes != 3 && h->nmodes != 4) || h->nnz == 0 || h->nnz >= 0xFFFFFFFFull) {
+    if ((h->nmodes != 3 && h->nmodes != 4) || h->nnz == 0 || h->nnz > kMaxSortItems) {
         for (uint32_t m = 0; m < h->nmodes; ++m) SPTOL_TRY(coo_prepare(h, m, s));
         return 0;
     }
@@ -663,7 +663,8 @@ static int coo_prepare_from(sptolDevCoo h, uint32_t mode, hipStream_t s, const u
     SortedCopy &sc = h->sorted[mode];
     if (sc.ready || h->nnz == 0) { sc.ready = true; return 0; }
     StreamScope scope(s);   // everything freed here was used on s only
-    if (h->nnz >= 0xFFFFFFFFull) SPTOL_CHECK(SPTERR_VALUE_ERROR, "DevCoo Prepare", "nnz >= 2^32");
+    if (h->nnz > kMaxSortItems)
+        SPTOL_CHECK(SPTERR_VALUE_ERROR, "DevCoo Prepare", "nnz > 2^31 - 1 (the device sort counts items in 32-bit ints)");
     const uint64_t n = h->nnz;
     uint32_t *perm = nullptr, *perm2 = nullptr, *keys2 = nullptr;
     void *tmp = nullptr;

@@ -988,7 +988,11 @@ int sptMTTKRPHiCOO(sptSparseTensorHiCOO const * const hitsr, sptMatrix * mats[],
     SPTOL_TRY(check_host_mats(hitsr->nmodes, hitsr->ndims, mats, mode, "GPU HiCOO SpTns MTTKRP"));
     sptolDevHicoo dx = nullptr;
     SPTOL_TRY(sptolDevHicooCreate(&dx, hitsr));
-    int r = sptolDevHicooPrepareMode(dx, mode);
+    // the mode's view is built with device sorts (at most 2^31 - 1 items
+    // each); a larger tensor runs the block-order kernel instead
+    const uint64_t nb = hitsr->bptr.len ? hitsr->bptr.len - 1 : 0;
+    int r = 0;
+    if (hitsr->nnz <= kMaxSortItems && nb <= kMaxSortItems) r = sptolDevHicooPrepareMode(dx, mode);
     if (r) { sptolDevHicooFree(dx); return r; }
     r = with_device_factors(hitsr->nmodes, hitsr->ndims, mats, mode, [&](sptolDevMatrix *dm, sptolDevMatrix *dout) {
         return sptolDevMTTKRPHiCOO(dx, dm, mats_order, mode, dout, nullptr);

@@ -1135,7 +1135,9 @@ int sptolCpdAlsDevice(sptolDevCoo X, sptIndex rank, sptIndex niters, double tol,
     // the workspace below is freed after the final synchronisation
     StreamScope scope(s);
     auto t0 = std::chrono::steady_clock::now();
-    SPTOL_TRY(coo_prepare_all(X, s));
+    // mode views need device sorts (at most 2^31 - 1 items); a larger tensor
+    // runs the unsorted (atomic) MTTKRP
+    if (sptolDevCooNnz(X) <= kMaxSortItems) SPTOL_TRY(coo_prepare_all(X, s));
     CpdJob job;
     job.exact_fit = stats ? stats->exact_fit != 0 : false;
     SPTOL_TRY(job.setup(X, rank, niters, tol, mats, stats ? stats->factor_dtype : SPTOL_F32, s));

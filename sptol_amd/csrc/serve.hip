@@ -205,7 +205,9 @@ static int prepare_request(sptolServer srv, Request *r, hipStream_t s)
 // prepare every mode, create the factors and set up the job.
 static int prepare_loaded(sptolServer srv, Request *r, hipStream_t s)
 {
-    SPTOL_TRY(coo_prepare_all(r->X, s));
+    // mode views need device sorts (at most 2^31 - 1 items); a larger tensor
+    // runs the unsorted (atomic) batched MTTKRP
+    if (r->nnz <= kMaxSortItems) SPTOL_TRY(coo_prepare_all(r->X, s));
     uint64_t st = r->seed;
     for (uint32_t m = 0; m < r->nmodes; ++m) {
         SPTOL_TRY(sptolDevMatrixCreate(&r->mats[m], r->ndims[m], r->R, SPTOL_F32));

@@ -382,18 +382,40 @@ __device__ __forceinline__ void seg_carry_slots(const float *__restrict__ carry,
     while (i > 0 && rows[i] == r) --i;
     if (i == 0 && rows[0] == r) return;   // began before the window: its head block sums it
     const uint64_t k = k0 + i;
+    // The run's ranges are contiguous (row r's nonzeros are), so the ranges
+    // that carry r form a prefix from k1: count them 256 at first, then 1024
+    // at a time with each thread's 4 loads in flight together (a heavy row
+    // of a skewed tensor spans thousands of ranges).
     uint64_t e = k1 + 1;                  // ranges k1 - 1 and k1 are in the run
-    for (uint64_t base = e;; base += 256) {
-        const uint64_t jj = base + t;
-        const bool in = jj < nwaves && carry_row[jj] == r;
-        const int cnt = __syncthreads_count(in);
-        e = base + (uint64_t)cnt;
-        if (cnt < 256) break;
+    for (uint64_t base = e, nq = 1;; base += nq * 256, nq = 4) {   // nq: block-uniform
+        bool in[4] = {false, false, false, false};
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const uint64_t jj = base + (uint64_t)q * 256 + t;
+            if ((uint64_t)q < nq) in[q] = jj < nwaves && carry_row[jj] == r;
+        }
+        __builtin_amdgcn_sched_barrier(0);   // the loads stay ahead of their uses
+        uint64_t cnt = 0;
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            if ((uint64_t)q < nq) cnt += (uint64_t)__syncthreads_count(in[q]);
+        e = base + cnt;
+        if (cnt < nq * 256) break;
     }
     float4 tot = make_float4(0.f, 0.f, 0.f, 0.f);
     if (c0 < S) {
-#pragma unroll 4
-        for (uint64_t j = k + g; j < e; j += G) seg_add(tot, seg_ld(carry + j * kSegCarryStride + c0));
+        // each group's ranges in ascending order: whole batches of 16 loads
+        // in flight together, then the rest one by one
+        uint64_t j = k + g;
+        for (; j + 15 * (uint64_t)G < e; j += 16 * (uint64_t)G) {
+            float4 v[16];
+#pragma unroll
+            for (int q = 0; q < 16; ++q) v[q] = seg_ld(carry + (j + (uint64_t)q * G) * kSegCarryStride + c0);
+            __builtin_amdgcn_sched_barrier(0);   // all 16 loads issue before the first add
+#pragma unroll
+            for (int q = 0; q < 16; ++q) seg_add(tot, v[q]);
+        }
+        for (; j < e; j += G) seg_add(tot, seg_ld(carry + j * kSegCarryStride + c0));
     }
     part[t] = tot;
     __syncthreads();

@@ -327,14 +327,25 @@ __device__ __forceinline__ void seg_range(const SegArgs &a, uint64_t wave, SegSh
     }
     seg_wave_sync();   // piece / info are rewritten by the next range
 
-    // zero the untouched rows of this range's slice
+    // zero the untouched rows of this range's slice: the slice's bitmap
+    // words are loaded once, one a lane (2048 rows at a time), and each row
+    // reads its word by shuffle, so the stores do not wait on loads
     SegAcc<C> zero;
     zero.zero();
     const uint64_t r0 = wave * (uint64_t)a.rows_per_wave;
     const uint64_t r1 = min<uint64_t>(r0 + a.rows_per_wave, a.nrows);
-    for (uint64_t r = r0 + g; r < r1; r += G) {
-        const uint32_t w = a.touched[r >> 5];
-        if (!((w >> (r & 31u)) & 1u) && col_ok) zero.store(out + r * S + c0);
+    for (uint64_t cb = r0 & ~31ull; cb < r1; cb += 2048) {
+        const uint64_t w0 = cb >> 5;
+        const uint64_t nw = min<uint64_t>(64, (r1 - cb + 31) >> 5);   // words w0 .. w0+nw-1 < ceil(nrows/32)
+        const uint32_t myw = lane < nw ? a.touched[w0 + lane] : 0u;
+        const uint64_t lo = max<uint64_t>(cb, r0), hi = min<uint64_t>(cb + 2048, r1);
+        for (uint64_t rb = lo; rb < hi; rb += G) {   // wave-uniform trip count
+            const uint64_t r = rb + g;
+            const bool in = r < hi;
+            const int k = (int)(((in ? r : lo) >> 5) - w0);
+            const uint32_t w = (uint32_t)__shfl((int)myw, k);
+            if (in && !((w >> (r & 31u)) & 1u) && col_ok) zero.store(out + r * S + c0);
+        }
     }
 }
 

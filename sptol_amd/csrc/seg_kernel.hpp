@@ -349,6 +349,37 @@ __device__ __forceinline__ void seg_range(const SegArgs &a, uint64_t wave, SegSh
     }
 }
 
+// One batch of a group's ranges j, j + G, ..., j + (NB-1)G of the run of
+// row r (r != kNoRow): loads at indices clamped into [0, nwaves), so that
+// none waits on a branch, then the ranges of the run, a prefix of the batch,
+// added to tot in ascending order (bitwise tests and selects, so no load is
+// moved into a branch).  False once the run has ended.
+template <int NB>
+__device__ __forceinline__ bool seg_carry_batch(const float *__restrict__ carry, const uint32_t *__restrict__ carry_row,
+                                                uint64_t nwaves, uint32_t r, uint64_t j, uint32_t G, uint32_t c0,
+                                                float4 &tot)
+{
+#pragma clang fp contract(off)
+    float4 v[NB];
+    uint32_t rr[NB];
+#pragma unroll
+    for (int q = 0; q < NB; ++q) {
+        const uint64_t jj = min<uint64_t>(j + (uint64_t)q * G, nwaves - 1);
+        rr[q] = carry_row[jj];
+        v[q] = seg_ld(carry + jj * kSegCarryStride + c0);
+    }
+    __builtin_amdgcn_sched_barrier(0);   // all loads issue before the first add
+    uint32_t m = 0;
+#pragma unroll
+    for (int q = 0; q < NB; ++q) m |= ((uint32_t)(j + (uint64_t)q * G < nwaves) & (uint32_t)(rr[q] == r)) << q;
+    const uint32_t n1 = (uint32_t)__builtin_ctz(~m);   // <= NB
+    // ranges past the run add -0.0f, which leaves every sum's bits unchanged
+    const float4 nz = make_float4(-0.f, -0.f, -0.f, -0.f);
+#pragma unroll
+    for (int q = 0; q < NB; ++q) seg_add(tot, (uint32_t)q < n1 ? v[q] : nz);
+    return n1 == (uint32_t)NB;
+}
+
 // Carries of ranges [k0, k1) of one MTTKRP (k1 - k0 <= kSegCarrySlots), by a
 // 256-thread block.  A run is a maximal sequence of consecutive ranges
 // carrying the same row; the block whose window holds a run's first range
@@ -379,9 +410,9 @@ __device__ __forceinline__ void seg_carry_slots(const float *__restrict__ carry,
         while (j < n && rows[j + 1] == r) ++j;
         if (j == n && rows[n + 1] == r) continue;   // continues past the window
         if (c0 < S) {
+            float4 o = seg_ld(out + (size_t)r * S + c0);   // in flight with the carries
             float4 s = seg_ld(carry + (k0 + i) * kSegCarryStride + c0);
             for (uint32_t q = i + 1; q < j; ++q) seg_add(s, seg_ld(carry + (k0 + q) * kSegCarryStride + c0));
-            float4 o = seg_ld(out + (size_t)r * S + c0);
             seg_add(o, s);
             seg_st(out + (size_t)r * S + c0, o);
         }
@@ -393,47 +424,27 @@ __device__ __forceinline__ void seg_carry_slots(const float *__restrict__ carry,
     while (i > 0 && rows[i] == r) --i;
     if (i == 0 && rows[0] == r) return;   // began before the window: its head block sums it
     const uint64_t k = k0 + i;
-    // The run's ranges are contiguous (row r's nonzeros are), so the ranges
-    // that carry r form a prefix from k1: count them 256 at first, then 1024
-    // at a time with each thread's 4 loads in flight together (a heavy row
-    // of a skewed tensor spans thousands of ranges).
-    uint64_t e = k1 + 1;                  // ranges k1 - 1 and k1 are in the run
-    for (uint64_t base = e, nq = 1;; base += nq * 256, nq = 4) {   // nq: block-uniform
-        bool in[4] = {false, false, false, false};
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const uint64_t jj = base + (uint64_t)q * 256 + t;
-            if ((uint64_t)q < nq) in[q] = jj < nwaves && carry_row[jj] == r;
-        }
-        __builtin_amdgcn_sched_barrier(0);   // the loads stay ahead of their uses
-        uint64_t cnt = 0;
-#pragma unroll
-        for (int q = 0; q < 4; ++q)
-            if ((uint64_t)q < nq) cnt += (uint64_t)__syncthreads_count(in[q]);
-        e = base + cnt;
-        if (cnt < nq * 256) break;
-    }
+    // The run's ranges are contiguous (row r's nonzeros are), so from k on
+    // the ranges that carry r form a prefix (a heavy row of a skewed tensor
+    // spans thousands of ranges).  Each group takes ranges k + g, k + g + G,
+    // ... (one, then 16 at a time), each range's row and carry loaded
+    // together, and adds them in ascending order up to the first range of
+    // another row.
     float4 tot = make_float4(0.f, 0.f, 0.f, 0.f);
+    float4 o = make_float4(0.f, 0.f, 0.f, 0.f);
     if (c0 < S) {
-        // each group's ranges in ascending order: whole batches of 16 loads
-        // in flight together, then the rest one by one
+        // every group reads the row (group 0 adds to it), in flight with the
+        // first batch; one range a group covers a short run whole
+        o = seg_ld(out + (size_t)r * S + c0);
         uint64_t j = k + g;
-        for (; j + 15 * (uint64_t)G < e; j += 16 * (uint64_t)G) {
-            float4 v[16];
-#pragma unroll
-            for (int q = 0; q < 16; ++q) v[q] = seg_ld(carry + (j + (uint64_t)q * G) * kSegCarryStride + c0);
-            __builtin_amdgcn_sched_barrier(0);   // all 16 loads issue before the first add
-#pragma unroll
-            for (int q = 0; q < 16; ++q) seg_add(tot, v[q]);
-        }
-        for (; j < e; j += G) seg_add(tot, seg_ld(carry + j * kSegCarryStride + c0));
+        if (seg_carry_batch<1>(carry, carry_row, nwaves, r, j, G, c0, tot))
+            for (j += G; seg_carry_batch<16>(carry, carry_row, nwaves, r, j, G, c0, tot);) j += 16 * (uint64_t)G;
     }
     part[t] = tot;
     __syncthreads();
     if (g == 0 && c0 < S) {
         float4 s = part[l];
         for (uint32_t gg = 1; gg < G; ++gg) seg_add(s, part[gg * lpn + l]);
-        float4 o = seg_ld(out + (size_t)r * S + c0);
         seg_add(o, s);
         seg_st(out + (size_t)r * S + c0, o);
     }

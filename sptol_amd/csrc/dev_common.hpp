@@ -29,6 +29,19 @@
 
 namespace sptol {
 
+// Runs a HIP call whose failure the caller handles or ignores, and drops the
+// error that call leaves in the thread's last-error slot: the next
+// hipGetLastError (after a kernel launch) would otherwise report it as the
+// launch's.  An error already pending before the call is kept.
+template <class F>
+inline hipError_t hip_quiet(F &&f)
+{
+    const hipError_t pending = hipPeekAtLastError();
+    const hipError_t e = f();
+    if (e != hipSuccess && pending == hipSuccess) (void)hipGetLastError();
+    return e;
+}
+
 // Budget-tracked device allocation (zero-size requests return nullptr).
 // dmalloc may hand out a cached block (see StreamScope); dfree returns the
 // block to HIP, or, inside a StreamScope, to the cache.

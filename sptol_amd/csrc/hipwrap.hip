@@ -82,7 +82,7 @@ static void *cache_take(size_t rounded)
     for (auto it = range.first; it != range.second && scanned < 16; ++it, ++scanned) {
         CachedBlock &b = it->second;
         bool ok = b.ev == nullptr || (tl_scoped && b.s == tl_stream);
-        if (!ok && hipEventQuery(b.ev) == hipSuccess) ok = true;
+        if (!ok && hip_quiet([&] { return hipEventQuery(b.ev); }) == hipSuccess) ok = true;
         if (!ok) continue;
         void *p = b.p;
         if (b.ev) g_spare_events.push_back(b.ev);
@@ -274,16 +274,17 @@ void dfree(void *ptr)
         }
     }
     if (rounded) {
-        if (!ev && hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) ev = nullptr;
-        if (ev && hipEventRecord(ev, tl_stream) == hipSuccess) {
+        if (!ev && hip_quiet([&] { return hipEventCreateWithFlags(&ev, hipEventDisableTiming); }) != hipSuccess)
+            ev = nullptr;
+        if (ev && hip_quiet([&] { return hipEventRecord(ev, tl_stream); }) == hipSuccess) {
             std::lock_guard<std::mutex> lk(g_mem_mu);
             g_cache.emplace(rounded, CachedBlock{ptr, tl_stream, ev});
             g_cached += rounded;
             return;
         }
-        if (ev) (void)hipEventDestroy(ev);
+        if (ev) (void)hip_quiet([&] { return hipEventDestroy(ev); });
     }
-    (void)hipFree(ptr);
+    (void)hip_quiet([&] { return hipFree(ptr); });
 }
 
 hipStream_t stream_of(void *s) { return (hipStream_t)s; }

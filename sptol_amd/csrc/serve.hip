@@ -227,7 +227,7 @@ static int prepare_loaded(sptolServer srv, Request *r, hipStream_t s)
 
 static void admit_loop(sptolServer srv, int worker)
 {
-    (void)hipSetDevice(srv->device);
+    (void)hip_quiet([&] { return hipSetDevice(srv->device); });
     hipStream_t s = srv->admit_stream[worker];
     for (;;) {
         Request *r = nullptr;
@@ -311,7 +311,7 @@ static void complete(sptolServer srv, Request *r)
 // to host memory and release its device memory.
 static void release_loop(sptolServer srv)
 {
-    (void)hipSetDevice(srv->device);
+    (void)hip_quiet([&] { return hipSetDevice(srv->device); });
     hipStream_t s = srv->release_stream;
     for (;;) {
         Request *r = nullptr;
@@ -356,7 +356,7 @@ static void release_loop(sptolServer srv)
 // on the cohorts.
 static void compute_loop(sptolServer srv)
 {
-    (void)hipSetDevice(srv->device);
+    (void)hip_quiet([&] { return hipSetDevice(srv->device); });
     std::vector<Request *> active;
     BatchBuffers bb[kCohorts];
     AlsBatchBuffers ab[kCohorts];
@@ -387,7 +387,7 @@ static void compute_loop(sptolServer srv)
         for (size_t i = 0; i < active.size();) {
             Request *r = active[i];
             if (r->job->fit_pending) {
-                if (hipEventQuery(r->fit_event) != hipSuccess) { ++i; continue; }
+                if (hip_quiet([&] { return hipEventQuery(r->fit_event); }) != hipSuccess) { ++i; continue; }
                 r->job->fit_ready();
             }
             if (r->job->done) {

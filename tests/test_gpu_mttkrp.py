@@ -181,6 +181,44 @@ def test_long_ranges_small_factors(gpu_lib):
     gpu_lib.sptolDevCooFree(h)
 
 
+def test_carry_runs_across_batch_edges(gpu_lib):
+    """Mode-0 rows that span from 1 to about 1,100 ranges of the segmented
+    view (at R=32 and this size a range holds about 1,280 nonzeros: 8 lane
+    groups of 160).  Their lengths are spread around multiples of that, so
+    the carry pass sees runs that end inside and at the edges of its batches:
+    first one range per lane group (32 ranges), then 16 per group (512).
+    Every output row is compared with the FP64 oracle on its own.  A range
+    dropped from a run, or added to it, is a relative error of at least
+    1/1,100 in that row (8.9e-4 for the longest).  The GPU's partials hold
+    at most a few hundred sequential adds, far below that bar; the
+    sequential FP32 order is itself off by 6.5e-4 in the longest row, so the
+    bar is the FP64 oracle."""
+    rng = np.random.default_rng(91)
+    unit = 1280
+    spans = [1, 2, 31, 32, 33, 34, 63, 64, 65, 95, 96, 97, 543, 544, 545, 546, 1100]
+    lens = [max(1, s * unit + int(rng.integers(-unit // 2, unit // 2))) for s in spans]
+    lens += [int(x) for x in rng.integers(1, 300, size=3000)]
+    lens = [lens[k] for k in rng.permutation(len(lens))]
+    rows = np.repeat(np.arange(len(lens), dtype=np.uint32), lens)
+    nnz = rows.size
+    perm = rng.permutation(nnz)   # the view sorts by mode; the input order is arbitrary
+    d = np.array([len(lens), 2000, 1000], dtype=np.uint32)
+    inds = [rows[perm], rng.integers(0, 2000, nnz).astype(np.uint32), rng.integers(0, 1000, nnz).astype(np.uint32)]
+    vals = gen.open_unit(rng, nnz)
+    A = gen.uniform_factors(d, 32, seed=92)
+    h = dev_coo(gpu_lib, d, inds, vals)
+    for mode in (0, 1):
+        _say("carry runs", mode, nnz)
+        order = [mode, (mode + 1) % 3, (mode + 2) % 3]
+        got = dev_mttkrp_coo(gpu_lib, d, inds, vals, A, order, mode, prepare=True, h=h)
+        ref = oracle.mttkrp_coo(inds, vals, A, order, mode, dtype=np.float64)
+        assert np.all(got[:, 32:] == 0), "padding columns must be zero"
+        g, r = got[:, :32].astype(np.float64), ref[:, :32]
+        err = np.linalg.norm(g - r, axis=1) / np.maximum(np.linalg.norm(r, axis=1), 1e-300)
+        assert err.max() <= 1e-4, (mode, int(err.argmax()), float(err.max()))
+    gpu_lib.sptolDevCooFree(h)
+
+
 def test_bf16_factors(gpu_lib):
     d, inds, vals = gen.zipf_tensor([3000, 2000, 4000], 100000, 1.1, seed=9)
     A = gen.uniform_factors(d, 32, seed=10)

@@ -20,7 +20,11 @@
 #include <stdio.h>
 #include <stdint.h>
 #include <inttypes.h>
+#include <math.h>
 #include <stdbool.h>
+#ifdef PARTI_USE_OPENMP
+#include <omp.h>
+#endif
 
 #ifdef __cplusplus
 extern "C" {
@@ -41,10 +45,41 @@ typedef float    sptValue;
 #define PARTI_INDEX_TYPEWIDTH 32
 #define PARTI_VALUE_TYPEWIDTH 32
 #define PARTI_ELEMENT_INDEX_TYPEWIDTH 8
+typedef unsigned __int128 sptMortonIndex;
+
+/* printf/scanf conversions of the scalar types (types.h:34-92); callers'
+ * option parsing reads them, e.g. sscanf(arg, "%"PARTI_SCN_INDEX, &mode). */
 #define PARTI_INDEX_MAX UINT32_MAX
 #define PARTI_PRI_INDEX PRIu32
+#define PARTI_SCN_INDEX SCNu32
+#define PARTI_PRI_BLOCK_INDEX PRIu32
+#define PARTI_SCN_BLOCK_INDEX SCNu32
+#define PARTI_PRI_BLOCKNNZ_INDEX PARTI_PRI_BLOCK_INDEX
+#define PARTI_SCN_BLOCKNNZ_INDEX PARTI_SCN_BLOCK_INDEX
+#define PARTI_PRI_ELEMENT_INDEX PRIu8
+#define PARTI_SCN_ELEMENT_INDEX SCNu8
+#define PARTI_PRI_BLOCKMATRIX_INDEX PRIu16
+#define PARTI_SCN_BLOCKMATRIX_INDEX SCNu16
 #define PARTI_PRI_NNZ_INDEX PRIu64
+#define PARTI_SCN_NNZ_INDEX SCNu64
 #define PARTI_PRI_VALUE "f"
+#define PARTI_SCN_VALUE "f"
+
+/* The reference's system-configuration constants (macros.h:23-36).  Callers
+ * use them to pick OpenMP schedules and privatisation and print them as
+ * tuning hints; the GPU path does not depend on them. */
+#define NUM_CORES 32
+#define L1_SIZE 32000
+#define LLC_SIZE 40960000
+#define PAR_DEGREE_REDUCE 20
+#define PAR_MAX_DEGREE 100
+#define PAR_MIN_DEGREE 4
+#ifndef PARTI_DEFAULT_NLOCKS
+#define PARTI_DEFAULT_NLOCKS 1024
+#endif
+#ifndef PARTI_DEFAULT_LOCK_PAD_SIZE
+#define PARTI_DEFAULT_LOCK_PAD_SIZE 16
+#endif
 
 /* Error codes (include/includes/macros.h:43-52).  HIP runtime errors are
  * reported as SPTERR_CUDA_ERROR + hipError_t, in the slot the reference
@@ -154,6 +189,19 @@ typedef struct sptTagTimer *sptTimer;
 int  sptGetLastError(const char **module, const char **file, unsigned *line, const char **err_str); /* error.c:89 */
 void sptClearLastError(void);                                                                      /* error.c:110 */
 
+/* Assertion that stays on under NDEBUG (helper_funcs.h:29-35): prints
+ * "file:line: assertion error: "expr"" to stderr and aborts. */
+void spt_Panic(const char *file, unsigned line, const char *expr);                             /* error.c:75 */
+#define sptAssert(expr) ((expr) ? (void)0 : spt_Panic(__FILE__, __LINE__, #expr))
+#if !defined(__cplusplus) && !defined(max)
+/* helper_funcs.h:22-25 (C callers only: a function-like `max` would break C++) */
+#define max(a, b) ({ __typeof__(a) sptol_a_ = (a); __typeof__(b) sptol_b_ = (b); sptol_a_ > sptol_b_ ? sptol_a_ : sptol_b_; })
+#endif
+
+/* "%0.2f" followed by B/KB/MB/GB/TB (powers of 1024); the caller frees the
+ * string.  NULL when out of memory. */
+char *sptBytesString(uint64_t const bytes);                                                    /* base.c:25 */
+
 /* ---------------------------------------------------------------------- */
 /* Timers (src/timer.c:29-96)                                             */
 /* ---------------------------------------------------------------------- */
@@ -162,6 +210,7 @@ int    sptStartTimer(sptTimer timer);                         /* timer.c:44 */
 int    sptStopTimer(sptTimer timer);                          /* timer.c:55 */
 double sptElapsedTime(const sptTimer timer);                  /* timer.c:66 */
 double sptPrintElapsedTime(const sptTimer timer, const char *name); /* timer.c:75 */
+double sptPrintAverageElapsedTime(const sptTimer timer, const int niters, const char *name); /* timer.c:82 */
 int    sptFreeTimer(sptTimer timer);                          /* timer.c:89 */
 
 /* ---------------------------------------------------------------------- */
@@ -183,6 +232,7 @@ int  sptNewNnzIndexVector(sptNnzIndexVector *vec, sptNnzIndex len, sptNnzIndex c
 int  sptAppendNnzIndexVector(sptNnzIndexVector *vec, sptNnzIndex value);
 void sptFreeNnzIndexVector(sptNnzIndexVector *vec);
 sptIndex sptMaxIndexArray(sptIndex const *indices, sptNnzIndex len);                   /* vector/array.c:39 */
+int  sptDumpIndexArray(sptIndex *array, sptNnzIndex const n, FILE *fp);                 /* vector/dump.c:143 */
 
 /* ---------------------------------------------------------------------- */
 /* Dense matrices (src/matrix/matrix.c, rankmatrix.c, dump.c)             */
@@ -198,6 +248,7 @@ int  sptDumpMatrix(sptMatrix *mtx, FILE *fp);                                   
 int  sptNewRankMatrix(sptRankMatrix *mtx, sptIndex const nrows, sptElementIndex const ncols); /* rankmatrix.c:38 */
 int  sptConstantRankMatrix(sptRankMatrix *mtx, sptValue const val);                         /* rankmatrix.c:87 */
 void sptFreeRankMatrix(sptRankMatrix *mtx);                                              /* rankmatrix.c:127 */
+int  sptDumpRankMatrix(sptRankMatrix *mtx, FILE *fp);                                     /* matrix/dump.c:60 */
 int  sptRandomizeRankMatrix(sptRankMatrix *mtx, sptIndex const nrows, sptElementIndex const ncols); /* rankmatrix.c:70 */
 int  sptMatrix2Norm(sptMatrix * const A, sptValue * const lambda);                      /* matrix.c:390 */
 int  sptMatrixMaxNorm(sptMatrix * const A, sptValue * const lambda);                    /* matrix.c:473 */
@@ -212,6 +263,30 @@ double SparseTensorFrobeniusNormSquared(sptSparseTensor const * const spten);   
 int    sptLoadSparseTensor(sptSparseTensor *tsr, sptIndex start_index, FILE *fp);            /* load.c:31 */
 int    sptDumpSparseTensor(const sptSparseTensor *tsr, sptIndex start_index, FILE *fp);      /* sptensor/dump.c:29 */
 void   sptSparseTensorSortIndex(sptSparseTensor *tsr, int force, int tk);                    /* sort.c:559 */
+/* Lexicographic order over mode_order[0], mode_order[1], ... (most
+ * significant first); sortorder becomes mode_order.  Without force, a tensor
+ * whose sortorder already equals mode_order is left alone. */
+void   sptSparseTensorSortIndexCustomOrder(sptSparseTensor *tsr, sptIndex const *mode_order,
+           int force, int tk);                                                               /* sort.c:523 */
+/* Lexicographic order over every mode but `mode`, then `mode` last. */
+void   sptSparseTensorSortIndexAtMode(sptSparseTensor *tsr, sptIndex const mode, int force, int tk); /* sort.c:588 */
+/* mode_order = [mode, the other modes by increasing dimension] (best) or
+ * [the other modes by decreasing dimension, mode] (worst); equal dimensions
+ * keep increasing mode number. */
+void   sptGetBestModeOrder(sptIndex *mode_order, sptIndex const mode, sptIndex const *ndims,
+           sptIndex const nmodes);                                                           /* sort.c:193 */
+void   sptGetWorstModeOrder(sptIndex *mode_order, sptIndex const mode, sptIndex const *ndims,
+           sptIndex const nmodes);                                                           /* sort.c:239 */
+/* HiCOO order without building HiCOO arrays: superblocks (2^sk_bits) row
+ * major, Z-Morton inside each. */
+int    sptSparseTensorMixedOrder(sptSparseTensor *tsr, const sptElementIndex sb_bits,
+           const sptElementIndex sk_bits, int const tk);                                     /* sort.c:283 */
+/* The reference's deterministic shuffle: for z = 0..nnz-1, srand(z+1) and
+ * swap nonzero z with nonzero (rand()/RAND_MAX * nnz) % nnz. */
+void   sptGetRandomShuffleElements(sptSparseTensor *tsr);                                    /* sort.c:371 */
+/* Dimensions, nnz, density, average slice lengths and COO storage. */
+void   sptSparseTensorStatus(sptSparseTensor *tsr, FILE *fp);                                /* status.c:37 */
+double sptSparseTensorDensity(sptSparseTensor const * const tsr);                           /* status.c:24 */
 /* Ingest additions (no counterpart in the reference, which keeps duplicated
  * coordinates as separate nonzeros, load.c:31-86):
  *   sptolSparseTensorMergeDuplicates: sort lexicographically and merge each
@@ -245,6 +320,9 @@ int  sptSetKernelPointers(sptNnzIndexVector *kptr, sptSparseTensor *tsr,
          const sptElementIndex sk_bits);                                                     /* convert.c:237 */
 int  sptDumpSparseTensorHiCOO(sptSparseTensorHiCOO * const hitsr, FILE *fp);                 /* hicoo/dump.c:28 */
 double sptSparseTensorFrobeniusNormSquaredHiCOO(sptSparseTensorHiCOO const * const hitsr);   /* hicoo.c:179 */
+/* Block/kernel/chunk counts, storage, schedule sizes and block-density
+ * statistics of a HiCOO tensor (nb >= 1). */
+void sptSparseTensorStatusHiCOO(sptSparseTensorHiCOO *hitsr, FILE *fp);                       /* hicoo/status.c:28 */
 
 /* ---------------------------------------------------------------------- */
 /* MTTKRP: the hot path.  All of these run on the GPU.                    */
@@ -282,6 +360,7 @@ int    sptNewRankKruskalTensor(sptRankKruskalTensor *ktsr, sptIndex nmodes, cons
                                sptElementIndex rank);                                        /* rankktensor.c:33 */
 void   sptFreeRankKruskalTensor(sptRankKruskalTensor *ktsr);                                  /* rankktensor.c:78 */
 int    sptDumpKruskalTensor(sptKruskalTensor *ktsr, FILE *fp);                                /* ktensor/dump.c:24 */
+int    sptDumpRankKruskalTensor(sptRankKruskalTensor *ktsr, FILE *fp);                        /* ktensor/dump.c:57 */
 /* fit = 1 - ||X - K|| / ||X||, with ||K||^2 from the full symmetric Gram
  * Hadamard product (the reference reads an unwritten triangle, ktensor.c:151). */
 double sptKruskalTensorFit(sptSparseTensor const * const spten, sptValue const * const lambda,
@@ -294,6 +373,13 @@ int sptCpdAls(sptSparseTensor const * const spten, sptIndex const rank, sptIndex
         double const tol, sptKruskalTensor * ktensor);                                      /* cpd.c:148 */
 int sptOmpCpdAls(sptSparseTensor const * const spten, sptIndex const rank, sptIndex const niters,
         double const tol, const int tk, const int use_reduce, sptKruskalTensor * ktensor);  /* cpd_omp.c:166 */
+/* rhs[i] := rhs[i] V^{-1} for every row of rhs, V = the Hadamard product of
+ * the Grams aTa[(mode + k) % nmodes], k = 1..nmodes-1 (their upper
+ * triangles), by an FP64 Cholesky factorisation on the GPU with an LU (partial
+ * pivoting) fallback when V is not positive definite; rows are left unsolved
+ * when V is exactly singular, as gesv leaves them.  aTa[nmodes] receives V.
+ * rank <= 64; rhs has rank columns and the Grams' stride. */
+int sptMatrixSolveNormals(sptIndex const mode, sptIndex const nmodes, sptMatrix ** aTa, sptMatrix * rhs); /* matrix_solver.c:32 */
 /* The reference's exported inner loop (cpd.c:32-137), with caller-supplied
  * factors mats[0..nmodes] (mats[nmodes] is scratch with >= max_dim rows).
  * Returns the last fit. */

@@ -178,13 +178,17 @@ int  sptolHipStreamDestroy(void *stream);
 typedef struct {
     int      factor_dtype;   /* in: SPTOL_F32 or SPTOL_BF16 storage for MTTKRP gathers       */
     int      exact_fit;      /* in: 1 = the returned fit takes <X,K> from an FP64 pass over the
-                                nonzeros, once, after the last iteration (3- and 4-mode
-                                tensors); the per-iteration fit that drives tol always comes
+                                nonzeros (COO or HiCOO, any order), once, after the last
+                                iteration; the per-iteration fit that drives tol always comes
                                 from the last mode's FP32 MTTKRP, as the reference
                                 (ktensor.c:173-257), which loses digits to cancellation when
-                                the fit nears 1                                             */
+                                the fit nears 1.  sptCpdAls, CpdAlsStep and sptCpdAlsHiCOO
+                                always return the exact fit                                 */
     int      iterations;     /* out: iterations executed                                    */
-    int      reserved;
+    int      singular_solves;/* out: mode updates whose normal equations were exactly singular
+                                (a zero pivot after partial pivoting): their factors keep the
+                                MTTKRP rows unsolved, as LAPACK gesv leaves its right-hand
+                                sides when info > 0 (matrix_solver.c:72-77)                 */
     double   fit;            /* out: last fit (FP64, correct ||K||^2)                       */
     double   seconds;        /* out: wall time of the ALS loop                              */
 } sptolCpdStats;
@@ -196,8 +200,7 @@ typedef struct {
 int  sptolCpdAlsDevice(sptolDevCoo X, sptIndex rank, sptIndex niters, double tol,
                        sptolDevMatrix *mats, sptValue *lambda, sptolCpdStats *stats, void *stream);
 /* The same loop over a device HiCOO tensor (src/sptensor/hicoo/cpd.c:34-135):
- * every mode's view is prepared first, the MTTKRP is the HiCOO one.  The
- * exact fit is not available here (stats->exact_fit is ignored). */
+ * every mode's view is prepared first, the MTTKRP is the HiCOO one. */
 int  sptolCpdAlsHicooDevice(sptolDevHicoo X, sptIndex rank, sptIndex niters, double tol,
                             sptolDevMatrix *mats, sptValue *lambda, sptolCpdStats *stats, void *stream);
 

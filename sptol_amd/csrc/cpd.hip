@@ -217,7 +217,9 @@ __global__ void k_scale(float *__restrict__ A, uint16_t *__restrict__ Abf, uint3
 // V = Hadamard of ata[p] for p = (m+k) % N, k = 1..N-1; then a Cholesky
 // factorisation V = L L^T.  If V is not positive definite, an LU
 // factorisation with partial pivoting of V replaces it (the gesv fallback
-// of matrix_solver.c:66-79).  Output: fac (R*R), piv (R), *kind = 0 (L) / 1 (LU).
+// of matrix_solver.c:66-79).  Output: fac (R*R), piv (R), kind[0] = 0 (L) /
+// 1 (LU) / 2 (exactly singular: no factor, the solve leaves its right-hand
+// sides), and kind[1] incremented for each singular case.
 //
 // One wave (64 threads, R <= RM <= 64).  The Cholesky is right-looking with
 // lane i holding row i in registers: at step j lane j publishes its updated
@@ -287,7 +289,7 @@ __device__ __forceinline__ void factor_body(const double *__restrict__ ata, uint
             for (int k = 0; k < RM; ++k)
                 if (k < (int)R) fac[t * R + k] = k <= (int)t ? row[k] : 0.0;
         }
-        if (t == 0) *kind = 0;
+        if (t == 0) kind[0] = 0;
         return;
     }
     // LU with partial pivoting of V (recomputed; in place): at step j swap row
@@ -304,6 +306,13 @@ __device__ __forceinline__ void factor_body(const double *__restrict__ ata, uint
         uint32_t best = j;
         for (uint32_t i = j + 1; i < R; ++i)
             if (fabs(V[i * LD + j]) > fabs(V[best * LD + j])) best = i;
+        if (V[best * LD + j] == 0.0) {
+            // exactly singular (every lane read the same LDS value): like
+            // LAPACK's gesv with info > 0 the right-hand sides stay as they
+            // are (kind 2); kind[1] counts such solves for the host
+            if (t == 0) { kind[0] = 2; kind[1] += 1; }
+            return;
+        }
         if (t == 0) piv[j] = (int)best;
         if (best != j)
             for (uint32_t k = t; k < R; k += 64) {
@@ -320,7 +329,7 @@ __device__ __forceinline__ void factor_body(const double *__restrict__ ata, uint
         chol_wave_sync();
     }
     for (uint32_t e = t; e < RR; e += 64) fac[e] = V[(e / R) * LD + e % R];
-    if (t == 0) *kind = 1;
+    if (t == 0) kind[0] = 1;
 }
 
 template <int RM>
@@ -347,7 +356,9 @@ __device__ __forceinline__ void solve_rows_body(const float *__restrict__ M, flo
     double x[RM];
 #pragma unroll
     for (int a = 0; a < RM; ++a) x[a] = a < (int)R ? (double)M[i * S + a] : 0.0;
-    if (!lu) {
+    if (lu == 2) {
+        // singular normal equations: the row stays the MTTKRP's
+    } else if (!lu) {
         // L y = x, then L^T z = y
 #pragma unroll
         for (int a = 0; a < RM; ++a) {
@@ -518,6 +529,106 @@ __global__ __launch_bounds__(256) void k_inner_exact_partial(const InnerArgs a, 
     if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
 }
 
+// The same FP64 <X,K> for any order N <= kMaxModes, over a COO tensor (inds)
+// or a HiCOO tensor (coordinate = (binds[m][b] << sb_bits) + einds[m][z] for
+// the block b holding nonzero z).  Each lane group takes a contiguous run of
+// its block's nonzeros (HiCOO: the run's first block by binary search over
+// bptr, then walking forward), kInnerGenU nonzeros per step with all their
+// gathers issued before use.
+struct InnerGenArgs {
+    const uint32_t *inds[kMaxModes];    // COO, or unused
+    const uint64_t *bptr;               // HiCOO: nb + 1 offsets, or null
+    const uint32_t *binds[kMaxModes];
+    const uint8_t *einds[kMaxModes];
+    uint64_t nb;
+    uint32_t sb_bits;
+    const float *vals;
+    const float *mats[kMaxModes];
+    uint64_t nnz;
+    uint32_t N, R, S;
+};
+constexpr int kInnerGenU = 2;
+
+template <int LPN, bool HICOO>
+__global__ __launch_bounds__(256) void k_inner_exact_gen(const InnerGenArgs a, const double *__restrict__ lambda,
+                                                         double *__restrict__ partial)
+{
+#pragma clang fp contract(off)
+    constexpr uint32_t G = 256 / LPN;
+    constexpr int U = kInnerGenU;
+    __shared__ double red[256];
+    const uint32_t g = threadIdx.x / LPN, l = threadIdx.x % LPN, c0 = 4 * l;
+    double lam[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) lam[q] = c0 + q < a.R ? lambda[c0 + q] : 0.0;
+    const uint32_t cc = c0 < a.S ? c0 : 0;
+    const uint64_t per = (a.nnz + gridDim.x - 1) / gridDim.x;
+    const uint64_t z0 = min<uint64_t>((uint64_t)blockIdx.x * per, a.nnz), z1 = min<uint64_t>(z0 + per, a.nnz);
+    const uint64_t run = (z1 - z0 + G - 1) / G;
+    const uint64_t y0 = min<uint64_t>(z0 + (uint64_t)g * run, z1), y1 = min<uint64_t>(y0 + run, z1);
+    uint64_t b = 0;
+    if (HICOO && y0 < y1) {   // last block whose first nonzero is <= y0
+        uint64_t lo = 0, hi = a.nb;   // bptr[lo] <= y0 < bptr[hi]
+        while (hi - lo > 1) {
+            const uint64_t mid = lo + (hi - lo) / 2;
+            if (a.bptr[mid] <= y0) lo = mid; else hi = mid;
+        }
+        b = lo;
+    }
+    double s = 0.0;
+    for (uint64_t zb = y0; zb < y1; zb += U) {
+        float4 f[U][kMaxModes];
+        float v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const bool in = zb + u < y1;
+            const uint64_t z = in ? zb + u : zb;   // zb < y1: in bounds
+            v[u] = in ? a.vals[z] : 0.0f;
+            if (HICOO) {
+                while (a.bptr[b + 1] <= z) ++b;   // z < nnz = bptr[nb]: stops at b < nb
+            }
+#pragma unroll
+            for (int m = 0; m < kMaxModes; ++m) {
+                if (m < (int)a.N) {
+                    const uint32_t i = HICOO ? (a.binds[m][b] << a.sb_bits) + (uint32_t)a.einds[m][z] : a.inds[m][z];
+                    f[u][m] = *reinterpret_cast<const float4 *>(a.mats[m] + (size_t)i * a.S + cc);
+                }
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            double p[4] = {lam[0], lam[1], lam[2], lam[3]};
+#pragma unroll
+            for (int m = 0; m < kMaxModes; ++m) {
+                if (m < (int)a.N) {
+                    p[0] *= (double)f[u][m].x; p[1] *= (double)f[u][m].y;
+                    p[2] *= (double)f[u][m].z; p[3] *= (double)f[u][m].w;
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q) p[q] = c0 + q < a.R ? p[q] : 0.0;
+            s += (double)v[u] * (((p[0] + p[1]) + p[2]) + p[3]);
+        }
+    }
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (uint32_t w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+template <bool HICOO>
+static void launch_inner_exact_gen(const InnerGenArgs &a, const double *lambda, double *partial, hipStream_t s)
+{
+    const dim3 g(kInnerBlocks), b(256);
+    if (a.S <= 8) hipLaunchKernelGGL((k_inner_exact_gen<2, HICOO>), g, b, 0, s, a, lambda, partial);
+    else if (a.S <= 16) hipLaunchKernelGGL((k_inner_exact_gen<4, HICOO>), g, b, 0, s, a, lambda, partial);
+    else if (a.S <= 32) hipLaunchKernelGGL((k_inner_exact_gen<8, HICOO>), g, b, 0, s, a, lambda, partial);
+    else hipLaunchKernelGGL((k_inner_exact_gen<16, HICOO>), g, b, 0, s, a, lambda, partial);
+}
+
 // Sum of squares of the values (||X||^2), two-stage.
 __global__ __launch_bounds__(256) void k_sqnorm_partial(const float *__restrict__ v, uint64_t n, double *__restrict__ partial)
 {
@@ -596,6 +707,11 @@ __global__ void k_set(double *p, uint32_t n, double v)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
+}
+
+__global__ void k_zero_i32(int *p, uint32_t n)
+{
+    for (uint32_t e = threadIdx.x; e < n; e += blockDim.x) p[e] = 0;
 }
 
 // Blocks of the two-stage row reductions: one per `per` rows, at most
@@ -928,7 +1044,7 @@ int CpdJob::init(uint32_t R_, uint32_t niters_, double tol_, sptolDevMatrix *mat
     SPTOL_TRY(dmalloc((void **)&w.partial, std::max<size_t>((size_t)kRedBlocks * R * R, kInnerBlocks + 1) * 8, "CPD"));
     SPTOL_TRY(dmalloc((void **)&w.fac, (size_t)R * R * 8, "CPD"));
     SPTOL_TRY(dmalloc((void **)&w.piv, (size_t)R * 4, "CPD"));
-    SPTOL_TRY(dmalloc((void **)&w.kind, 4, "CPD"));
+    SPTOL_TRY(dmalloc((void **)&w.kind, 8, "CPD"));   // [0] factor kind, [1] singular solves so far
     SPTOL_TRY(dmalloc((void **)&w.lambda, (size_t)R * 8, "CPD"));
     SPTOL_TRY(dmalloc((void **)&w.norms, (size_t)R * 8, "CPD"));
     SPTOL_TRY(dmalloc((void **)&w.normx, 8, "CPD"));
@@ -941,13 +1057,15 @@ int CpdJob::init(uint32_t R_, uint32_t niters_, double tol_, sptolDevMatrix *mat
         if (fdt == SPTOL_BF16) { mt[m].dtype = SPTOL_BF16; mt[m].values = w.bf[m]; }
     }
     Mout.nrows = maxI; Mout.ncols = R; Mout.stride = S; Mout.dtype = SPTOL_F32; Mout.values = w.M;
-    static_assert((kMaxRank + 1) * sizeof(double) <= kPinnedSlotBytes, "fit and weights fit one pinned slot");
+    static_assert((kMaxRank + 2) * sizeof(double) <= kPinnedSlotBytes,
+                  "fit, weights and the singular-solve count fit one pinned slot");
     SPTOL_TRY(pinned_slot((void **)&h_fit));
     // ||X||^2 and the initial Grams (cpd.c:65-70)
     hipLaunchKernelGGL(k_sqnorm_partial, dim3(kRedBlocks), dim3(256), 0, s, vals, nnz, w.partial);
     hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, s, w.partial, (uint32_t)kRedBlocks, 1u, w.normx);
     // (k_sqnorm_partial runs kRedBlocks blocks over the nonzeros)
     hipLaunchKernelGGL(k_set, dim3(1), dim3(kMaxRank), 0, s, w.lambda, R, 1.0);
+    hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, s, w.kind, 2u);
     SPTOL_HIP(hipGetLastError(), "CPD init");
     for (uint32_t m = 0; m < N; ++m) {
         SPTOL_TRY(gram((const float *)mats[m].values, ndims[m], R, S, w.ata + (size_t)m * R * R, w.partial, s));
@@ -1043,18 +1161,39 @@ void CpdJob::fit_ready()
 // pinned slot.
 int CpdJob::finish_enqueue(hipStream_t s)
 {
-    if (exact_fit && X && N <= 4) {
-        InnerArgs ia;
-        std::memset(&ia, 0, sizeof(ia));
-        for (uint32_t q = 0; q < N; ++q) { ia.inds[q] = X->inds[q]; ia.mats[q] = (const float *)mats[q].values; }
-        ia.vals = X->vals;
-        ia.nnz = X->nnz;
-        ia.N = N; ia.R = R; ia.S = S;
-        const dim3 g(kInnerBlocks);
-        if (S <= 8) hipLaunchKernelGGL(k_inner_exact_partial<2>, g, dim3(256), 0, s, ia, w.lambda, w.partial);
-        else if (S <= 16) hipLaunchKernelGGL(k_inner_exact_partial<4>, g, dim3(256), 0, s, ia, w.lambda, w.partial);
-        else if (S <= 32) hipLaunchKernelGGL(k_inner_exact_partial<8>, g, dim3(256), 0, s, ia, w.lambda, w.partial);
-        else hipLaunchKernelGGL(k_inner_exact_partial<16>, g, dim3(256), 0, s, ia, w.lambda, w.partial);
+    if (exact_fit) {
+        if (X && N <= 4) {
+            InnerArgs ia;
+            std::memset(&ia, 0, sizeof(ia));
+            for (uint32_t q = 0; q < N; ++q) { ia.inds[q] = X->inds[q]; ia.mats[q] = (const float *)mats[q].values; }
+            ia.vals = X->vals;
+            ia.nnz = X->nnz;
+            ia.N = N; ia.R = R; ia.S = S;
+            const dim3 g(kInnerBlocks);
+            if (S <= 8) hipLaunchKernelGGL(k_inner_exact_partial<2>, g, dim3(256), 0, s, ia, w.lambda, w.partial);
+            else if (S <= 16) hipLaunchKernelGGL(k_inner_exact_partial<4>, g, dim3(256), 0, s, ia, w.lambda, w.partial);
+            else if (S <= 32) hipLaunchKernelGGL(k_inner_exact_partial<8>, g, dim3(256), 0, s, ia, w.lambda, w.partial);
+            else hipLaunchKernelGGL(k_inner_exact_partial<16>, g, dim3(256), 0, s, ia, w.lambda, w.partial);
+        } else {
+            InnerGenArgs ga;
+            std::memset(&ga, 0, sizeof(ga));
+            for (uint32_t q = 0; q < N; ++q) ga.mats[q] = (const float *)mats[q].values;
+            ga.N = N; ga.R = R; ga.S = S;
+            if (X) {
+                for (uint32_t q = 0; q < N; ++q) ga.inds[q] = X->inds[q];
+                ga.vals = X->vals;
+                ga.nnz = X->nnz;
+                launch_inner_exact_gen<false>(ga, w.lambda, w.partial, s);
+            } else {
+                for (uint32_t q = 0; q < N; ++q) { ga.binds[q] = H->binds[q]; ga.einds[q] = H->einds[q]; }
+                ga.bptr = H->bptr;
+                ga.nb = H->nb;
+                ga.sb_bits = H->sb_bits;
+                ga.vals = H->vals;
+                ga.nnz = H->nnz;
+                launch_inner_exact_gen<true>(ga, w.lambda, w.partial, s);
+            }
+        }
         // block partials -> one scalar (kept after the partials)
         hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, s, w.partial, kInnerBlocks, 1u,
                            w.partial + kInnerBlocks);
@@ -1066,13 +1205,15 @@ int CpdJob::finish_enqueue(hipStream_t s)
     for (uint32_t q = 0; q < N; ++q)
         SPTOL_TRY(colnorm((float *)mats[q].values, nullptr, ndims[q], R, S, 0, 1, w, s));
     SPTOL_HIP(hipMemcpyAsync(h_fit + 1, w.lambda, (size_t)R * 8, hipMemcpyDeviceToHost, s), "CPD");
+    SPTOL_HIP(hipMemcpyAsync(h_fit + 1 + kMaxRank, w.kind + 1, sizeof(int), hipMemcpyDeviceToHost, s), "CPD");
     return 0;
 }
 
 void CpdJob::finish_read(sptValue *lambda_out)
 {
-    if (exact_fit && X && N <= 4) fit = h_fit[0];
+    if (exact_fit) fit = h_fit[0];
     for (uint32_t r = 0; r < R; ++r) lambda_out[r] = (sptValue)h_fit[1 + r];
+    std::memcpy(&singular_solves, h_fit + 1 + kMaxRank, sizeof(int));
 }
 
 int CpdJob::finish(hipStream_t s, sptValue *lambda_out)
@@ -1118,6 +1259,7 @@ static int run_job(CpdJob &job, sptValue *lambda_out, sptolCpdStats *stats, hipS
     if (!wait_fits) job.fit = job.h_fit[0];   // the last iteration's fit, or the exact one
     if (stats) {
         stats->iterations = job.iters;
+        stats->singular_solves = job.singular_solves;
         stats->fit = job.fit;
         stats->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     }
@@ -1152,8 +1294,76 @@ int sptolCpdAlsHicooDevice(sptolDevHicoo X, sptIndex rank, sptIndex niters, doub
     StreamScope scope(s);   // the workspace below is freed after the final synchronisation
     auto t0 = std::chrono::steady_clock::now();
     CpdJob job;
+    job.exact_fit = stats ? stats->exact_fit != 0 : false;
     SPTOL_TRY(job.setup_hicoo(X, rank, niters, tol, mats, stats ? stats->factor_dtype : SPTOL_F32, s));
     return run_job(job, lambda_out, stats, s, t0);
+}
+
+// sptMatrixSolveNormals (matrix_solver.c:32-83): rhs[i] := rhs[i] V^{-1}
+// for every row, V = Hadamard of aTa[(mode + k) % nmodes], k = 1..nmodes-1,
+// read from each Gram's upper triangle (sptMatrixDotMulSeqTriangle,
+// matrix.c:341-386).  FP64 Cholesky with the LU-with-pivoting fallback on
+// the device; rows stay as they are when V is exactly singular, as gesv
+// leaves its right-hand sides.  aTa[nmodes] receives V (the reference leaves
+// its factor there).  The reference's notices are printed to stdout.
+int sptMatrixSolveNormals(sptIndex const mode, sptIndex const nmodes, sptMatrix **aTa, sptMatrix *rhs)
+{
+    if (nmodes < 1 || mode >= nmodes) SPTOL_CHECK(SPTERR_VALUE_ERROR, "SolveNormals", "mode >= nmodes");
+    const uint32_t R = aTa[0]->ncols, S = aTa[0]->stride;
+    if (R < 1 || R > (uint32_t)kMaxRank) SPTOL_CHECK(SPTERR_VALUE_ERROR, "SolveNormals", "rank must be in [1, 64]");
+    for (uint32_t m = 0; m <= nmodes; ++m)
+        if (aTa[m]->nrows < R || aTa[m]->ncols != R || aTa[m]->stride != S)
+            SPTOL_CHECK(SPTERR_SHAPE_MISMATCH, "SolveNormals", "aTa[m] must be rank x rank with one stride");
+    if (rhs->ncols != R || rhs->stride != S)
+        SPTOL_CHECK(SPTERR_SHAPE_MISMATCH, "SolveNormals", "rhs must have rank columns and the Grams' stride");
+    const uint32_t N = nmodes, I = rhs->nrows;
+    std::vector<double> g((size_t)N * R * R), V((size_t)R * R, 1.0);
+    for (uint32_t m = 0; m < N; ++m)
+        for (uint32_t i = 0; i < R; ++i)
+            for (uint32_t j = 0; j < R; ++j) {
+                const uint32_t a = i < j ? i : j, b = i < j ? j : i;   // upper triangle
+                g[((size_t)m * R + i) * R + j] = (double)aTa[m]->values[(size_t)a * S + b];
+            }
+    for (uint32_t q = 1; q < N; ++q)
+        for (size_t e = 0; e < (size_t)R * R; ++e) V[e] *= g[(size_t)((mode + q) % N) * R * R + e];
+    for (uint32_t i = 0; i < R; ++i)
+        for (uint32_t j = 0; j < R; ++j) aTa[N]->values[(size_t)i * S + j] = (sptValue)V[(size_t)i * R + j];
+    hipStream_t s = stream_of(nullptr);
+    StreamScope scope(s);
+    CpdWork w;
+    float *M = nullptr, *A = nullptr;
+    const size_t rb = (size_t)(I ? I : 1) * S * 4;
+    SPTOL_TRY(dmalloc((void **)&w.ata, g.size() * 8, "SolveNormals"));
+    SPTOL_TRY(dmalloc((void **)&w.fac, (size_t)R * R * 8, "SolveNormals"));
+    SPTOL_TRY(dmalloc((void **)&w.piv, (size_t)R * 4, "SolveNormals"));
+    SPTOL_TRY(dmalloc((void **)&w.kind, 8, "SolveNormals"));
+    int r = dmalloc((void **)&M, rb, "SolveNormals");
+    if (!r) r = dmalloc((void **)&A, rb, "SolveNormals");
+    int kind[2] = {0, 0};
+    auto run = [&]() -> int {
+        SPTOL_HIP(hipMemcpyAsync(w.ata, g.data(), g.size() * 8, hipMemcpyHostToDevice, s), "SolveNormals");
+        SPTOL_HIP(hipMemcpyAsync(w.kind, kind, sizeof(kind), hipMemcpyHostToDevice, s), "SolveNormals");
+        if (I) SPTOL_HIP(hipMemcpyAsync(M, rhs->values, (size_t)I * S * 4, hipMemcpyHostToDevice, s), "SolveNormals");
+        SPTOL_TRY(solve(M, A, I, R, S, w, N, mode, s));
+        if (I) SPTOL_HIP(hipMemcpyAsync(rhs->values, A, (size_t)I * S * 4, hipMemcpyDeviceToHost, s), "SolveNormals");
+        SPTOL_HIP(hipMemcpyAsync(kind, w.kind, sizeof(kind), hipMemcpyDeviceToHost, s), "SolveNormals");
+        SPTOL_HIP(hipStreamSynchronize(s), "SolveNormals");
+        return 0;
+    };
+    if (!r) r = run();
+    dfree(M);
+    dfree(A);
+    if (r) return r;
+    if (kind[0] != 0) printf("Gram matrix is not SPD. Trying `gesv`.\n");
+    if (kind[0] == 2) printf("gesv: the normal equations are singular; the right-hand sides are left unsolved.\n");
+    return 0;
+}
+
+static void report_singular(int n)
+{
+    if (n > 0)
+        printf("CPD-ALS: the normal equations were singular in %d mode update%s; those factors keep the "
+               "unsolved MTTKRP rows, as gesv leaves its right-hand sides.\n", n, n == 1 ? "" : "s");
 }
 
 // The reference's exported ALS body (cpd.c:32-137) with caller-supplied
@@ -1179,7 +1389,9 @@ double CpdAlsStep(sptSparseTensor const * const spten, sptIndex const rank, sptI
     sptolCpdStats st;
     std::memset(&st, 0, sizeof(st));
     st.factor_dtype = SPTOL_F32;
+    st.exact_fit = 1;   // the returned fit: <X,K> by an FP64 pass over the nonzeros
     if (!r) r = sptolCpdAlsDevice(X, rank, niters, tol, dm, lambda, &st, nullptr);
+    if (!r) report_singular(st.singular_solves);
     for (uint32_t m = 0; m < N && !r; ++m) r = sptolDevMatrixDownload(mats[m], &dm[m]);
     // mats[N]: the last mode's MTTKRP with the final factors, as the
     // reference leaves its scratch matrix
@@ -1298,8 +1510,10 @@ int sptCpdAlsHiCOO(sptSparseTensorHiCOO const * const hitsr, sptIndex const rank
     sptolCpdStats st;
     std::memset(&st, 0, sizeof(st));
     st.factor_dtype = SPTOL_F32;
+    st.exact_fit = 1;   // the returned fit: <X,K> by an FP64 pass over the nonzeros
     auto t0 = std::chrono::steady_clock::now();
     if (!r) r = sptolCpdAlsHicooDevice(H, rank, niters, tol, dm, ktensor->lambda, &st, nullptr);
+    if (!r) report_singular(st.singular_solves);
     for (uint32_t m = 0; m < N && !r; ++m) {
         sptMatrix v = view(mats[m]);
         r = sptolDevMatrixDownload(&v, &dm[m]);

@@ -17,7 +17,7 @@ struct CpdWork {
     double *partial = nullptr;     // kRedBlocks x R x R
     double *fac = nullptr;         // R x R Cholesky or LU factor
     int *piv = nullptr;            // R
-    int *kind = nullptr;           // 0 Cholesky, 1 LU
+    int *kind = nullptr;           // [0]: 0 Cholesky, 1 LU, 2 singular; [1]: singular solves so far
     double *lambda = nullptr;      // R
     double *norms = nullptr;       // R
     double *normx = nullptr;       // ||X||^2
@@ -52,9 +52,11 @@ struct CpdJob {
     uint32_t it = 0, m = 0;               // current iteration and mode
     double fit = 0.0, oldfit = 0.0;
     int iters = 0;
+    int singular_solves = 0;              // mode updates whose normal equations were exactly singular
     bool done = false, fit_pending = false;
     bool exact_fit = false;               // final fit with <X,K> by an FP64 pass over the nonzeros
-    double *h_fit = nullptr;              // pinned slot: [0] fit, [1 .. R] final lambda
+    double *h_fit = nullptr;              // pinned slot: [0] fit, [1 .. R] final lambda,
+                                          // [1 + kMaxRank] singular-solve count (an int)
     std::chrono::steady_clock::time_point t_iter;
 
     ~CpdJob();

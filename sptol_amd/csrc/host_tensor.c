@@ -269,6 +269,91 @@ void sptSparseTensorSortIndex(sptSparseTensor *tsr, int force, int tk)
     sort_by_chunks(tsr, 0, tsr->nnz, ch, nc, tk);
 }
 
+/* Lexicographic sort over the modes in `order` (most significant first):
+ * the chunks are built over a view of the tensor whose i-th mode is mode
+ * order[i], and the view's arrays are the tensor's, permuted in place. */
+static int sort_lex_order(sptSparseTensor *tsr, const sptIndex *order, int tk)
+{
+    const sptIndex N = tsr->nmodes;
+    sptSparseTensor v = *tsr;
+    v.ndims = (sptIndex *)malloc((size_t)N * sizeof(sptIndex));
+    v.inds = (sptIndexVector *)malloc((size_t)N * sizeof(sptIndexVector));
+    key_chunk *ch = (key_chunk *)malloc(((size_t)N + 1) * sizeof(key_chunk));
+    if (!v.ndims || !v.inds || !ch) {
+        free(v.ndims); free(v.inds); free(ch);
+        SPTOL_CHECK_OS(1, "SpTns Sort");
+    }
+    for (sptIndex m = 0; m < N; ++m) {
+        v.ndims[m] = tsr->ndims[order[m]];
+        v.inds[m] = tsr->inds[order[m]];
+    }
+    int nc = lex_chunks(&v, 0, KEY_LEX, ch);
+    int r = sort_by_chunks(&v, 0, v.nnz, ch, nc, tk);
+    free(v.ndims); free(v.inds); free(ch);
+    return r;
+}
+
+void sptSparseTensorSortIndexCustomOrder(sptSparseTensor *tsr, sptIndex const *mode_order, int force, int tk)
+{
+    const sptIndex N = tsr->nmodes;
+    if (!force && memcmp(tsr->sortorder, mode_order, (size_t)N * sizeof(sptIndex)) == 0) return;
+    if (sort_lex_order(tsr, mode_order, tk)) return;
+    memcpy(tsr->sortorder, mode_order, (size_t)N * sizeof(sptIndex));
+}
+
+void sptSparseTensorSortIndexAtMode(sptSparseTensor *tsr, sptIndex const mode, int force, int tk)
+{
+    const sptIndex N = tsr->nmodes;
+    int needsort = 0;
+    /* sortorder: the other modes in increasing order, then `mode` */
+    for (sptIndex m = 0, k = 0; m < N; ++m) {
+        if (m == mode) continue;
+        if (tsr->sortorder[k] != m) { tsr->sortorder[k] = m; needsort = 1; }
+        ++k;
+    }
+    if (tsr->sortorder[N - 1] != mode) { tsr->sortorder[N - 1] = mode; needsort = 1; }
+    if (!(needsort || force)) return;
+    sort_lex_order(tsr, tsr->sortorder, tk);
+}
+
+int sptSparseTensorMixedOrder(sptSparseTensor *tsr, const sptElementIndex sb_bits, const sptElementIndex sk_bits,
+                              int const tk)
+{
+    /* superblocks row major, then Z-Morton inside each superblock: the
+     * order sptSparseTensorToHiCOO leaves the tensor in */
+    sptSparseTensorSortIndexRowBlock(tsr, 1, 0, tsr->nnz, sk_bits, tk);
+    sptNnzIndexVector kptr;
+    SPTOL_CHECK(sptNewNnzIndexVector(&kptr, 0, 0), "HiSpTns New", NULL);
+    int r = sptSetKernelPointers(&kptr, tsr, sk_bits);
+    if (r) { sptFreeNnzIndexVector(&kptr); return r; }
+    for (sptNnzIndex k = 0; k + 1 < kptr.len; ++k)
+        sptSparseTensorSortIndexMorton(tsr, 1, kptr.data[k], kptr.data[k + 1], sb_bits, tk);
+    sptFreeNnzIndexVector(&kptr);
+    return 0;
+}
+
+void sptGetRandomShuffleElements(sptSparseTensor *tsr)
+{
+    /* The reference's order: a pass of swaps whose targets come from C's
+     * rand() reseeded with z + 1 at every step (sort.c:371-381), so the same
+     * C library yields the same permutation. */
+    const sptNnzIndex nnz = tsr->nnz;
+    const sptIndex N = tsr->nmodes;
+    for (sptNnzIndex z = 0; z < nnz; ++z) {
+        srand((unsigned)(z + 1));
+        const sptValue u = (sptValue)rand() / (sptValue)RAND_MAX;
+        const sptNnzIndex t = (sptNnzIndex)(u * nnz) % nnz;
+        for (sptIndex m = 0; m < N; ++m) {
+            sptIndex *d = tsr->inds[m].data, x = d[z];
+            d[z] = d[t];
+            d[t] = x;
+        }
+        sptValue *v = tsr->values.data, x = v[z];
+        v[z] = v[t];
+        v[t] = x;
+    }
+}
+
 void sptSparseTensorSortIndexRowBlock(sptSparseTensor *tsr, int force, const sptNnzIndex begin,
                                       const sptNnzIndex end, const sptElementIndex sk_bits, int const tk)
 {
@@ -517,7 +602,8 @@ int sptol_read_binary_header(FILE *fp, uint32_t *nmodes, uint64_t *nnz, uint32_t
     if (here < 0 || fseek(fp, 0, SEEK_END) != 0) SPTOL_CHECK_OS(1, "SpTns Load Binary");
     long end = ftell(fp);
     if (end < 0 || fseek(fp, here, SEEK_SET) != 0) SPTOL_CHECK_OS(1, "SpTns Load Binary");
-    if ((uint64_t)(end - here) < *nnz * 4ull * (nm + 1ull))
+    /* by division: a crafted nnz must not wrap the product of sizes */
+    if (end < here || *nnz > (uint64_t)(end - here) / (4ull * (nm + 1ull)))
         SPTOL_CHECK(SPTERR_VALUE_ERROR, "SpTns Load Binary", "file shorter than its arrays");
     return 0;
 }

@@ -105,6 +105,8 @@ static size_t estimate_bytes(const Request &r)
 {
     const uint32_t N = r.nmodes;
     const uint32_t S = ((r.R - 1) / 8 + 1) * 8;
+    // a count whose byte total could wrap is more than any budget
+    if (r.nnz > (SIZE_MAX >> 12)) return SIZE_MAX;
     size_t b = (size_t)r.nnz * (4 * N + 4);                    // tensor
     if (N == 3 || N == 4)
         b += (size_t)r.nnz * (N == 4 ? 20 : 16) * N;           // one packed mode-sorted view per mode

@@ -86,7 +86,7 @@ class sptolDevMatrix(C.Structure):
 
 class sptolCpdStats(C.Structure):
     _fields_ = [("factor_dtype", C.c_int), ("exact_fit", C.c_int), ("iterations", C.c_int),
-                ("reserved", C.c_int), ("fit", C.c_double), ("seconds", C.c_double)]
+                ("singular_solves", C.c_int), ("fit", C.c_double), ("seconds", C.c_double)]
 
 
 class sptolServerConfig(C.Structure):
@@ -326,6 +326,17 @@ def _declare(lib):
         "sptolServerDrain": (C.c_int, [C.c_void_p]),
         "sptolServerTicks": (C.c_uint64, [C.c_void_p]),
         "sptolServerDestroy": (None, [C.c_void_p]),
+        "sptGetBestModeOrder": (None, [P(sptIndex), sptIndex, P(sptIndex), sptIndex]),
+        "sptGetWorstModeOrder": (None, [P(sptIndex), sptIndex, P(sptIndex), sptIndex]),
+        "sptGetRandomShuffleElements": (None, [P(sptSparseTensor)]),
+        "sptSparseTensorSortIndexCustomOrder": (None, [P(sptSparseTensor), P(sptIndex), C.c_int, C.c_int]),
+        "sptSparseTensorSortIndexAtMode": (None, [P(sptSparseTensor), sptIndex, C.c_int, C.c_int]),
+        "sptSparseTensorMixedOrder": (C.c_int, [P(sptSparseTensor), sptElementIndex, sptElementIndex, C.c_int]),
+        "sptSparseTensorStatus": (None, [P(sptSparseTensor), C.c_void_p]),
+        "sptSparseTensorStatusHiCOO": (None, [P(sptSparseTensorHiCOO), C.c_void_p]),
+        "sptBytesString": (C.c_void_p, [C.c_uint64]),
+        "sptDumpIndexArray": (C.c_int, [P(sptIndex), sptNnzIndex, C.c_void_p]),
+        "sptDumpRankMatrix": (C.c_int, [P(sptRankMatrix), C.c_void_p]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name, None)

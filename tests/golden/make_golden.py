@@ -20,6 +20,15 @@ Fixtures (numpy .npz, no pickles) hold inputs and the reference's outputs:
   *.dump.txt      sptDumpSparseTensorHiCOO text for two small cases
   tns_*.npz       tensors read by sptLoadSparseTensor from the reference's
                   tensors/*.tns fixtures
+  driver.npz      sptGetBestModeOrder / sptGetWorstModeOrder (sort.c:193,239)
+                  on dimension lists with ties, every mode; a tensor after
+                  sptGetRandomShuffleElements (sort.c:371)
+  solve.npz       sptMatrixSolveNormals (matrix_solver.c:32, FP32 LAPACK):
+                  SPD Grams (Cholesky), a symmetric indefinite matrix (the
+                  gesv fallback) and exactly singular ones (rows left as
+                  they are)
+  status_*.txt    sptSparseTensorStatus (status.c:37) and
+                  sptSparseTensorStatusHiCOO (hicoo/status.c:28) text
 """
 from __future__ import annotations
 
@@ -55,6 +64,12 @@ def ref_lib():
     lib.sptFreeSparseTensor.argtypes = [P(sptSparseTensor)]
     lib.CpdAlsStep.argtypes = [P(sptSparseTensor), sptIndex, sptIndex, C.c_double, P(P(sptMatrix)), P(sptValue)]
     lib.CpdAlsStep.restype = C.c_double
+    lib.sptGetBestModeOrder.argtypes = [P(sptIndex), sptIndex, P(sptIndex), sptIndex]
+    lib.sptGetWorstModeOrder.argtypes = [P(sptIndex), sptIndex, P(sptIndex), sptIndex]
+    lib.sptGetRandomShuffleElements.argtypes = [P(sptSparseTensor)]
+    lib.sptSparseTensorStatus.argtypes = [P(sptSparseTensor), C.c_void_p]
+    lib.sptSparseTensorStatusHiCOO.argtypes = [P(sptSparseTensorHiCOO), C.c_void_p]
+    lib.sptMatrixSolveNormals.argtypes = [sptIndex, sptIndex, P(P(sptMatrix)), P(sptMatrix)]
     return lib
 
 
@@ -253,9 +268,91 @@ def write_tns():
         np.savez(os.path.join(HERE, f"tns_{name}.npz"), ndims=d, vals=v, **{f"ind{m}": i[m] for m in range(len(d))})
 
 
+DRIVER_DIMS = [[5, 3, 7], [4, 4, 4], [10, 2, 10], [3, 3, 9, 3], [8, 1, 8, 1, 8], [2, 7, 7, 2, 7, 2],
+               [6, 5, 4, 3, 2, 1], [1, 2, 3, 4, 5, 6, 7, 8], [9, 9, 9, 9, 9, 9, 9, 9]]
+
+
+def write_driver(cases):
+    out = {}
+    for k, dims in enumerate(DRIVER_DIMS):
+        N = len(dims)
+        nd = np.asarray(dims, dtype=np.uint32)
+        for mode in range(N):
+            for which, fn in (("best", ref.sptGetBestModeOrder), ("worst", ref.sptGetWorstModeOrder)):
+                o = np.zeros(N + 1, dtype=np.uint32)   # the worst order reads one slot past the end
+                fn(ptr(o, sptIndex), mode, ptr(nd, sptIndex), N)
+                out[f"{which}/{k}/{mode}"] = o[:N].copy()
+        out[f"dims/{k}"] = nd
+    d, inds, vals, _ = cases["3D_12031"]
+    coo = CooTensor(d, [a.copy() for a in inds], vals.copy())
+    ref.sptGetRandomShuffleElements(coo.ref())
+    out["shuffle/ndims"] = d
+    out["shuffle/vals_in"] = vals
+    out["shuffle/vals_out"] = coo.vals.copy()
+    for m in range(len(d)):
+        out[f"shuffle/ind_in{m}"] = inds[m]
+        out[f"shuffle/ind_out{m}"] = coo.inds[m].copy()
+    np.savez_compressed(os.path.join(HERE, "driver.npz"), **out)
+    for name in ("3D_12031", "4d_small"):
+        d, inds, vals, _ = cases[name]
+        coo = CooTensor(d, [a.copy() for a in inds], vals.copy())
+        fp = libc.fopen(os.path.join(HERE, f"status_coo_{name}.txt").encode(), b"w")
+        ref.sptSparseTensorStatus(coo.ref(), fp)
+        libc.fclose(fp)
+    for name, sb, sk in (("3D_12031", 2, 3), ("4d_small", 1, 2), ("zipf3", 7, 14)):
+        d, inds, vals, _ = cases[name]
+        _, h, _ = ref_hicoo(ref, d, inds, vals, sb, sk)
+        fp = libc.fopen(os.path.join(HERE, f"status_hicoo_{name}_sb{sb}_sk{sk}.txt").encode(), b"w")
+        ref.sptSparseTensorStatusHiCOO(C.byref(h), fp)
+        libc.fclose(fp)
+        ref.sptFreeSparseTensorHiCOO(C.byref(h))
+
+
+def solve_cases():
+    """(name, mode, grams, rhs) inputs of sptMatrixSolveNormals."""
+    rng = np.random.default_rng(77)
+    cases = []
+    R = 6
+    grams = []
+    for _ in range(3):
+        a = rng.random((40, R)).astype(np.float32)
+        grams.append((a.T.astype(np.float64) @ a.astype(np.float64)).astype(np.float32))
+    cases.append(("spd", 1, grams, rng.random((50, R)).astype(np.float32)))
+    R = 5
+    q, _ = np.linalg.qr(rng.standard_normal((R, R)))
+    v = (q * np.array([3.0, -2.0, 1.0, 0.5, -1.0])) @ q.T
+    v = ((v + v.T) / 2).astype(np.float32)
+    cases.append(("indefinite", 0, [np.eye(R, dtype=np.float32), v], rng.standard_normal((20, R)).astype(np.float32)))
+    R = 4
+    b = rng.integers(0, 4, size=(7, R)).astype(np.float32)
+    b[:, 1] = b[:, 0]                  # two equal columns: an exactly singular Gram
+    g = b.T @ b
+    cases.append(("singular", 0, [np.eye(R, dtype=np.float32), g], rng.random((10, R)).astype(np.float32)))
+    g0 = g.copy()
+    g0[2, :] = 0
+    g0[:, 2] = 0                       # a zero row and column
+    cases.append(("zero_column", 1, [g0, np.ones((R, R), dtype=np.float32), g], rng.random((9, R)).astype(np.float32)))
+    return cases
+
+
+def write_solve():
+    out = {}
+    for name, mode, grams, rhs in solve_cases():
+        R = grams[0].shape[0]
+        mats = [Matrix(gm) for gm in grams] + [Matrix(nrows=R, ncols=R)]
+        b = Matrix(rhs)
+        assert ref.sptMatrixSolveNormals(mode, len(grams), mats_array(mats), C.byref(b.m)) == 0
+        out[f"{name}/mode"] = np.int64(mode)
+        out[f"{name}/rhs_in"] = rhs
+        out[f"{name}/rhs_out"] = b.array().copy()
+        for m, gm in enumerate(grams):
+            out[f"{name}/gram{m}"] = gm
+    np.savez_compressed(os.path.join(HERE, "solve.npz"), **out)
+
+
 if __name__ == "__main__":
     # optional arguments name the fixtures to write (default: all)
-    want = set(sys.argv[1:]) or {"tns", "coo", "hicoo", "cpd", "omp_reduce"}
+    want = set(sys.argv[1:]) or {"tns", "coo", "hicoo", "cpd", "omp_reduce", "driver", "solve"}
     ref = ref_lib()
     cases = coo_cases()
     if "tns" in want:
@@ -268,4 +365,8 @@ if __name__ == "__main__":
         write_cpd(cases)
     if "omp_reduce" in want:
         write_omp_reduce(cases)
+    if "driver" in want:
+        write_driver(cases)
+    if "solve" in want:
+        write_solve()
     print("golden fixtures written to", HERE)

@@ -8,16 +8,17 @@ inputs: factors and lambda within 1e-4, fit within 1e-6 (FP32 factors and
 MTTKRP, FP64 solve/norms/fit).  BF16-stored factors: fit within 1e-4 of the
 FP64 oracle on a planted low-rank tensor."""
 import ctypes as C
+import os
 import sys
 
 import numpy as np
 import pytest
 
 import oracle
-from conftest import rel_fro
+from conftest import GOLDEN, rel_fro
 from sptol_amd import gen
 from sptol_amd.parti import (CooTensor, Matrix, SPTOL_BF16, SPTOL_F32, check, mats_array, ptr,
-                             sptKruskalTensor, sptIndex, sptolCpdStats, sptolDevMatrix, sptRankKruskalTensor,
+                             sptKruskalTensor, sptMatrix, sptIndex, sptolCpdStats, sptolDevMatrix, sptRankKruskalTensor,
                              sptRankMatrix, to_hicoo)
 
 pytestmark = pytest.mark.gpu
@@ -38,14 +39,15 @@ def _cpd_step(lib, d, inds, vals, init, iters, tol=-1.0):
 
 
 def test_cpd_step_vs_reference_and_oracle(gpu_lib, golden_cpd):
-    """The reference-method fit reuses the last FP32 MTTKRP for <X,K>; when
-    the fit nears 1, ||X||^2 + ||K||^2 - 2<X,K> cancels and a relative error
-    e of <X,K> becomes an absolute fit error of about sqrt(2e): for FP32
-    (e ~ 1e-7) that is ~5e-4, hence the looser bound on that path.  The
-    exact-fit pass (FP64 products and sums) measures the fit of the FP32
-    factors themselves, whose rounding (~6e-8 relative per entry) bounds
+    """CpdAlsStep returns the fit with <X,K> from an FP64 pass over the
+    nonzeros (FP64 products and sums of the FP32 factors), so its error is
+    the FP32 factors' own rounding (~6e-8 relative per entry), which bounds
     ||X - K|| / ||X|| near 1e-6 on an exactly low-rank tensor (the FP64
-    oracle reaches 1 - 1e-14 there): it is held to 1e-5."""
+    oracle reaches 1 - 1e-14 on the rank-2 test_cpd case): held to 1e-5.
+    The per-iteration fit that drives tol reuses the last FP32 MTTKRP for
+    <X,K>, as the reference does; near a fit of 1, ||X||^2 + ||K||^2 -
+    2<X,K> cancels and a relative error e of <X,K> becomes an absolute fit
+    error of about sqrt(2e), ~5e-4 for FP32: that fit is never returned."""
     g = golden_cpd
     for case in g.cases():
         d, inds, vals = g.tensor(case)
@@ -60,9 +62,76 @@ def test_cpd_step_vs_reference_and_oracle(gpu_lib, golden_cpd):
         for m in range(len(d)):
             assert rel_fro(A[m], g.get(case, f"A{m}")) < 2e-3, (case, m)
             assert rel_fro(A[m], om[m]) < 1e-4, (case, m)
-        assert abs(fit - ofits[-1]) < 1e-3, (case, fit, ofits[-1])
+        _say(f"  fit gpu {fit:.12f} oracle {ofits[-1]:.12f}")
+        assert abs(fit - ofits[-1]) < 1e-5, (case, fit, ofits[-1])
         _, _, st = _device_cpd(gpu_lib, d, inds, vals, init, iters, -1.0, SPTOL_F32, exact_fit=1)
         assert abs(st.fit - ofits[-1]) < 1e-5, (case, st.fit, ofits[-1])
+        assert st.fit == fit, case
+
+
+def _dense_planted(dims, R, seed, noise=1e-4):
+    """Every coordinate of the grid, values from a planted U[0,1) rank-R model
+    plus noise*rms Gaussian noise: exactly low rank up to the noise, so ALS
+    reaches a fit near 1 - noise."""
+    rng = np.random.default_rng(seed)
+    grid = np.stack(np.meshgrid(*[np.arange(x) for x in dims], indexing="ij"), -1).reshape(-1, len(dims))
+    inds = [grid[:, m].astype(np.uint32) for m in range(len(dims))]
+    factors = [rng.random((x, R)) for x in dims]
+    vals, _ = gen.planted_values(inds, factors, noise, rng)
+    return np.asarray(dims, dtype=np.uint32), inds, vals
+
+
+def test_exact_fit_order5_and_hicoo(gpu_lib):
+    """The exact fit beyond the 3-/4-mode COO kernel: a 5-mode COO tensor
+    (CpdAlsStep) and a HiCOO tensor of both view kinds
+    (sptolCpdAlsHicooDevice), both dense and exactly low rank up to noise
+    of 1e-4 of the rms, so that the reference-method fit would cancel; each within
+    1e-5 of the FP64 oracle, and the HiCOO fit within 1e-6 of the COO one
+    on the same tensor in the same order (their MTTKRPs sum in different
+    orders, so the FP32 factors differ in their last bits)."""
+    from gpu_util import VIEW_KIND
+    lib = gpu_lib
+    d, inds, vals = _dense_planted([6, 5, 4, 3, 3], 3, seed=51)
+    init = gen.uniform_factors(d, 3, seed=52)
+    iters = 30
+    _say("order-5 CpdAlsStep")
+    A, lam, fit, _ = _cpd_step(lib, d, inds, vals, init, iters)
+    om, olam, ofits = oracle.cpd_als(d, inds, vals, init, iters)
+    _say(f"  fit gpu {fit:.12f} oracle {ofits[-1]:.12f}")
+    assert ofits[-1] > 0.99
+    assert abs(fit - ofits[-1]) < 1e-5
+    assert rel_fro(lam, olam) < 1e-4
+    d3, inds3, vals3 = _dense_planted([16, 12, 10], 2, seed=53)
+    coo = CooTensor(d3, [a.copy() for a in inds3], vals3.copy())
+    h, _ = to_hicoo(lib, coo, 3, 7)            # coo now in HiCOO order
+    init3 = gen.uniform_factors(d3, 2, seed=54)
+    _, _, ofits3 = oracle.cpd_als(d3, coo.inds, coo.vals, init3, iters)
+    _, _, st_coo = _device_cpd(lib, d3, coo.inds, coo.vals, init3, iters, -1.0, SPTOL_F32, exact_fit=1)
+    for kind in ("tiled", "seg"):
+        check(lib, lib.sptolSetHicooViewKind(VIEW_KIND[kind]), "view kind")
+        try:
+            t = C.c_void_p()
+            check(lib, lib.sptolDevHicooCreate(C.byref(t), C.byref(h)), "DevHicooCreate")
+            dm = (sptolDevMatrix * 3)()
+            hm = [Matrix(a) for a in init3]
+            for m in range(3):
+                check(lib, lib.sptolDevMatrixCreate(C.byref(dm[m]), int(d3[m]), 2, SPTOL_F32), "create")
+                check(lib, lib.sptolDevMatrixUpload(C.byref(dm[m]), C.byref(hm[m].m)), "upload")
+            lam3 = np.zeros(2, np.float32)
+            st = sptolCpdStats()
+            st.exact_fit = 1
+            check(lib, lib.sptolCpdAlsHicooDevice(t, 2, iters, -1.0, dm, ptr(lam3, C.c_float), C.byref(st), None),
+                  "cpd hicoo")
+            for m in range(3):
+                lib.sptolDevMatrixFree(C.byref(dm[m]))
+            lib.sptolDevHicooFree(t)
+        finally:
+            lib.sptolSetHicooViewKind(0)
+        _say(f"  hicoo {kind}: fit {st.fit:.12f} coo {st_coo.fit:.12f} oracle {ofits3[-1]:.12f}")
+        assert ofits3[-1] > 0.99
+        assert abs(st.fit - ofits3[-1]) < 1e-5, kind
+        assert abs(st.fit - st_coo.fit) < 1e-6, kind
+    lib.sptFreeSparseTensorHiCOO(C.byref(h))
 
 
 def test_sptcpdals_dropin(gpu_lib):
@@ -262,3 +331,81 @@ def test_hicoo_device_cpd_matches_coo(gpu_lib):
     assert abs(out["hicoo"][2] - ofits[-1]) < 1e-5
     assert rel_fro(out["hicoo"][1], olam) < 1e-3
     lib.sptFreeSparseTensorHiCOO(C.byref(h))
+
+
+def _solve_normals(lib, mode, grams, rhs):
+    R = grams[0].shape[0]
+    mats = [Matrix(g) for g in grams] + [Matrix(nrows=R, ncols=R)]
+    b = Matrix(rhs)
+    lib.sptMatrixSolveNormals.argtypes = [C.c_uint32, C.c_uint32, C.POINTER(C.POINTER(sptMatrix)),
+                                          C.POINTER(sptMatrix)]
+    check(lib, lib.sptMatrixSolveNormals(mode, len(grams), mats_array(mats), C.byref(b.m)), "SolveNormals")
+    return b.array().copy(), mats[-1].array().copy()
+
+
+def _hadamard_upper(grams, mode):
+    N, R = len(grams), grams[0].shape[0]
+    V = np.ones((R, R))
+    for k in range(1, N):
+        g = np.triu(grams[(mode + k) % N].astype(np.float64))
+        V *= g + np.triu(g, 1).T
+    return V
+
+
+def test_solve_normals_cholesky_lu_and_singular(gpu_lib):
+    """sptMatrixSolveNormals (matrix_solver.c:32) on the device: SPD Grams
+    (Cholesky), a symmetric indefinite V (the LU-with-pivoting fallback) and
+    exactly singular V (rows left as they are).  Against numpy's FP64 solve
+    of the same V within 1e-6, against the compiled reference's FP32 LAPACK
+    (golden) within 1e-3, and bit for bit on the singular cases, where the
+    reference leaves the rows too."""
+    z = np.load(os.path.join(GOLDEN, "solve.npz"), allow_pickle=False)
+    for name in ("spd", "indefinite", "singular", "zero_column"):
+        mode = int(z[f"{name}/mode"])
+        grams = []
+        while f"{name}/gram{len(grams)}" in z:
+            grams.append(z[f"{name}/gram{len(grams)}"])
+        rhs = z[f"{name}/rhs_in"]
+        got, V = _solve_normals(gpu_lib, mode, grams, rhs)
+        want_V = _hadamard_upper(grams, mode)
+        assert np.array_equal(V, want_V.astype(np.float32)), name
+        ref = z[f"{name}/rhs_out"]
+        if name in ("singular", "zero_column"):
+            assert np.linalg.matrix_rank(want_V) < want_V.shape[0]
+            assert np.array_equal(got.view(np.uint32), rhs.view(np.uint32)), name
+            assert np.array_equal(ref.view(np.uint32), rhs.view(np.uint32)), name
+            continue
+        exact = np.linalg.solve(want_V, rhs.astype(np.float64).T).T
+        _say(f"  {name}: vs numpy {rel_fro(got, exact):.2e}, vs reference {rel_fro(got, ref):.2e}")
+        assert rel_fro(got, exact) < 1e-6, name
+        assert rel_fro(got, ref) < 1e-3, name
+    if name == "indefinite":
+        assert np.min(np.linalg.eigvalsh(want_V)) < 0
+
+
+def test_cpd_singular_normal_equations(gpu_lib):
+    """Initial factors with two equal columns of small dyadic values make the
+    first mode update's V exactly singular (Cholesky meets an exact zero,
+    LU an exact zero pivot): the solve reports it in singular_solves and
+    the run stays finite."""
+    d, inds, vals = gen.uniform_tensor([16, 12, 10], 800, seed=61)
+    R = 4
+    rng = np.random.default_rng(62)
+    init = []
+    for n in d:
+        a = (rng.integers(1, 5, size=(int(n), R)) * 0.5).astype(np.float32)
+        a[:, 1] = a[:, 0]
+        init.append(a)
+    # column 0 of each Gram: a perfect square, so sqrt is exact
+    for a in init:
+        s = float((a[:, 0].astype(np.float64) ** 2).sum())
+        k = 0
+        while np.sqrt(s) != np.floor(np.sqrt(s)) or np.sqrt(s) == 0:
+            a[k % len(a), 0] += 0.5
+            a[k % len(a), 1] = a[k % len(a), 0]
+            s = float((a[:, 0].astype(np.float64) ** 2).sum())
+            k += 1
+    A, lam, st = _device_cpd(gpu_lib, d, inds, vals, init, 2, -1.0, SPTOL_F32)
+    _say(f"  singular solves {st.singular_solves}, fit {st.fit:.6f}")
+    assert st.singular_solves >= 1
+    assert all(np.isfinite(x).all() for x in A) and np.isfinite(lam).all() and np.isfinite(st.fit)

@@ -103,6 +103,18 @@ sptNnzIndex sptolDevCooNnz(sptolDevCoo h);
  * Semantics of mats_order/mode as sptMTTKRP. */
 int  sptolDevMTTKRP(sptolDevCoo X, const sptolDevMatrix *mats, const sptIndex mats_order[],
                     sptIndex mode, sptolDevMatrix *out, void *stream);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops): the same
+ * result as sptolDevMTTKRP, bit for bit, with no device allocation.  A mode
+ * prepared by sptolDevCooPrepareMode (3- and 4-mode tensors) runs the
+ * segmented kernels with their carry space and work counter in a fixed
+ * library array of 16.3 MiB per GPU, held until the process ends; a tensor
+ * that needs more than 65536 ranges (about 134M nonzeros at rank 32) is
+ * rejected with SPTERR_VALUE_ERROR.  An unprepared mode runs the unsorted
+ * kernel, which needs no scratch.  Calls that overlap on one GPU share the
+ * array and race: order them on one stream. */
+int  sptolDevMTTKRPNoAlloc(sptolDevCoo X, const sptolDevMatrix *mats, const sptIndex mats_order[],
+                           sptIndex mode, sptolDevMatrix *out, void *stream);
 
 /* ---------------------------------------------------------------------- */
 /* Device-resident HiCOO tensors and MTTKRP                               */

@@ -151,6 +151,10 @@ uint32_t seg_chg(uint64_t nnz, uint32_t G, bool l2_factors);
 // out[0 .. nrows*stride) = MTTKRP of the view (no fill needed beforehand).
 int seg_mttkrp(SegView &v, const void *const *mats, const uint32_t *order, uint32_t R, uint32_t stride,
                int dtype, float *out, hipStream_t s);
+// The same kernels with the carry space and work counter in a fixed device
+// array (no allocation; at most 65536 ranges; overlapping calls on one GPU race).
+int seg_mttkrp_noalloc(const SegView &v, const void *const *mats, const uint32_t *order, uint32_t R,
+                       uint32_t stride, int dtype, float *out, hipStream_t s);
 
 // Device COO tensor: structure-of-arrays copy of the host tensor, plus
 // optional copies sorted by one mode.

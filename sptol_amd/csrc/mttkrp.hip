@@ -812,6 +812,25 @@ int sptolDevMTTKRP(sptolDevCoo X, const sptolDevMatrix *mats, const sptIndex mat
     return coo_mttkrp(X, mats, mats_order, mode, out, stream_of(stream));
 }
 
+// sptolDevMTTKRP without any device allocation: a prepared mode runs the
+// segmented kernels with their carry space and work counter in the
+// library's fixed scratch array; an unprepared one runs the unsorted kernel,
+// which needs no scratch.
+int sptolDevMTTKRPNoAlloc(sptolDevCoo X, const sptolDevMatrix *mats, const sptIndex mats_order[], sptIndex mode,
+                          sptolDevMatrix *out, void *stream)
+{
+    SPTOL_TRY(check_dev_mats(X->nmodes, X->ndims, mats, mats_order, mode, out, "DevCoo MTTKRP no-alloc"));
+    hipStream_t s = stream_of(stream);
+    SortedCopy &sc = X->sorted[mode];
+    if (sc.ready && sc.seg.ready && seg_ok(X->nmodes, out->stride)) {
+        const void *mp[kMaxModes];
+        for (uint32_t m = 0; m < X->nmodes; ++m) mp[m] = mats[m].values;
+        return seg_mttkrp_noalloc(sc.seg, mp, mats_order, out->ncols, out->stride, mats[0].dtype,
+                                  (float *)out->values, s);
+    }
+    return coo_mttkrp(X, mats, mats_order, mode, out, s);
+}
+
 int sptolDevHicooCreate(sptolDevHicoo *h, const sptSparseTensorHiCOO *X)
 {
     *h = nullptr;

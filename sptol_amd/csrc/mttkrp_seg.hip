@@ -12,6 +12,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 
 namespace sptol {
 
@@ -358,6 +359,70 @@ static void seg_launch_ft(const SegArgs &a, uint32_t *counter, hipStream_t s)
         n3 ? seg_launch_main<seg_lanes<FT, 3>(), 3, FT>(a, counter, s) : seg_launch_main<seg_lanes<FT, 3>(), 4, FT>(a, counter, s);
         break;
     }
+}
+
+// ---------------------------------------------------------------------------
+// Allocation-free launches: the carry space and the work counter of the
+// segmented kernel come from `scratch`, a fixed device array of this file,
+// instead of the view's own buffers (which the view allocates on its first
+// MTTKRP).  Every GPU the library runs on holds its own copy until the
+// process ends.  Layout: kNoAllocRanges carries of kSegCarryStride floats,
+// then kNoAllocRanges row ids, then the work counter.  The counter is zero
+// when the module loads and the carry kernel resets it at the end of every
+// call; carries and row ids are written by every range before the carry
+// kernel reads them, so nothing else needs clearing.  Calls that overlap on
+// one GPU share the array and race.
+// ---------------------------------------------------------------------------
+constexpr uint64_t kNoAllocRanges = 65536;
+constexpr size_t kNoAllocBytes = kNoAllocRanges * (kSegCarryStride + 1) * 4 + 256;
+static __device__ __align__(256) uint8_t scratch[kNoAllocBytes];
+
+constexpr int kNoAllocMaxDevices = 64;
+static uint8_t *g_scratch_ptr[kNoAllocMaxDevices];
+static std::mutex g_scratch_mu;
+
+// The device address of this device's `scratch` (the array's name is a host
+// stand-in in host code): looked up once per device with that device current.
+static int noalloc_scratch(uint8_t **p)
+{
+    int dev = 0;
+    SPTOL_HIP(hipGetDevice(&dev), "MTTKRP no-alloc");
+    if (dev < 0 || dev >= kNoAllocMaxDevices) SPTOL_CHECK(SPTERR_VALUE_ERROR, "MTTKRP no-alloc", "device index");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *q = nullptr;
+        SPTOL_HIP(hipGetSymbolAddress(&q, HIP_SYMBOL(scratch)), "MTTKRP no-alloc");
+        if (!q) SPTOL_CHECK(SPTERR_UNKNOWN, "MTTKRP no-alloc", "scratch address");
+        g_scratch_ptr[dev] = (uint8_t *)q;
+    }
+    *p = g_scratch_ptr[dev];
+    return 0;
+}
+
+int seg_mttkrp_noalloc(const SegView &v, const void *const *mats, const uint32_t *order, uint32_t R,
+                       uint32_t stride, int dtype, float *out, hipStream_t s)
+{
+    SegArgs a;
+    uint64_t fbytes = 0;
+    for (uint32_t k = 1; k < v.nmodes; ++k)
+        fbytes += (uint64_t)v.ndims[order[k]] * stride * (dtype == SPTOL_BF16 ? 2u : 4u);
+    const bool l2 = fbytes <= ((uint64_t)4 << 20);
+    SPTOL_TRY(seg_fill_args(v, mats, order, R, stride, dtype, out, seg_chg(v.nnz, 64 / seg_glpn(stride, dtype), l2), a));
+    if (a.nwaves > kNoAllocRanges)
+        SPTOL_CHECK(SPTERR_VALUE_ERROR, "MTTKRP no-alloc",
+                    "the tensor needs more ranges than the fixed scratch holds (at most 65536)");
+    uint8_t *base = nullptr;
+    SPTOL_TRY(noalloc_scratch(&base));
+    a.carry = (float *)base;
+    a.carry_row = (uint32_t *)(base + kNoAllocRanges * kSegCarryStride * 4);
+    uint32_t *counter = (uint32_t *)(base + kNoAllocRanges * (kSegCarryStride + 1) * 4);
+    if (dtype == SPTOL_BF16) seg_launch_ft<uint16_t>(a, counter, s);
+    else seg_launch_ft<float>(a, counter, s);
+    SPTOL_HIP(hipGetLastError(), "MTTKRP no-alloc launch");
+    const uint64_t cblocks = (a.nwaves + kSegCarrySlots - 1) / kSegCarrySlots;
+    hipLaunchKernelGGL(k_seg_carry, dim3((unsigned)cblocks), dim3(256), 0, s, a, counter);
+    SPTOL_HIP(hipGetLastError(), "MTTKRP no-alloc carry launch");
+    return 0;
 }
 
 int seg_mttkrp(SegView &v, const void *const *mats, const uint32_t *order, uint32_t R, uint32_t stride, int dtype,

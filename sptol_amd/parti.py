@@ -277,6 +277,8 @@ def _declare(lib):
         "sptolDevCooFree": (None, [C.c_void_p]),
         "sptolDevMTTKRP": (C.c_int, [C.c_void_p, P(sptolDevMatrix), P(sptIndex), sptIndex,
                                      P(sptolDevMatrix), C.c_void_p]),
+        "sptolDevMTTKRPNoAlloc": (C.c_int, [C.c_void_p, P(sptolDevMatrix), P(sptIndex), sptIndex,
+                                            P(sptolDevMatrix), C.c_void_p]),
         "sptolDevHicooCreate": (C.c_int, [P(C.c_void_p), P(sptSparseTensorHiCOO)]),
         "sptolDevHicooFree": (None, [C.c_void_p]),
         "sptolDevHicooPrepareMode": (C.c_int, [C.c_void_p, sptIndex]),

@@ -17,6 +17,7 @@
 // Within an item each lane group walks a contiguous slice of nonzeros with
 // the same gather pipeline as the plain HiCOO kernel.
 #include "dev_common.hpp"
+#include <atomic>
 #include <hipcub/hipcub.hpp>
 #include <vector>
 #include <cstring>
@@ -475,7 +476,7 @@ int build_hicoo_seg(sptolDevHicoo X, uint32_t mode, HicooView &v, hipStream_t s)
 #undef HS_HIP
 }
 
-static int g_view_kind = SPTOL_HICOO_VIEW_AUTO;
+static std::atomic<int> g_view_kind{SPTOL_HICOO_VIEW_AUTO};   // process-wide, set from any thread
 
 }  // namespace sptol
 
@@ -505,8 +506,9 @@ int hicoo_prepare(sptolDevHicoo h, uint32_t mode, hipStream_t s)
     if (v.ready || h->nnz == 0) return 0;
     const bool seg_shape = h->nmodes == 3 || h->nmodes == 4;
     bool use_seg = false;
-    if (g_view_kind == SPTOL_HICOO_VIEW_SEG) use_seg = seg_shape;
-    else if (g_view_kind == SPTOL_HICOO_VIEW_AUTO) use_seg = seg_shape && h->nnz < 8 * h->nb;
+    const int kind = g_view_kind.load();
+    if (kind == SPTOL_HICOO_VIEW_SEG) use_seg = seg_shape;
+    else if (kind == SPTOL_HICOO_VIEW_AUTO) use_seg = seg_shape && h->nnz < 8 * h->nb;
     if (use_seg) return build_hicoo_seg(h, mode, v, s);
     return build_hicoo_view(h, mode, v, s);
 }

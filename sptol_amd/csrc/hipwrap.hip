@@ -299,7 +299,11 @@ struct ProfRec {
 };
 static std::mutex g_prof_mu;
 static std::atomic<int> g_prof_on{0};
-static std::vector<ProfRec> g_prof;
+// open and closed records by token; a drain takes only the closed ones, so a
+// launch still between prof_start and prof_stop on another thread keeps its
+// record (and its token cannot name another launch's record)
+static std::unordered_map<int, ProfRec> g_prof;
+static int g_prof_next = 0;
 struct ProfTotal { double ms = 0; uint64_t n = 0; };
 static std::unordered_map<std::string, ProfTotal> g_prof_tot;
 
@@ -311,38 +315,48 @@ int prof_start(const char *name, hipStream_t s)
     ProfRec r;
     r.name = name;
     if (hipEventCreate(&r.a) != hipSuccess || hipEventCreate(&r.b) != hipSuccess) return -1;
-    if (hipEventRecord(r.a, s) != hipSuccess) return -1;
+    if (hipEventRecord(r.a, s) != hipSuccess) {
+        (void)hipEventDestroy(r.a);
+        (void)hipEventDestroy(r.b);
+        return -1;
+    }
     std::lock_guard<std::mutex> lk(g_prof_mu);
-    g_prof.push_back(r);
-    return (int)g_prof.size() - 1;
+    const int token = g_prof_next;
+    g_prof_next = g_prof_next == 0x7FFFFFFF ? 0 : g_prof_next + 1;
+    g_prof[token] = r;
+    return token;
 }
 
 void prof_stop(int token, hipStream_t s)
 {
     if (token < 0) return;
     std::lock_guard<std::mutex> lk(g_prof_mu);
-    if ((size_t)token < g_prof.size()) {
-        (void)hipEventRecord(g_prof[token].b, s);
-        g_prof[token].closed = true;
+    auto it = g_prof.find(token);
+    if (it != g_prof.end()) {
+        (void)hipEventRecord(it->second.b, s);
+        it->second.closed = true;
     }
 }
 
 static void prof_drain()
 {
     std::lock_guard<std::mutex> lk(g_prof_mu);
-    for (auto &r : g_prof) {
-        if (r.closed) {
-            float ms = 0;
-            if (hipEventSynchronize(r.b) == hipSuccess && hipEventElapsedTime(&ms, r.a, r.b) == hipSuccess) {
-                auto &t = g_prof_tot[r.name];
-                t.ms += ms;
-                t.n += 1;
-            }
+    for (auto it = g_prof.begin(); it != g_prof.end();) {
+        ProfRec &r = it->second;
+        if (!r.closed) {
+            ++it;
+            continue;
+        }
+        float ms = 0;
+        if (hipEventSynchronize(r.b) == hipSuccess && hipEventElapsedTime(&ms, r.a, r.b) == hipSuccess) {
+            auto &t = g_prof_tot[r.name];
+            t.ms += ms;
+            t.n += 1;
         }
         (void)hipEventDestroy(r.a);
         (void)hipEventDestroy(r.b);
+        it = g_prof.erase(it);
     }
-    g_prof.clear();
 }
 
 }  // namespace sptol
@@ -514,6 +528,7 @@ int sptolDevMatrixCreate(sptolDevMatrix *m, sptIndex nrows, sptIndex ncols, int 
 
 int sptolDevMatrixUpload(sptolDevMatrix *m, const sptMatrix *src)
 {
+    if (!m->values && m->nrows) SPTOL_CHECK(SPTERR_VALUE_ERROR, "DevMtx Upload", "matrix not created");
     if (src->stride != m->stride || src->nrows < m->nrows)
         SPTOL_CHECK(SPTERR_SHAPE_MISMATCH, "DevMtx Upload", "stride or rows differ");
     size_t n = (size_t)m->nrows * m->stride;
@@ -525,6 +540,7 @@ int sptolDevMatrixUpload(sptolDevMatrix *m, const sptMatrix *src)
 
 int sptolDevMatrixDownload(sptMatrix *dst, const sptolDevMatrix *m)
 {
+    if (!m->values && m->nrows) SPTOL_CHECK(SPTERR_VALUE_ERROR, "DevMtx Download", "matrix not created");
     if (dst->stride != m->stride || dst->cap < m->nrows)
         SPTOL_CHECK(SPTERR_SHAPE_MISMATCH, "DevMtx Download", "stride or capacity differ");
     size_t n = (size_t)m->nrows * m->stride;

@@ -15,13 +15,16 @@
 /* ------------------------------------------------------------------------ */
 /* Last error (thread-local, src/error/error.c:28-34)                        */
 /* ------------------------------------------------------------------------ */
+/* The reason is copied into a per-thread buffer (truncated to 255 bytes):
+ * nothing is allocated, so a worker thread that exits leaves nothing behind. */
 static __thread struct {
     const char *module;
     int errcode;
     const char *file;
     unsigned line;
-    char *reason;
-} t_err = { NULL, 0, NULL, 0, NULL };
+    char *reason;          /* t_err.buf, or NULL when no reason was given */
+    char buf[256];
+} t_err = { NULL, 0, NULL, 0, NULL, {0} };
 
 void sptol_complain(const char *module, int errcode, const char *file, unsigned line,
                     const char *reason)
@@ -31,8 +34,8 @@ void sptol_complain(const char *module, int errcode, const char *file, unsigned 
     t_err.file = file;
     t_err.line = line;
     if (reason) {
-        free(t_err.reason);
-        t_err.reason = strdup(reason);
+        snprintf(t_err.buf, sizeof(t_err.buf), "%s", reason);
+        t_err.reason = t_err.buf;
     }
 #ifndef NDEBUG
     if (t_err.reason && t_err.reason[0])
@@ -57,7 +60,7 @@ void sptClearLastError(void)
     t_err.errcode = 0;
     t_err.file = NULL;
     t_err.line = 0;
-    free(t_err.reason);
+    t_err.buf[0] = 0;
     t_err.reason = NULL;
 }
 
@@ -162,13 +165,16 @@ sptIndex sptMaxIndexArray(sptIndex const *indices, sptNnzIndex len)
 /* Seeded generator: splitmix64.  sptolSetSeed (declared in sptol_hip.h)     */
 /* resets it; the default seed is fixed so runs are reproducible.            */
 /* ------------------------------------------------------------------------ */
+/* The state only ever advances by a constant, so concurrent callers each
+ * take a distinct step with one atomic add; one thread alone sees the plain
+ * splitmix64 sequence. */
 static uint64_t g_rng_state = 0x5EED5EED5EED5EEDull;
 
-void sptolSetSeed(uint64_t seed) { g_rng_state = seed; }
+void sptolSetSeed(uint64_t seed) { __atomic_store_n(&g_rng_state, seed, __ATOMIC_RELAXED); }
 
 uint64_t sptol_rng_next(void)
 {
-    uint64_t z = (g_rng_state += 0x9E3779B97F4A7C15ull);
+    uint64_t z = __atomic_add_fetch(&g_rng_state, 0x9E3779B97F4A7C15ull, __ATOMIC_RELAXED);
     z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
     z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
     return z ^ (z >> 31);

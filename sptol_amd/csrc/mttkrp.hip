@@ -277,18 +277,17 @@ __global__ __launch_bounds__(256) void k_mttkrp_hicoo(const MttkrpArgs a)
 // ---------------------------------------------------------------------------
 // Launch logic
 // ---------------------------------------------------------------------------
-static int g_num_cus = 0;
-
 int num_cus()
 {
-    if (g_num_cus == 0) {
-        int dev = 0;
+    // read once, by whichever thread first asks (a thread-safe static)
+    static const int n = [] {
+        int dev = 0, c = 0;
         hipDeviceProp_t prop;
         if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)
-            g_num_cus = prop.multiProcessorCount;
-        if (g_num_cus <= 0) g_num_cus = 256;
-    }
-    return g_num_cus;
+            c = prop.multiProcessorCount;
+        return c > 0 ? c : 256;
+    }();
+    return n;
 }
 
 int launch_fill_zero(float *p, uint64_t n, hipStream_t s)

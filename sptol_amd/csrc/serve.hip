@@ -99,7 +99,10 @@ struct Request {
 // Compute streams (cohorts of active requests; 3 or 4 measured slower than 2)
 // and the most admission workers a server runs.
 constexpr int kCohorts = 2;
-constexpr int kMaxAdmitThreads = 4;
+#ifndef SPTOL_MAX_ADMIT_THREADS
+#define SPTOL_MAX_ADMIT_THREADS 4
+#endif
+constexpr int kMaxAdmitThreads = SPTOL_MAX_ADMIT_THREADS;
 
 static size_t estimate_bytes(const Request &r)
 {

@@ -1,18 +1,22 @@
 """GPU ingest: a device COO tensor streamed from a binary tensor file
 (sptolDevCooLoadBinary) equals one uploaded from host arrays
 (sptolDevCooCreate): the MTTKRP of every mode is bit-identical, since both
-build the same mode-sorted views.  A file with an index >= ndims is
-rejected.  The server's file submission (sptolServerSubmitFile) returns the
-same result, bit for bit, as submitting the host arrays."""
+build the same mode-sorted views, and within 1e-5 of the FP64 oracle.  A
+file with an index >= ndims is rejected.  The server's file submission
+(sptolServerSubmitFile) returns the same result, bit for bit, as submitting
+the host arrays, and its exact fit is within 1e-5 of the FP64 oracle's
+(factors and weights within 1e-4)."""
 import ctypes as C
 
 import numpy as np
 import pytest
 
+import oracle
+from conftest import rel_fro
 from gpu_util import dev_mttkrp_coo
 from sptol_amd import gen
 from sptol_amd.parti import (CooTensor, Matrix, SPTERR_VALUE_ERROR, SPTOL_F32, SPTOL_REQ_DONE, check,
-                             mats_array, ptr, sptolRequestInfo, sptolServerConfig)
+                             mats_array, ptr, seeded_uniform_factors, sptolRequestInfo, sptolServerConfig)
 
 pytestmark = pytest.mark.gpu
 
@@ -38,6 +42,10 @@ def test_load_binary_matches_upload(gpu_lib, tmp_path):
         a = dev_mttkrp_coo(lib, d, inds, vals, factors, order, mode, prepare=True, h=h)
         b = dev_mttkrp_coo(lib, d, inds, vals, factors, order, mode, prepare=True)
         assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), mode
+        want = oracle.mttkrp_coo(inds, vals, factors, np.asarray(order, np.uint32), mode, dtype=np.float64)
+        err = rel_fro(a[:, :16], want[:, :16])
+        _say(f"  mode {mode}: streamed tensor vs FP64 oracle {err:.2e}")
+        assert err < 1e-5, mode
     lib.sptolDevCooFree(h)
     bad = CooTensor(d, [a.copy() for a in inds], vals)
     bad.inds[2][len(vals) - 1] = d[2]
@@ -58,6 +66,7 @@ def test_server_submit_file(gpu_lib, tmp_path):
     cfg.hbm_budget = 16 << 30
     cfg.max_active = 4
     cfg.batch_mttkrp = 1
+    cfg.exact_fit = 1
     srv = C.c_void_p()
     check(lib, lib.sptolServerCreate(C.byref(srv), C.byref(cfg)), "ServerCreate")
     ids = []
@@ -87,3 +96,12 @@ def test_server_submit_file(gpu_lib, tmp_path):
     assert np.array_equal(l0.view(np.uint32), l1.view(np.uint32))
     for x, y in zip(A0, A1):
         assert np.array_equal(x.view(np.uint32), y.view(np.uint32))
+    # the file-submitted solve against the FP64 oracle from the request's
+    # seeded initial factors
+    init = seeded_uniform_factors(99, d, 8)
+    om, olam, ofits = oracle.cpd_als(d, inds, vals, init, 12)
+    _say(f"  served fit {f1:.9f} oracle {ofits[-1]:.9f}")
+    assert abs(f1 - ofits[-1]) < 1e-5
+    assert rel_fro(l1, olam) < 1e-4
+    for m in range(4):
+        assert rel_fro(A1[m], om[m]) < 1e-4, m

@@ -39,11 +39,23 @@ constexpr int kSegWPB = 4;
 #define SPTOL_SEG_WAVES4 4
 #endif
 
+// Records a lane-group step and the register budget (waves per SIMD) of a
+// class: 3-mode FP32 classes of 32 or more columns (factor rows that mostly
+// miss the caches) keep 8 records' gathers in flight, with 3 waves per SIMD
+// so that they fit in registers (the 100M-nonzero Zipf headline: 6.34 ms
+// per 3-mode step against 6.74 ms at 4 records and 4 waves); every other
+// class takes 4 records at the default budget.
 template <int LPN, int NM, typename FT>
-__global__ __launch_bounds__(kSegWPB * 64) __attribute__((amdgpu_waves_per_eu(NM == 3 ? SPTOL_SEG_WAVES3 : SPTOL_SEG_WAVES4, 8)))
+constexpr int seg_u() { return (NM == 3 && sizeof(FT) == 4 && LPN >= 8) ? 8 : kSegU; }
+template <int LPN, int NM, typename FT>
+constexpr int seg_waves() { return seg_u<LPN, NM, FT>() == 8 ? 3 : (NM == 3 ? SPTOL_SEG_WAVES3 : SPTOL_SEG_WAVES4); }
+
+template <int LPN, int NM, typename FT>
+__global__ __launch_bounds__(kSegWPB * 64) __attribute__((amdgpu_waves_per_eu(seg_waves<LPN, NM, FT>(), 8)))
 void k_mttkrp_seg(const SegArgs a, uint32_t *__restrict__ counter)
 {
-    __shared__ SegSharedOf<LPN, NM, FT, kSegWPB> sh;
+    constexpr int U = seg_u<LPN, NM, FT>();
+    __shared__ SegSharedOf<LPN, NM, FT, kSegWPB, U> sh;
     const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;
     // Wave w of the W in the grid takes range w first, then pulls ranges
     // W, W+1, ... from the counter: no atomics while every wave starts, none
@@ -51,7 +63,7 @@ void k_mttkrp_seg(const SegArgs a, uint32_t *__restrict__ counter)
     const uint64_t W = (uint64_t)gridDim.x * kSegWPB;
     uint64_t wave = (uint64_t)blockIdx.x * kSegWPB + wv;
     while (wave < a.nwaves) {
-        seg_range<LPN, NM, FT, kSegWPB>(a, wave, sh, wv, lane);
+        seg_range<LPN, NM, FT, kSegWPB, U>(a, wave, sh, wv, lane);
         if (W >= a.nwaves) break;   // every range was taken by index
         uint32_t wq = 0;
         if (lane == 0) wq = atomicAdd(counter, 1u);

@@ -62,7 +62,7 @@ int seg_fill_args(const SegView &v, const void *const *mats, const uint32_t *ord
                   int dtype, float *out, uint32_t chg, SegArgs &a);
 
 constexpr uint32_t kSegCarryStride = 64;
-constexpr int kSegU = 4;                 // records per lane-group step
+constexpr int kSegU = 4;                 // records per lane-group step (default)
 constexpr uint32_t kSegCarrySlots = 64;  // ranges per carry block
 
 __device__ __forceinline__ uint32_t seg_comp(const uint4 r, uint32_t k)
@@ -157,29 +157,29 @@ struct SegAcc {
     }
 };
 
-// Per-wave LDS of the range body, for WPB waves of C columns a lane.
-template <int LPN, int NM, int WPB, int C>
+// Per-wave LDS of the range body, for WPB waves of C columns a lane and U
+// records a lane-group step.
+template <int LPN, int NM, int WPB, int C, int U = kSegU>
 struct SegShared {
     static constexpr uint32_t G = 64 / LPN;
-    static constexpr uint32_t SL = kSegU + 1;   // staging slots per group (+1 spreads banks)
+    static constexpr uint32_t SL = U + 1;   // staging slots per group (+1 spreads banks)
     uint4 stage[WPB][G][SL];
     float stagev[WPB][G][NM == 4 ? SL : 1];
     float4 piece[WPB][64][C / 4];
     uint32_t info[WPB][G][2];
 };
 
-template <int LPN, int NM, typename FT, int WPB>
-using SegSharedOf = SegShared<LPN, NM, WPB, seg_cols<FT>()>;
+template <int LPN, int NM, typename FT, int WPB, int U = kSegU>
+using SegSharedOf = SegShared<LPN, NM, WPB, seg_cols<FT>(), U>;
 
-template <int LPN, int NM, typename FT, int WPB>
-__device__ __forceinline__ void seg_range(const SegArgs &a, uint64_t wave, SegSharedOf<LPN, NM, FT, WPB> &sh,
+template <int LPN, int NM, typename FT, int WPB, int U = kSegU>
+__device__ __forceinline__ void seg_range(const SegArgs &a, uint64_t wave, SegSharedOf<LPN, NM, FT, WPB, U> &sh,
                                           uint32_t wv, uint32_t lane)
 {
 #pragma clang fp contract(off)
     constexpr int C = seg_cols<FT>();
     using Cols = SegCols<FT, C>;
     using Raw = typename Cols::Raw;
-    constexpr int U = kSegU;
     constexpr uint32_t KL = (U + LPN - 1) / LPN;   // records each lane stages per step
     constexpr uint32_t G = 64 / LPN;
     const uint32_t g = lane / LPN, l = lane % LPN;

@@ -8,14 +8,20 @@ lognormal(0,1) float32, HiCOO block 2^7 and superblock 2^14, float32
 factors U[0,1) with R=32.  One step = one MTTKRP on each of the 3 modes
 (mats_order = [n, n+1, n+2] mod 3), tensor, factors and outputs resident in
 HBM.  The same tensor in COO form (one mode-sorted view per mode) is timed
-beside it.  Other workloads: --workload coo | cpd | batch | serve | copy.
+beside it.  The same line carries the served half of the metric
+(`served_cpd`): 400 requests of the served CPD-ALS stream at t=0, solves/s
+with p50/p99 latency, against the same requests through the plain
+one-at-a-time device path.  Other workloads: --workload coo | cpd | batch |
+serve | copy.
 
 Prints one JSON line: metric, value, unit, timing fields, the workload's
 config, `roofline` for the dominant kernel (algorithmic bytes per launch
 over its average duration measured with HIP events on its launch stream,
 against 8 TB/s; `traffic` = measured HBM bytes per launch from the committed
-rocprofv3 counter summary) and `cpu_baseline` (the compiled reference or
-the oracle on a bounded sample, on this host's cores).
+rocprofv3 counter summary), `cpu_baseline` (the compiled reference's OpenMP
+COO MTTKRP on the full tensor, on this host's cores; the oracle's
+restatement where the reference build is absent) and the reference's
+superblock-scheduled HiCOO kernel on a 4M-nonzero sample beside it.
 """
 from __future__ import annotations
 
@@ -38,6 +44,8 @@ from sptol_amd.parti import (CooTensor, Matrix, SPTOL_BF16, SPTOL_F32, check, lo
 HBM_PEAK = 8.0e12
 DIMS = [12_000_000, 9_000_000, 28_000_000]
 HICOO_NNZ = 100_000_000
+SERVE_REQUESTS_DEFAULT = 400
+METRIC = "MTTKRP Gnnz/s & %HBM roofline; CPD-ALS solves/s + p99 vs plain, 1 GPU"
 # counter summary of k_mttkrp_seg on the full-size HiCOO workload
 TRAFFIC_PROFILE = "k_mttkrp_seg_hicoo_12Mx9Mx28M_100Mnnz_R32"
 
@@ -71,7 +79,47 @@ def prof(lib, name):
     return ms.value, int(n.value)
 
 
-def cpu_baseline(lib, coo_inds, vals, dims, factors, order_of, budget_s=20.0):
+def cpu_baseline_full(coo_inds, vals, dims, factors, order_of):
+    """The reference's OpenMP COO MTTKRP sptOmpMTTKRP (mttkrp_omp.c:125,
+    compiled from the reference sources into oracle/_ref) on the full
+    benchmark tensor, every mode once, on the box's cores (at most 16); the
+    oracle's restatement of it when oracle/_ref is absent.  On this tensor it
+    is the reference's fastest CPU path (its superblock-scheduled HiCOO
+    kernel loops over every superblock row for each of nkiters steps)."""
+    tk = host_threads()
+    ref = ref_library()
+    nnz = len(vals)
+    R = factors[0].shape[1]
+    t_tot = 0.0
+    if ref is not None:
+        from sptol_amd.parti import mats_array
+        coo = CooTensor(dims, coo_inds, vals)
+        out = Matrix(nrows=int(max(dims)), ncols=R)
+        mats = [Matrix(f) for f in factors] + [out]
+        arr = mats_array(mats)
+        ref.sptOmpMTTKRP.argtypes = [C.POINTER(sptSparseTensor), C.c_void_p, C.POINTER(sptIndex), sptIndex, C.c_int]
+        with quiet_stdout():
+            for mode in range(3):
+                o = np.ascontiguousarray(order_of(mode), dtype=np.uint32)
+                out.m.nrows = int(dims[mode])
+                t0 = time.perf_counter()
+                rc = ref.sptOmpMTTKRP(coo.ref(), C.cast(arr, C.c_void_p), ptr(o, sptIndex), mode, tk)
+                t_tot += time.perf_counter() - t0
+                assert rc == 0
+        what, kind = "the compiled reference's sptOmpMTTKRP (COO, atomic, mttkrp_omp.c:125)", "reference"
+    else:
+        import oracle
+        for mode in range(3):
+            t0 = time.perf_counter()
+            oracle.mttkrp_coo(coo_inds, vals, factors, order_of(mode), mode, threads=tk)
+            t_tot += time.perf_counter() - t0
+        what, kind = "oracle restatement of sptOmpMTTKRP (COO, atomic, mttkrp_omp.c:125)", "port"
+    return {"value": 3 * nnz / t_tot / 1e9, "unit": "Gnnz/s", "cores": tk, "kind": kind,
+            "sample": f"{what}, tk={tk}, the full {nnz}-nonzero benchmark tensor, all 3 modes once, "
+                      f"R={R}, {t_tot:.1f} s of CPU time"}
+
+
+def cpu_baseline(lib, coo_inds, vals, dims, factors, order_of, budget_s=8.0):
     """The reference's OpenMP superblock-scheduled HiCOO MTTKRP
     (hicoo/mttkrp_omp.c:158, compiled from the reference sources into
     oracle/_ref) on a random sample of the same tensor; falls back to the
@@ -273,7 +321,7 @@ def run_hicoo(args, lib):
     achieved = B_avg / (k_ms * 1e-3)
 
     result = {
-        "metric": "MTTKRP Gnnz/s & %HBM roofline",
+        "metric": METRIC,
         "value": round(value, 4),
         "unit": "Gnnz/s",
         "n_gpus": 1,
@@ -337,10 +385,24 @@ def run_hicoo(args, lib):
 
     if not args.no_cpu:
         t0 = time.time()
-        result["cpu_baseline"] = cpu_baseline(lib, coo_keep, vals, dims, factors, lambda m: orders[m])
+        result["cpu_baseline"] = cpu_baseline_full(coo_keep, vals, dims, factors, lambda m: orders[m])
         log(f"CPU baseline in {time.time() - t0:.1f}s: {result['cpu_baseline']}")
+        t0 = time.time()
+        result["cpu_hicoo_scheduled_sample"] = cpu_baseline(lib, coo_keep, vals, dims, factors, lambda m: orders[m])
+        log(f"CPU HiCOO scheduled sample in {time.time() - t0:.1f}s: {result['cpu_hicoo_scheduled_sample']}")
     else:
         result["cpu_baseline"] = None
+    del coo_keep, vals, factors
+    if not args.no_serve:
+        # the served half of the metric: 400 requests of the served CPD stream
+        # at t=0, and the same requests through the plain one-at-a-time path
+        sargs = argparse.Namespace(nnz=None, requests=SERVE_REQUESTS_DEFAULT, rate=0.0, no_plain=False,
+                                   steps=args.steps, warmup=args.warmup)
+        sv = run_serve(sargs, lib)
+        result["served_cpd"] = {"value": sv["value"], "unit": "solves/s", "requests": SERVE_REQUESTS_DEFAULT,
+                                "latency_s": sv["latency_s"], "plain_one_at_a_time": sv.get("plain_one_at_a_time"),
+                                "speedup_vs_plain": sv.get("speedup_vs_plain"),
+                                "mean_iterations": sv["mean_iterations"], "config": sv["config"]}
     return result
 
 
@@ -770,6 +832,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-plain", action="store_true", help="serve: skip the plain one-at-a-time comparison")
     ap.add_argument("--no-coo", action="store_true", help="hicoo: skip the COO comparison")
+    ap.add_argument("--no-serve", action="store_true",
+                    help="hicoo: skip the served CPD leg (400 requests and the plain path)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: the library has no multi-GPU path (sharding is a "

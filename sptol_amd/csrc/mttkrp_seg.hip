@@ -50,7 +50,11 @@ constexpr int seg_u() { return (NM == 3 && sizeof(FT) == 4 && LPN >= 8) ? 8 : kS
 template <int LPN, int NM, typename FT>
 constexpr int seg_waves() { return seg_u<LPN, NM, FT>() == 8 ? 3 : (NM == 3 ? SPTOL_SEG_WAVES3 : SPTOL_SEG_WAVES4); }
 
-template <int LPN, int NM, typename FT>
+// Gathered factors above this many bytes (the L2 of all XCDs) live in HBM:
+// the 8-record classes then load records and store outputs non-temporally.
+constexpr uint64_t kSegNtFactorBytes = (uint64_t)32 << 20;
+
+template <int LPN, int NM, typename FT, bool NT = false>
 __global__ __launch_bounds__(kSegWPB * 64) __attribute__((amdgpu_waves_per_eu(seg_waves<LPN, NM, FT>(), 8)))
 void k_mttkrp_seg(const SegArgs a, uint32_t *__restrict__ counter)
 {
@@ -63,7 +67,7 @@ void k_mttkrp_seg(const SegArgs a, uint32_t *__restrict__ counter)
     const uint64_t W = (uint64_t)gridDim.x * kSegWPB;
     uint64_t wave = (uint64_t)blockIdx.x * kSegWPB + wv;
     while (wave < a.nwaves) {
-        seg_range<LPN, NM, FT, kSegWPB, U>(a, wave, sh, wv, lane);
+        seg_range<LPN, NM, FT, kSegWPB, U, NT>(a, wave, sh, wv, lane);
         if (W >= a.nwaves) break;   // every range was taken by index
         uint32_t wq = 0;
         if (lane == 0) wq = atomicAdd(counter, 1u);
@@ -336,17 +340,31 @@ static int seg_args(SegView &v, const void *const *mats, const uint32_t *order, 
     return 0;
 }
 
-template <int LPN, int NM, typename FT>
-static void seg_launch_main(const SegArgs &a, uint32_t *counter, hipStream_t s)
+template <int LPN, int NM, typename FT, bool NT>
+static void seg_launch_kernel(const SegArgs &a, uint32_t *counter, hipStream_t s)
 {
-    const void *fn = reinterpret_cast<const void *>(&k_mttkrp_seg<LPN, NM, FT>);
+    const void *fn = reinterpret_cast<const void *>(&k_mttkrp_seg<LPN, NM, FT, NT>);
     int per_cu = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, kSegWPB * 64, 0) != hipSuccess || per_cu < 1)
         per_cu = 1;
     uint64_t blocks = (uint64_t)num_cus() * per_cu;
     const uint64_t need = (a.nwaves + kSegWPB - 1) / kSegWPB;
     if (blocks > need) blocks = need;
-    hipLaunchKernelGGL((k_mttkrp_seg<LPN, NM, FT>), dim3((unsigned)blocks), dim3(kSegWPB * 64), 0, s, a, counter);
+    hipLaunchKernelGGL((k_mttkrp_seg<LPN, NM, FT, NT>), dim3((unsigned)blocks), dim3(kSegWPB * 64), 0, s, a, counter);
+}
+
+// nt: the gathered factors live in HBM (only the 8-record classes have a
+// non-temporal variant).
+template <int LPN, int NM, typename FT>
+static void seg_launch_main(const SegArgs &a, uint32_t *counter, hipStream_t s, bool nt)
+{
+    if constexpr (seg_u<LPN, NM, FT>() == 8) {
+        if (nt) {
+            seg_launch_kernel<LPN, NM, FT, true>(a, counter, s);
+            return;
+        }
+    }
+    seg_launch_kernel<LPN, NM, FT, false>(a, counter, s);
 }
 
 // lanes per group of the range kernels: FP32 {2, 4, 8, 16}, BF16 {1, 2, 4, 8}
@@ -354,21 +372,21 @@ template <typename FT, int I>
 constexpr int seg_lanes() { return (sizeof(FT) == 2 ? 1 : 2) << I; }
 
 template <typename FT>
-static void seg_launch_ft(const SegArgs &a, uint32_t *counter, hipStream_t s)
+static void seg_launch_ft(const SegArgs &a, uint32_t *counter, hipStream_t s, bool nt)
 {
     const bool n3 = a.nmodes == 3;
     switch (seg_lane_class(a.glpn, sizeof(FT) == 2 ? SPTOL_BF16 : SPTOL_F32)) {
     case 0:
-        n3 ? seg_launch_main<seg_lanes<FT, 0>(), 3, FT>(a, counter, s) : seg_launch_main<seg_lanes<FT, 0>(), 4, FT>(a, counter, s);
+        n3 ? seg_launch_main<seg_lanes<FT, 0>(), 3, FT>(a, counter, s, nt) : seg_launch_main<seg_lanes<FT, 0>(), 4, FT>(a, counter, s, nt);
         break;
     case 1:
-        n3 ? seg_launch_main<seg_lanes<FT, 1>(), 3, FT>(a, counter, s) : seg_launch_main<seg_lanes<FT, 1>(), 4, FT>(a, counter, s);
+        n3 ? seg_launch_main<seg_lanes<FT, 1>(), 3, FT>(a, counter, s, nt) : seg_launch_main<seg_lanes<FT, 1>(), 4, FT>(a, counter, s, nt);
         break;
     case 2:
-        n3 ? seg_launch_main<seg_lanes<FT, 2>(), 3, FT>(a, counter, s) : seg_launch_main<seg_lanes<FT, 2>(), 4, FT>(a, counter, s);
+        n3 ? seg_launch_main<seg_lanes<FT, 2>(), 3, FT>(a, counter, s, nt) : seg_launch_main<seg_lanes<FT, 2>(), 4, FT>(a, counter, s, nt);
         break;
     default:
-        n3 ? seg_launch_main<seg_lanes<FT, 3>(), 3, FT>(a, counter, s) : seg_launch_main<seg_lanes<FT, 3>(), 4, FT>(a, counter, s);
+        n3 ? seg_launch_main<seg_lanes<FT, 3>(), 3, FT>(a, counter, s, nt) : seg_launch_main<seg_lanes<FT, 3>(), 4, FT>(a, counter, s, nt);
         break;
     }
 }
@@ -418,7 +436,7 @@ int seg_mttkrp_noalloc(const SegView &v, const void *const *mats, const uint32_t
     uint64_t fbytes = 0;
     for (uint32_t k = 1; k < v.nmodes; ++k)
         fbytes += (uint64_t)v.ndims[order[k]] * stride * (dtype == SPTOL_BF16 ? 2u : 4u);
-    const bool l2 = fbytes <= ((uint64_t)4 << 20);
+    const bool l2 = fbytes <= ((uint64_t)4 << 20), nt = fbytes > kSegNtFactorBytes;
     SPTOL_TRY(seg_fill_args(v, mats, order, R, stride, dtype, out, seg_chg(v.nnz, 64 / seg_glpn(stride, dtype), l2), a));
     if (a.nwaves > kNoAllocRanges)
         SPTOL_CHECK(SPTERR_VALUE_ERROR, "MTTKRP no-alloc",
@@ -428,8 +446,8 @@ int seg_mttkrp_noalloc(const SegView &v, const void *const *mats, const uint32_t
     a.carry = (float *)base;
     a.carry_row = (uint32_t *)(base + kNoAllocRanges * kSegCarryStride * 4);
     uint32_t *counter = (uint32_t *)(base + kNoAllocRanges * (kSegCarryStride + 1) * 4);
-    if (dtype == SPTOL_BF16) seg_launch_ft<uint16_t>(a, counter, s);
-    else seg_launch_ft<float>(a, counter, s);
+    if (dtype == SPTOL_BF16) seg_launch_ft<uint16_t>(a, counter, s, nt);
+    else seg_launch_ft<float>(a, counter, s, nt);
     SPTOL_HIP(hipGetLastError(), "MTTKRP no-alloc launch");
     const uint64_t cblocks = (a.nwaves + kSegCarrySlots - 1) / kSegCarrySlots;
     hipLaunchKernelGGL(k_seg_carry, dim3((unsigned)cblocks), dim3(256), 0, s, a, counter);
@@ -445,11 +463,11 @@ int seg_mttkrp(SegView &v, const void *const *mats, const uint32_t *order, uint3
     uint64_t fbytes = 0;
     for (uint32_t k = 1; k < v.nmodes; ++k)
         fbytes += (uint64_t)v.ndims[order[k]] * stride * (dtype == SPTOL_BF16 ? 2u : 4u);
-    const bool l2 = fbytes <= ((uint64_t)4 << 20);
+    const bool l2 = fbytes <= ((uint64_t)4 << 20), nt = fbytes > kSegNtFactorBytes;
     SPTOL_TRY(seg_args(v, mats, order, R, stride, dtype, out, seg_chg(v.nnz, 64 / seg_glpn(stride, dtype), l2), a));
     int tok = prof_start("mttkrp_seg", s);
-    if (dtype == SPTOL_BF16) seg_launch_ft<uint16_t>(a, v.counter, s);
-    else seg_launch_ft<float>(a, v.counter, s);
+    if (dtype == SPTOL_BF16) seg_launch_ft<uint16_t>(a, v.counter, s, nt);
+    else seg_launch_ft<float>(a, v.counter, s, nt);
     prof_stop(tok, s);
     SPTOL_HIP(hipGetLastError(), "Seg MTTKRP launch");
     // always launched: it also resets the work counter

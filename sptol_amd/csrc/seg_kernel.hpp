@@ -74,6 +74,38 @@ __device__ __forceinline__ float4 seg_ld(const float *p) { return *reinterpret_c
 
 __device__ __forceinline__ void seg_st(float *p, const float4 v) { *reinterpret_cast<float4 *>(p) = v; }
 
+// With NT, output stores and record loads are non-temporal: neither is
+// read again by the launch, so their lines leave L2 first and factor rows
+// that are gathered again stay.  The classes whose factors live in HBM use
+// it (the 100M-nonzero Zipf headline: 6.17 ms per 3-mode step against
+// 6.35 ms with plain accesses); classes whose factors fit in L2 do not
+// (the 4-D CPD at R=32 ran 5% slower in FP32 and 13% in BF16 with it, its
+// outputs being read again at once by the solve).  Gathers stay plain:
+// non-temporal gathers of rarely used rows made the headline 25% slower.
+typedef float seg_f32x4 __attribute__((ext_vector_type(4)));
+typedef uint32_t seg_u32x4 __attribute__((ext_vector_type(4)));
+
+template <bool NT>
+__device__ __forceinline__ void seg_st_hint(float *p, const float4 v)
+{
+    if (NT) {
+        const seg_f32x4 w = {v.x, v.y, v.z, v.w};
+        __builtin_nontemporal_store(w, reinterpret_cast<seg_f32x4 *>(p));
+    } else {
+        seg_st(p, v);
+    }
+}
+
+template <bool NT>
+__device__ __forceinline__ uint4 seg_ld_rec(const uint4 *p)
+{
+    if (NT) {
+        const seg_u32x4 v = __builtin_nontemporal_load(reinterpret_cast<const seg_u32x4 *>(p));
+        return make_uint4(v.x, v.y, v.z, v.w);
+    }
+    return *p;
+}
+
 __device__ __forceinline__ void seg_add(float4 &a, const float4 b)
 {
 #pragma clang fp contract(off)
@@ -136,10 +168,12 @@ struct SegAcc {
 #pragma unroll
         for (int k = 0; k < C; ++k) v[k] += b.v[k];
     }
+    template <bool NT = false>
     __device__ __forceinline__ void store(float *p) const
     {
 #pragma unroll
-        for (int j = 0; j < C / 4; ++j) seg_st(p + 4 * j, make_float4(v[4 * j], v[4 * j + 1], v[4 * j + 2], v[4 * j + 3]));
+        for (int j = 0; j < C / 4; ++j)
+            seg_st_hint<NT>(p + 4 * j, make_float4(v[4 * j], v[4 * j + 1], v[4 * j + 2], v[4 * j + 3]));
     }
     __device__ __forceinline__ void to_lds(float4 *p) const
     {
@@ -172,7 +206,7 @@ struct SegShared {
 template <int LPN, int NM, typename FT, int WPB, int U = kSegU>
 using SegSharedOf = SegShared<LPN, NM, WPB, seg_cols<FT>(), U>;
 
-template <int LPN, int NM, typename FT, int WPB, int U = kSegU>
+template <int LPN, int NM, typename FT, int WPB, int U = kSegU, bool NT = false>
 __device__ __forceinline__ void seg_range(const SegArgs &a, uint64_t wave, SegSharedOf<LPN, NM, FT, WPB, U> &sh,
                                           uint32_t wv, uint32_t lane)
 {
@@ -217,7 +251,7 @@ __device__ __forceinline__ void seg_range(const SegArgs &a, uint64_t wave, SegSh
             const uint32_t idx = l + k * LPN;
             if (idx < U) {
                 const uint64_t zz = min<uint64_t>(zb + idx, ze - 1);
-                nx[k] = rec[zz];
+                nx[k] = seg_ld_rec<NT>(rec + zz);
                 if (NM == 4) nv[k] = a.vals[zz];
             }
         }
@@ -269,7 +303,7 @@ __device__ __forceinline__ void seg_range(const SegArgs &a, uint64_t wave, SegSh
             if (row != cur) {
                 if (cur != kNoRow) {
                     if (in_first && first_cont) facc = acc;
-                    else if (col_ok) acc.store(out + (size_t)cur * S + c0);
+                    else if (col_ok) acc.template store<NT>(out + (size_t)cur * S + c0);
                     in_first = false;
                 }
                 cur = row;
@@ -314,7 +348,7 @@ __device__ __forceinline__ void seg_range(const SegArgs &a, uint64_t wave, SegSh
     if (nonempty && !(single && first_cont)) {
         // this group holds the first nonzero of its last row
         const SegAcc<C> tot = chain(acc, g + 1);
-        if (col_ok) tot.store(out + (size_t)cur * S + c0);
+        if (col_ok) tot.template store<NT>(out + (size_t)cur * S + c0);
     }
     if (g == 0) {
         uint32_t crow = kNoRow;
@@ -344,7 +378,7 @@ __device__ __forceinline__ void seg_range(const SegArgs &a, uint64_t wave, SegSh
             const bool in = r < hi;
             const int k = (int)(((in ? r : lo) >> 5) - w0);
             const uint32_t w = (uint32_t)__shfl((int)myw, k);
-            if (in && !((w >> (r & 31u)) & 1u) && col_ok) zero.store(out + r * S + c0);
+            if (in && !((w >> (r & 31u)) & 1u) && col_ok) zero.template store<NT>(out + r * S + c0);
         }
     }
 }

@@ -175,7 +175,9 @@ __global__ __launch_bounds__(kBatchWPB * 64) __attribute__((amdgpu_waves_per_eu(
                                                               uint64_t total, uint32_t *__restrict__ counter)
 {
     __shared__ SegSharedOf<LPN, NM, FT, kBatchWPB> sh;
-    const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;
+    // the wave index is uniform across the wave: read from lane 0, the
+    // range's indices, bounds and arguments live in scalar registers
+    const uint32_t lane = threadIdx.x & 63u, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     // first range by wave index, then W, W+1, ... from the class counter
     const uint64_t W = (uint64_t)gridDim.x * kBatchWPB;
     for (uint64_t t = (uint64_t)blockIdx.x * kBatchWPB + wv; t < total;) {

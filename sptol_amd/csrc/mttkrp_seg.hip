@@ -60,7 +60,9 @@ void k_mttkrp_seg(const SegArgs a, uint32_t *__restrict__ counter)
 {
     constexpr int U = seg_u<LPN, NM, FT>();
     __shared__ SegSharedOf<LPN, NM, FT, kSegWPB, U> sh;
-    const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;
+    // the wave index is uniform across the wave: read from lane 0, the
+    // range's indices, bounds and arguments live in scalar registers
+    const uint32_t lane = threadIdx.x & 63u, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     // Wave w of the W in the grid takes range w first, then pulls ranges
     // W, W+1, ... from the counter: no atomics while every wave starts, none
     // at all in a launch of at most W ranges, balancing after that.

@@ -180,6 +180,14 @@ struct SegAcc {
 #pragma unroll
         for (int j = 0; j < C / 4; ++j) p[j] = make_float4(v[4 * j], v[4 * j + 1], v[4 * j + 2], v[4 * j + 3]);
     }
+    __device__ __forceinline__ void from_lds(const float4 *p)
+    {
+#pragma unroll
+        for (int j = 0; j < C / 4; ++j) {
+            const float4 t = p[j];
+            v[4 * j] = t.x; v[4 * j + 1] = t.y; v[4 * j + 2] = t.z; v[4 * j + 3] = t.w;
+        }
+    }
     __device__ __forceinline__ void add_lds(const float4 *p)
     {
 #pragma clang fp contract(off)
@@ -231,9 +239,12 @@ __device__ __forceinline__ void seg_range(const SegArgs &a, uint64_t wave, SegSh
     const uint64_t zs = (wave * G + g) * (uint64_t)a.chg;
     const uint64_t ze = zs < a.nnz ? min<uint64_t>(zs + a.chg, a.nnz) : zs;
     const bool nonempty = zs < ze;
-    SegAcc<C> acc, facc;
+    // A group's "piece" (what it adds to a row begun before it) goes to its
+    // lanes' LDS slots as soon as it is known, instead of being held in
+    // registers across the loop: the first row's sum when that row ends
+    // inside the group, else (a single row) the whole sum at the end.
+    SegAcc<C> acc;
     acc.zero();
-    facc.zero();
     uint32_t cur = kNoRow, frow = kNoRow;
     bool first_cont = false, in_first = true;
     if (nonempty) {
@@ -302,7 +313,7 @@ __device__ __forceinline__ void seg_range(const SegArgs &a, uint64_t wave, SegSh
             const uint32_t row = rw[u];
             if (row != cur) {
                 if (cur != kNoRow) {
-                    if (in_first && first_cont) facc = acc;
+                    if (in_first && first_cont) acc.to_lds(sh.piece[wv][lane]);
                     else if (col_ok) acc.template store<NT>(out + (size_t)cur * S + c0);
                     in_first = false;
                 }
@@ -323,13 +334,11 @@ __device__ __forceinline__ void seg_range(const SegArgs &a, uint64_t wave, SegSh
     }
 
     // Merge the groups' boundary rows.  flags: 1 nonempty, 2 first row
-    // continues from the previous nonzero, 4 single row.  A group's "piece"
-    // is what it adds to a row begun before it: its whole sum when single,
-    // else its first row's sum.
+    // continues from the previous nonzero, 4 single row.  A piece is read
+    // only from groups whose first row continues (flags 1 and 2).
     const bool single = in_first;
     const uint32_t flags = (nonempty ? 1u : 0u) | (first_cont ? 2u : 0u) | (single ? 4u : 0u);
-    const SegAcc<C> piece = single ? acc : facc;
-    piece.to_lds(sh.piece[wv][lane]);
+    if (single) acc.to_lds(sh.piece[wv][lane]);
     if (l == 0) {
         sh.info[wv][g][0] = frow;
         sh.info[wv][g][1] = flags;
@@ -353,7 +362,9 @@ __device__ __forceinline__ void seg_range(const SegArgs &a, uint64_t wave, SegSh
     if (g == 0) {
         uint32_t crow = kNoRow;
         if (nonempty && first_cont) {
-            const SegAcc<C> tot = single ? chain(piece, 1) : piece;
+            SegAcc<C> tot;
+            if (single) tot = chain(acc, 1);
+            else tot.from_lds(sh.piece[wv][lane]);
             crow = frow;
             if (col_ok) tot.store(a.carry + wave * kSegCarryStride + c0);
         }

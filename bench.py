@@ -429,6 +429,25 @@ def free_factors(lib, arr):
         lib.sptolDevMatrixFree(C.byref(arr[m]))
 
 
+def dropin_rate(lib, coo, factors, dims, R, order, nnz, reps=20):
+    """The reference-named sptMTTKRP (mttkrp.c:39) on host arrays, as a
+    reference caller uses it: every call uploads the tensor and the factors,
+    runs the MTTKRP on the GPU and copies the output back, so the rate
+    includes PCIe and the per-call upload (not the `value` of any line)."""
+    from sptol_amd.parti import mats_array
+    out = Matrix(nrows=int(dims[0]), ncols=R)
+    mats = [Matrix(f) for f in factors] + [out]
+    arr = mats_array(mats)
+    o = ptr(order, sptIndex)
+    check(lib, lib.sptMTTKRP(coo.ref(), arr, o, 0), "sptMTTKRP")   # warm-up (allocator, module load)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        check(lib, lib.sptMTTKRP(coo.ref(), arr, o, 0), "sptMTTKRP")
+    dt = time.perf_counter() - t0
+    return {"value": round(nnz * reps / dt / 1e9, 4), "unit": "Gnnz/s", "ms_per_call": round(dt / reps * 1e3, 4),
+            "calls": reps, "what": "sptMTTKRP with host arrays in and out (upload, GPU MTTKRP, download per call)"}
+
+
 def run_coo(args, lib):
     """3-D COO MTTKRP, mode 0: 2000x1500x1000, 1e6 uniform nonzeros, R=16
     (the reference driver's configuration, examples/mttkrp.c)."""
@@ -466,6 +485,7 @@ def run_coo(args, lib):
                        "kernel": KERNEL_SYMBOL[t["kernel"]], "kernel_avg_ms": round(k_ms, 5),
                        "algorithmic_bytes_per_launch": B, "carry_kernel_avg_ms": round(t["mttkrp_seg_carry_ms"], 5),
                        "fill_zero_avg_ms": round(t["fill_zero_ms"], 5)}
+    res["dropin_sptMTTKRP"] = dropin_rate(lib, coo, factors, dims, R, order, nnz)
     if not args.no_cpu:
         res["cpu_baseline"] = cpu_baseline_coo(dims, inds, vals, factors, order)
     lib.sptolDevMatrixFree(C.byref(out))

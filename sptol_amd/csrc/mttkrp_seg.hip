@@ -263,12 +263,16 @@ int seg_build(SegView &v, uint32_t nmodes, uint32_t mode, uint32_t nrows, uint64
 
 // Nonzeros per lane group for a tensor of nnz nonzeros: about 32 ranges per
 // compute unit, between 16 and 256; when the gathered factors fit in one
-// XCD's L2, about 16 ranges per compute unit (one per resident wave) and up
-// to 768 (SPTOL_SEG_CHG overrides, for tuning).  With factors in HBM,
+// XCD's L2, about 8 ranges per compute unit (half the resident waves) and
+// up to 768 (SPTOL_SEG_CHG overrides, for tuning).  With factors in HBM,
 // shorter ranges balance the waves better (the 12M x 9M x 28M HiCOO
 // headline: 42.1 Gnnz/s at 256, 41.0 at 512); with factors in L2 the
 // per-range work (merge, carry, zero fill) counts more (the 4-D CPD at R=32,
-// per iteration: FP32 4.24 ms at 256 and 4.12 at 768, BF16 3.28 and 3.22).
+// per iteration: FP32 4.24 ms at 256 and 4.12 at 768, BF16 3.28 and 3.22;
+// the 1M-nonzero 2000 x 1500 x 1000 MTTKRP at R=16: 19.6 us per call at 16
+// nonzeros a group, 15.5 at 32, 16.2 at 48).  Ranges beyond the resident
+// waves are handed out by one atomic counter, whose serialised updates cost
+// more than short ranges save: that MTTKRP took 144 us at 8 a group.
 uint32_t seg_chg(uint64_t nnz, uint32_t G, bool l2_factors)
 {
     static const int env = env_int("SPTOL_SEG_CHG");
@@ -276,7 +280,7 @@ uint32_t seg_chg(uint64_t nnz, uint32_t G, bool l2_factors)
     if (env > 0) {
         c = (uint64_t)env;
     } else {
-        const uint64_t target = (uint64_t)num_cus() * (l2_factors ? 16 : 32);
+        const uint64_t target = (uint64_t)num_cus() * (l2_factors ? 8 : 32);
         c = (nnz + G * target - 1) / (G * target);
         const uint64_t cap = l2_factors ? 768 : 256;
         if (c < 16) c = 16;

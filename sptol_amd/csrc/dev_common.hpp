@@ -174,6 +174,9 @@ struct sptolDevCoo_s {
     uint32_t *inds[sptol::kMaxModes] = {};
     float *vals = nullptr;
     sptol::SortedCopy sorted[sptol::kMaxModes];
+    // mode m's indices are already non-decreasing in the tensor's own order
+    // (found by the upload's check pass): its view needs no sort
+    bool in_order[sptol::kMaxModes] = {};
 };
 
 struct sptolHicooViews;

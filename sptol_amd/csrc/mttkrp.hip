@@ -484,16 +484,31 @@ struct CheckArgs {
     uint32_t nmodes;
 };
 
-// bad[0] |= 1 for an index >= ndims, |= 2 for a NaN or infinite value: one
-// pass over every array of the tensor.
+// bad[0] |= 1 for an index >= ndims, |= 2 for a NaN or infinite value,
+// |= 4 << m where mode m's index decreases from one nonzero to the next:
+// one pass over every array of the tensor.
 __global__ void k_check_tensor(const CheckArgs a, uint32_t *__restrict__ bad)
 {
     uint32_t b = 0;
     for (uint64_t z = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; z < a.nnz; z += (uint64_t)gridDim.x * blockDim.x) {
-        for (uint32_t m = 0; m < a.nmodes; ++m) b |= a.inds[m][z] >= a.ndims[m] ? 1u : 0u;
+        for (uint32_t m = 0; m < a.nmodes; ++m) {
+            const uint32_t i = a.inds[m][z];
+            b |= i >= a.ndims[m] ? 1u : 0u;
+            b |= (z > 0 && i < a.inds[m][z - 1]) ? (4u << m) : 0u;
+        }
         if (a.vals) b |= (__float_as_uint(a.vals[z]) & 0x7F800000u) == 0x7F800000u ? 2u : 0u;
     }
-    if (b) atomicOr(bad, b);
+    // Out-of-order modes set bits in most threads: merge them in LDS and
+    // let one thread per block add only bits the flag does not hold yet, so
+    // the global atomics stay few (one per block at most, none once every
+    // bit is known).
+    __shared__ uint32_t sb;
+    if (threadIdx.x == 0) sb = 0u;
+    __syncthreads();
+    if (b) atomicOr(&sb, b);
+    __syncthreads();
+    if (threadIdx.x == 0 && sb && (sb & ~__hip_atomic_load(bad, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)))
+        atomicOr(bad, sb);
 }
 
 // Reject any index >= ndims (and, with check_vals, any non-finite value)
@@ -521,6 +536,7 @@ static int check_inds_dev(sptolDevCoo t, hipStream_t s, const char *module, bool
     SPTOL_HIP(e, module);
     if (hb & 1u) SPTOL_CHECK(SPTERR_VALUE_ERROR, module, "index >= ndims");
     if (hb & 2u) SPTOL_CHECK(SPTERR_VALUE_ERROR, module, "non-finite value");
+    for (uint32_t m = 0; m < t->nmodes; ++m) t->in_order[m] = !(hb & (4u << m));
     return 0;
 }
 
@@ -623,10 +639,13 @@ int coo_prepare(sptolDevCoo h, uint32_t mode, hipStream_t s)
 
 int coo_prepare_all(sptolDevCoo h, hipStream_t s)
 {
-    bool todo = false;
-    for (uint32_t m = 0; m < h->nmodes; ++m) todo = todo || !h->sorted[m].ready;
+    bool todo = false, sorts = false;
+    for (uint32_t m = 0; m < h->nmodes; ++m) {
+        todo = todo || !h->sorted[m].ready;
+        sorts = sorts || (!h->sorted[m].ready && !h->in_order[m]);
+    }
     if (!todo) return 0;
-    if ((h->nmodes != 3 && h->nmodes != 4) || h->nnz == 0 || h->nnz > kMaxSortItems) {
+    if ((h->nmodes != 3 && h->nmodes != 4) || h->nnz == 0 || h->nnz > kMaxSortItems || !sorts) {
         for (uint32_t m = 0; m < h->nmodes; ++m) SPTOL_TRY(coo_prepare(h, m, s));
         return 0;
     }
@@ -636,12 +655,13 @@ int coo_prepare_all(sptolDevCoo h, hipStream_t s)
     // modes of at most 2^16 rows sort 16-bit keys (6 bytes a pass instead of 8)
     uint16_t *k16[kMaxModes] = {}, *k16_all = nullptr;
     uint32_t n16 = 0;
-    for (uint32_t m = 0; m < h->nmodes; ++m) n16 += (!h->sorted[m].ready && h->ndims[m] <= 65536u) ? 1u : 0u;
+    for (uint32_t m = 0; m < h->nmodes; ++m)
+        n16 += (!h->sorted[m].ready && !h->in_order[m] && h->ndims[m] <= 65536u) ? 1u : 0u;
     int r = dmalloc((void **)&rec0, h->nnz * sizeof(uint4), "DevCoo Prepare");
     if (!r) r = dmalloc((void **)&iota, h->nnz * 4, "DevCoo Prepare");
     if (!r && n16) r = dmalloc((void **)&k16_all, h->nnz * 2 * n16, "DevCoo Prepare");
     for (uint32_t m = 0, k = 0; m < h->nmodes && !r && n16; ++m)
-        if (!h->sorted[m].ready && h->ndims[m] <= 65536u) k16[m] = k16_all + (size_t)(k++) * h->nnz;
+        if (!h->sorted[m].ready && !h->in_order[m] && h->ndims[m] <= 65536u) k16[m] = k16_all + (size_t)(k++) * h->nnz;
     if (!r) r = seg_pack_unsorted(rec0, h->nmodes, h->nnz, h->inds, h->vals, s, k16);
     if (!r) hipLaunchKernelGGL(k_iota, dim3(grid_for(h->nnz)), dim3(256), 0, s, iota, h->nnz);
     for (uint32_t m = 0; m < h->nmodes && !r; ++m) r = coo_prepare_from(h, m, s, rec0, iota, k16[m]);
@@ -669,6 +689,13 @@ static int coo_prepare_from(sptolDevCoo h, uint32_t mode, hipStream_t s, const u
     void *tmp = nullptr;
     size_t tmp_bytes = 0;
     int r = 0;
+    if ((h->nmodes == 3 || h->nmodes == 4) && h->in_order[mode]) {
+        // already in mode order: the records are packed in the tensor's own
+        // order, which a stable sort would leave unchanged
+        SPTOL_TRY(seg_build(sc.seg, h->nmodes, mode, h->ndims[mode], n, h->inds, h->vals, true, s, h->ndims));
+        sc.ready = true;
+        return 0;
+    }
     // 16-bit keys need the packed records (the view is built by gathering them)
     const bool k16 = keys16 && rec0 && (h->nmodes == 3 || h->nmodes == 4) && h->ndims[mode] <= 65536u;
     auto cleanup = [&]() { dfree(perm); dfree(perm2); dfree(keys2); dfree(tmp); };

@@ -116,6 +116,14 @@ struct SegView {
     uint32_t *carry_row = nullptr; // per wave: that row, or kNoRow
     uint64_t carry_waves = 0;      // capacity in waves (64 floats each)
     uint32_t *counter = nullptr;   // work counter of the persistent kernel (0 between launches), after touched
+    // Ranges of whole rows (built on demand, for one range size): range k
+    // holds the nonzeros of rows [k*rr_rows, (k+1)*rr_rows), from
+    // rr_start[k] to rr_start[k+1], so no row continues across ranges
+    uint64_t *rr_start = nullptr;  // device, rr_nwaves + 1 offsets
+    uint64_t rr_key = 0;           // the nonzeros-per-range target it was built for (0: not built)
+    uint64_t rr_nwaves = 0;
+    uint32_t rr_rows = 0;
+    bool rr_ok = false;            // balanced enough to use
     bool ready = false;
     void release();
 };

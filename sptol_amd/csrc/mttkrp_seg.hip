@@ -4,8 +4,10 @@
 // The main kernel is persistent: each wave takes its first range of
 // nonzeros by its own index and the following ones from a work counter
 // until they run out, so ranges of uneven cost balance without a burst of
-// atomics at the start; the carry kernel that always follows resets the
-// counter for the next launch.
+// atomics at the start; the carry kernel that follows resets the counter
+// for the next launch.  A launch over ranges of whole rows takes every
+// range by index (they fit in the resident waves), leaves the counter at 0
+// and has no carry kernel.
 // Calls on one view must be ordered (one stream): they share the view's
 // counter and carry buffers.
 #include "seg_kernel.hpp"
@@ -13,6 +15,8 @@
 #include <cstdlib>
 #include <cstring>
 #include <mutex>
+#include <vector>
+#include <algorithm>
 
 namespace sptol {
 
@@ -27,6 +31,10 @@ void SegView::release()
     dfree(carry_row); carry_row = nullptr;
     carry_waves = 0;
     counter = nullptr;
+    dfree(rr_start); rr_start = nullptr;
+    rr_key = rr_nwaves = 0;
+    rr_rows = 0;
+    rr_ok = false;
     ready = false;
 }
 
@@ -204,6 +212,8 @@ int seg_pack_unsorted(uint4 *rec0, uint32_t nmodes, uint64_t nnz, uint32_t *cons
     return 0;
 }
 
+static int seg_row_ranges(SegView &v, hipStream_t s);
+
 int seg_build(SegView &v, uint32_t nmodes, uint32_t mode, uint32_t nrows, uint64_t nnz, uint32_t *const *inds,
               const float *vals, bool copy_vals, hipStream_t s, const uint32_t *ndims, const uint32_t *perm,
               const uint32_t *sorted_rows, const uint4 *rec0, const uint16_t *sorted_rows16)
@@ -258,7 +268,7 @@ int seg_build(SegView &v, uint32_t nmodes, uint32_t mode, uint32_t nrows, uint64
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     if (e != hipSuccess) { v.release(); SPTOL_HIP(e, "Seg view"); }
     v.ready = true;
-    return 0;
+    return seg_row_ranges(v, s);
 }
 
 // Nonzeros per lane group for a tensor of nnz nonzeros: about 32 ranges per
@@ -328,11 +338,74 @@ int seg_fill_args(const SegView &v, const void *const *mats, const uint32_t *ord
     return 0;
 }
 
-// Arguments of a single MTTKRP on the view, with the view's own carry space.
-static int seg_args(SegView &v, const void *const *mats, const uint32_t *order, uint32_t R, uint32_t stride,
-                    int dtype, float *out, uint32_t chg, SegArgs &a)
+// First nonzero of every range of whole rows: start[k] = the first nonzero
+// whose row is >= k * rows (binary search over the row-sorted records),
+// for k = 0 .. nranges.
+__global__ __launch_bounds__(256) void k_seg_row_starts(const uint4 *__restrict__ rec, uint64_t nnz, uint32_t mode,
+                                                        uint32_t rows, uint64_t nranges, uint64_t *__restrict__ start)
 {
-    SPTOL_TRY(seg_fill_args(v, mats, order, R, stride, dtype, out, chg, a));
+    const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k > nranges) return;
+    const uint64_t target = k * rows;
+    uint64_t lo = 0, hi = nnz;   // first z in [lo, hi] with row >= target
+    while (lo < hi) {
+        const uint64_t mid = lo + (hi - lo) / 2;
+        if ((uint64_t)seg_comp(rec[mid], mode) < target) lo = mid + 1;
+        else hi = mid;
+    }
+    start[k] = k == nranges ? nnz : lo;
+}
+
+// Ranges of whole rows for the view: about 8 ranges per compute unit (the
+// range count of a tensor whose factors fit in L2), each a run of rows
+// averaging nnz / ranges nonzeros.  Kept (rr_ok) when no range holds more
+// than 1.5 times the average (rows of even length, as in a uniform tensor)
+// and every range fits in the resident waves of one launch of any range
+// kernel class (at least 3 waves per SIMD).  The range kernels split each
+// range evenly over their lane groups, so one table serves every rank and
+// factor type.  Synchronises s.
+static int seg_row_ranges(SegView &v, hipStream_t s)
+{
+    dfree(v.rr_start);
+    v.rr_start = nullptr;
+    v.rr_ok = false;
+    v.rr_nwaves = 0;
+    v.rr_rows = 0;
+    if (!v.nnz || !v.nrows) return 0;
+    const uint64_t target = (uint64_t)num_cus() * 8, max_ranges = (uint64_t)num_cus() * 12;
+    const double per_range = (double)v.nnz / (double)target;
+    uint64_t rows = (uint64_t)((double)v.nrows * per_range / (double)v.nnz);
+    if (rows < 1) rows = 1;
+    if (rows > v.nrows) rows = v.nrows;
+    const uint64_t n = (v.nrows + rows - 1) / rows;
+    if (n > max_ranges || rows > 0xFFFFFFFFull) return 0;
+    SPTOL_TRY(dmalloc((void **)&v.rr_start, (n + 1) * 8, "Seg row ranges"));
+    hipLaunchKernelGGL(k_seg_row_starts, dim3((unsigned)((n + 256) / 256)), dim3(256), 0, s, v.rec, v.nnz, v.mode,
+                       (uint32_t)rows, n, v.rr_start);
+    std::vector<uint64_t> h(n + 1);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(h.data(), v.rr_start, (n + 1) * 8, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) {
+        dfree(v.rr_start);
+        v.rr_start = nullptr;
+        SPTOL_HIP(e, "Seg row ranges");
+    }
+    uint64_t most = 0;
+    for (uint64_t k = 0; k < n; ++k) most = std::max(most, h[k + 1] - h[k]);
+    v.rr_rows = (uint32_t)rows;
+    v.rr_nwaves = n;
+    v.rr_ok = (double)most <= 1.5 * (double)v.nnz / (double)n;
+    if (!v.rr_ok) {
+        dfree(v.rr_start);
+        v.rr_start = nullptr;
+    }
+    return 0;
+}
+
+// The view's carry space for a launch of a.nwaves ranges.
+static int seg_carry_space(SegView &v, SegArgs &a)
+{
     if (v.carry_waves < a.nwaves) {
         dfree(v.carry); v.carry = nullptr;
         dfree(v.carry_row); v.carry_row = nullptr;
@@ -346,14 +419,21 @@ static int seg_args(SegView &v, const void *const *mats, const uint32_t *order, 
     return 0;
 }
 
+// Waves of one launch of the class's range kernel that are resident at once.
 template <int LPN, int NM, typename FT, bool NT>
-static void seg_launch_kernel(const SegArgs &a, uint32_t *counter, hipStream_t s)
+static uint64_t seg_resident_waves()
 {
     const void *fn = reinterpret_cast<const void *>(&k_mttkrp_seg<LPN, NM, FT, NT>);
     int per_cu = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, kSegWPB * 64, 0) != hipSuccess || per_cu < 1)
         per_cu = 1;
-    uint64_t blocks = (uint64_t)num_cus() * per_cu;
+    return (uint64_t)num_cus() * per_cu * kSegWPB;
+}
+
+template <int LPN, int NM, typename FT, bool NT>
+static void seg_launch_kernel(const SegArgs &a, uint32_t *counter, hipStream_t s)
+{
+    uint64_t blocks = seg_resident_waves<LPN, NM, FT, NT>() / kSegWPB;
     const uint64_t need = (a.nwaves + kSegWPB - 1) / kSegWPB;
     if (blocks > need) blocks = need;
     hipLaunchKernelGGL((k_mttkrp_seg<LPN, NM, FT, NT>), dim3((unsigned)blocks), dim3(kSegWPB * 64), 0, s, a, counter);
@@ -377,6 +457,28 @@ static void seg_launch_main(const SegArgs &a, uint32_t *counter, hipStream_t s, 
 template <typename FT, int I>
 constexpr int seg_lanes() { return (sizeof(FT) == 2 ? 1 : 2) << I; }
 
+template <int LPN, int NM, typename FT>
+static uint64_t seg_resident_main(bool nt)
+{
+    if constexpr (seg_u<LPN, NM, FT>() == 8) {
+        if (nt) return seg_resident_waves<LPN, NM, FT, true>();
+    }
+    return seg_resident_waves<LPN, NM, FT, false>();
+}
+
+// Resident waves of the range kernel that seg_launch_ft runs for a.
+template <typename FT>
+static uint64_t seg_resident_ft(const SegArgs &a, bool nt)
+{
+    const bool n3 = a.nmodes == 3;
+    switch (seg_lane_class(a.glpn, sizeof(FT) == 2 ? SPTOL_BF16 : SPTOL_F32)) {
+    case 0: return n3 ? seg_resident_main<seg_lanes<FT, 0>(), 3, FT>(nt) : seg_resident_main<seg_lanes<FT, 0>(), 4, FT>(nt);
+    case 1: return n3 ? seg_resident_main<seg_lanes<FT, 1>(), 3, FT>(nt) : seg_resident_main<seg_lanes<FT, 1>(), 4, FT>(nt);
+    case 2: return n3 ? seg_resident_main<seg_lanes<FT, 2>(), 3, FT>(nt) : seg_resident_main<seg_lanes<FT, 2>(), 4, FT>(nt);
+    default: return n3 ? seg_resident_main<seg_lanes<FT, 3>(), 3, FT>(nt) : seg_resident_main<seg_lanes<FT, 3>(), 4, FT>(nt);
+    }
+}
+
 template <typename FT>
 static void seg_launch_ft(const SegArgs &a, uint32_t *counter, hipStream_t s, bool nt)
 {
@@ -395,6 +497,25 @@ static void seg_launch_ft(const SegArgs &a, uint32_t *counter, hipStream_t s, bo
         n3 ? seg_launch_main<seg_lanes<FT, 3>(), 3, FT>(a, counter, s, nt) : seg_launch_main<seg_lanes<FT, 3>(), 4, FT>(a, counter, s, nt);
         break;
     }
+}
+
+// SPTOL_SEG_ROWS=0 turns ranges of whole rows off (for measurements).
+static bool seg_whole_rows()
+{
+    static const bool on = [] {
+        const char *e = std::getenv("SPTOL_SEG_ROWS");
+        return e ? std::atoi(e) != 0 : true;
+    }();
+    return on;
+}
+
+// Whether a launch on v takes its ranges of whole rows (built with the view,
+// and fitting in the class's resident waves).
+static bool seg_use_row_ranges(const SegView &v, const SegArgs &a, int dtype, bool nt)
+{
+    if (!v.rr_ok || !v.rr_start) return false;
+    const uint64_t W = dtype == SPTOL_BF16 ? seg_resident_ft<uint16_t>(a, nt) : seg_resident_ft<float>(a, nt);
+    return v.rr_nwaves <= W;
 }
 
 // ---------------------------------------------------------------------------
@@ -444,6 +565,12 @@ int seg_mttkrp_noalloc(const SegView &v, const void *const *mats, const uint32_t
         fbytes += (uint64_t)v.ndims[order[k]] * stride * (dtype == SPTOL_BF16 ? 2u : 4u);
     const bool l2 = fbytes <= ((uint64_t)4 << 20), nt = fbytes > kSegNtFactorBytes;
     SPTOL_TRY(seg_fill_args(v, mats, order, R, stride, dtype, out, seg_chg(v.nnz, 64 / seg_glpn(stride, dtype), l2), a));
+    const bool rows = seg_whole_rows() && seg_use_row_ranges(v, a, dtype, nt);
+    if (rows) {   // the same ranges as seg_mttkrp: results bit for bit alike
+        a.zstart = v.rr_start;
+        a.nwaves = v.rr_nwaves;
+        a.rows_per_wave = v.rr_rows;
+    }
     if (a.nwaves > kNoAllocRanges)
         SPTOL_CHECK(SPTERR_VALUE_ERROR, "MTTKRP no-alloc",
                     "the tensor needs more ranges than the fixed scratch holds (at most 65536)");
@@ -455,6 +582,7 @@ int seg_mttkrp_noalloc(const SegView &v, const void *const *mats, const uint32_t
     if (dtype == SPTOL_BF16) seg_launch_ft<uint16_t>(a, counter, s, nt);
     else seg_launch_ft<float>(a, counter, s, nt);
     SPTOL_HIP(hipGetLastError(), "MTTKRP no-alloc launch");
+    if (rows) return 0;   // no row continues across ranges: nothing to carry
     const uint64_t cblocks = (a.nwaves + kSegCarrySlots - 1) / kSegCarrySlots;
     hipLaunchKernelGGL(k_seg_carry, dim3((unsigned)cblocks), dim3(256), 0, s, a, counter);
     SPTOL_HIP(hipGetLastError(), "MTTKRP no-alloc carry launch");
@@ -470,12 +598,22 @@ int seg_mttkrp(SegView &v, const void *const *mats, const uint32_t *order, uint3
     for (uint32_t k = 1; k < v.nmodes; ++k)
         fbytes += (uint64_t)v.ndims[order[k]] * stride * (dtype == SPTOL_BF16 ? 2u : 4u);
     const bool l2 = fbytes <= ((uint64_t)4 << 20), nt = fbytes > kSegNtFactorBytes;
-    SPTOL_TRY(seg_args(v, mats, order, R, stride, dtype, out, seg_chg(v.nnz, 64 / seg_glpn(stride, dtype), l2), a));
+    SPTOL_TRY(seg_fill_args(v, mats, order, R, stride, dtype, out, seg_chg(v.nnz, 64 / seg_glpn(stride, dtype), l2), a));
+    // ranges of whole rows where rows are short and even: no carries, and
+    // every range is taken by wave index (so the work counter stays 0)
+    const bool rows = seg_whole_rows() && seg_use_row_ranges(v, a, dtype, nt);
+    if (rows) {
+        a.zstart = v.rr_start;
+        a.nwaves = v.rr_nwaves;
+        a.rows_per_wave = v.rr_rows;
+    }
+    SPTOL_TRY(seg_carry_space(v, a));
     int tok = prof_start("mttkrp_seg", s);
     if (dtype == SPTOL_BF16) seg_launch_ft<uint16_t>(a, v.counter, s, nt);
     else seg_launch_ft<float>(a, v.counter, s, nt);
     prof_stop(tok, s);
     SPTOL_HIP(hipGetLastError(), "Seg MTTKRP launch");
+    if (rows) return 0;   // no row continues across ranges: nothing to carry
     // always launched: it also resets the work counter
     int tok2 = prof_start("mttkrp_seg_carry", s);
     const uint64_t cblocks = (a.nwaves + kSegCarrySlots - 1) / kSegCarrySlots;

@@ -33,6 +33,11 @@
 //    (rows_per_wave rows), so every output row is written exactly once.
 // seg_carry_slots then adds each run of carries, summed in a fixed order, to
 // the row's stored value.  The result is deterministic.
+//
+// When rows are short and of even length (a uniform tensor), ranges may
+// instead be runs of whole rows (SegArgs::zstart, the first nonzero of each
+// range): no row then continues across ranges, so no carry exists and the
+// carry pass is not launched.
 #pragma once
 
 #include "dev_common.hpp"
@@ -54,6 +59,7 @@ struct SegArgs {
     uint32_t *carry_row;     // nwaves
     uint32_t nmodes;
     uint32_t glpn;           // lanes per group of the range kernel (lpn: of the carry kernel, 4 columns a lane)
+    const uint64_t *zstart;  // null: range w is nonzeros [w*G*chg, (w+1)*G*chg); else [zstart[w], zstart[w+1])
 };
 
 // SegArgs of one MTTKRP on view v with factors of type dtype (carry space
@@ -236,8 +242,17 @@ __device__ __forceinline__ void seg_range(const SegArgs &a, uint64_t wave, SegSh
     const uint4 *__restrict__ rec = a.rec;
     float *__restrict__ out = a.out;
 
-    const uint64_t zs = (wave * G + g) * (uint64_t)a.chg;
-    const uint64_t ze = zs < a.nnz ? min<uint64_t>(zs + a.chg, a.nnz) : zs;
+    uint64_t zs, ze;
+    if (a.zstart) {
+        // a range of whole rows, split evenly over the groups
+        const uint64_t rb = a.zstart[wave], re = a.zstart[wave + 1];
+        const uint64_t cg = (re - rb + G - 1) / G;
+        zs = min<uint64_t>(rb + g * cg, re);
+        ze = min<uint64_t>(zs + cg, re);
+    } else {
+        zs = (wave * G + g) * (uint64_t)a.chg;
+        ze = zs < a.nnz ? min<uint64_t>(zs + a.chg, a.nnz) : zs;
+    }
     const bool nonempty = zs < ze;
     // A group's "piece" (what it adds to a row begun before it) goes to its
     // lanes' LDS slots as soon as it is known, instead of being held in

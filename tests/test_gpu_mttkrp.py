@@ -13,7 +13,7 @@ import pytest
 
 import oracle
 from conftest import rel_fro
-from gpu_util import bf16_round, dev_coo, dev_mttkrp_coo, dev_mttkrp_hicoo
+from gpu_util import DevFactors, DevOut, bf16_round, dev_coo, dev_mttkrp_coo, dev_mttkrp_hicoo
 from sptol_amd import gen
 from sptol_amd.parti import (CooTensor, Matrix, SPTOL_BF16, check, mats_array, ptr, sptIndex, to_hicoo)
 
@@ -160,6 +160,53 @@ def test_c1_full_size(gpu_lib):
         e32, e64 = _check(got, ref32, ref64, 16)
         _say(f"  rel err vs FP32 reference {e32:.2e}, vs FP64 oracle {e64:.2e}; "
              f"reference FP32 vs FP64 {rel_fro(ref32, ref64):.2e}")
+
+
+def _launches(lib, name):
+    ms, n = C.c_double(0), C.c_uint64(0)
+    lib.sptolProfileQuery(name.encode(), C.byref(ms), C.byref(n))
+    return int(n.value)
+
+
+@pytest.mark.parametrize("R", [16, 32])
+def test_whole_row_ranges_have_no_carry(gpu_lib, R):
+    """A uniform tensor (rows of nearly equal length) takes ranges of whole
+    rows: no row continues across ranges, so no carry pass is launched; the
+    result meets the bar in FP32 and BF16, is deterministic, and the
+    allocation-free entry point gives the same bits.  A Zipf tensor (a few
+    very long rows) keeps fixed-size ranges and its carry pass."""
+    lib = gpu_lib
+    cases = [("uniform", gen.uniform_tensor([2000, 1500, 1000], 1_000_000, seed=2024), 0),
+             ("zipf", gen.zipf_tensor([3000, 2000, 1000], 400_000, 1.1, seed=5), 1)]
+    for name, (d, inds, vals), carries in cases:
+        A = gen.uniform_factors(d, R, seed=2025)
+        h = dev_coo(lib, d, inds, vals)
+        check(lib, lib.sptolDevCooPrepareMode(h, 0), "PrepareMode")
+        order = np.array([0, 1, 2], np.uint32)
+        for dtype in (0, SPTOL_BF16):
+            _say("whole-row ranges", name, R, dtype)
+            Au = [bf16_round(a) for a in A] if dtype else A
+            lib.sptolProfileEnable(1)
+            lib.sptolProfileReset()
+            try:
+                got = dev_mttkrp_coo(lib, d, inds, vals, A, order, 0, dtype=dtype, h=h)
+                assert _launches(lib, "mttkrp_seg") == 1
+                assert _launches(lib, "mttkrp_seg_carry") == carries, name
+            finally:
+                lib.sptolProfileEnable(0)
+            again = dev_mttkrp_coo(lib, d, inds, vals, A, order, 0, dtype=dtype, h=h)
+            assert np.array_equal(got.view(np.uint32), again.view(np.uint32))
+            F = DevFactors(lib, A, dtype)
+            out = DevOut(lib, int(d[0]), R)
+            check(lib, lib.sptolDevMTTKRPNoAlloc(h, F.arr, ptr(order, sptIndex), 0, C.byref(out.d), None), "NoAlloc")
+            check(lib, lib.sptolHipStreamSynchronize(None), "sync")
+            noalloc = out.fetch()
+            F.free()
+            out.free()
+            assert np.array_equal(got.view(np.uint32), noalloc.view(np.uint32))
+            _check(got, oracle.mttkrp_coo(inds, vals, Au, order, 0),
+                   oracle.mttkrp_coo(inds, vals, Au, order, 0, dtype=np.float64), R)
+        lib.sptolDevCooFree(h)
 
 
 def test_long_ranges_small_factors(gpu_lib):

@@ -235,13 +235,88 @@ __device__ __forceinline__ void chol_wave_sync()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Column c of V^{-1} from V = L L^T (L row-major in LDS, leading dimension
+// RM + 1): L y = e_c, then L^T x = y, in FP64; fac[a * R + c] = x[a].
+template <int RM>
+__device__ __forceinline__ void inv_column_chol(const double *__restrict__ L, uint32_t R, uint32_t c,
+                                                double *__restrict__ fac)
+{
+#pragma clang fp contract(off)
+    constexpr uint32_t LD = RM + 1;
+    double y[RM];
+#pragma unroll
+    for (int a = 0; a < RM; ++a) {
+        y[a] = 0.0;
+        if (a < (int)R) {
+            double s = a == (int)c ? 1.0 : 0.0;
+#pragma unroll
+            for (int k = 0; k < a; ++k) s -= L[a * LD + k] * y[k];
+            y[a] = s / L[a * LD + a];
+        }
+    }
+#pragma unroll
+    for (int a = RM - 1; a >= 0; --a) {
+        if (a < (int)R) {
+            double s = y[a];
+#pragma unroll
+            for (int k = a + 1; k < RM; ++k)
+                if (k < (int)R) s -= L[k * LD + a] * y[k];
+            y[a] = s / L[a * LD + a];
+        }
+    }
+#pragma unroll
+    for (int a = 0; a < RM; ++a)
+        if (a < (int)R) fac[a * R + c] = y[a];
+}
+
+// Column c of V^{-1} from P V = L U (unit L and U in LDS, leading dimension
+// RM + 1; pv: the row swaps in order): L U x = P e_c, in FP64.
+template <int RM>
+__device__ __forceinline__ void inv_column_lu(const double *__restrict__ LU, const int *__restrict__ pv, uint32_t R,
+                                              uint32_t c, double *__restrict__ fac)
+{
+#pragma clang fp contract(off)
+    constexpr uint32_t LD = RM + 1;
+    uint32_t pos = c;   // where the swaps move the 1 of e_c
+    for (uint32_t j = 0; j < R; ++j) {
+        const uint32_t p = (uint32_t)pv[j];
+        if (pos == j) pos = p;
+        else if (pos == p) pos = j;
+    }
+    double y[RM];
+#pragma unroll
+    for (int a = 0; a < RM; ++a) {
+        y[a] = 0.0;
+        if (a < (int)R) {
+            double s = a == (int)pos ? 1.0 : 0.0;
+#pragma unroll
+            for (int k = 0; k < a; ++k) s -= LU[a * LD + k] * y[k];
+            y[a] = s;
+        }
+    }
+#pragma unroll
+    for (int a = RM - 1; a >= 0; --a) {
+        if (a < (int)R) {
+            double s = y[a];
+#pragma unroll
+            for (int k = a + 1; k < RM; ++k)
+                if (k < (int)R) s -= LU[a * LD + k] * y[k];
+            y[a] = s / LU[a * LD + a];
+        }
+    }
+#pragma unroll
+    for (int a = 0; a < RM; ++a)
+        if (a < (int)R) fac[a * R + c] = y[a];
+}
+
 template <int RM>
 __device__ __forceinline__ void factor_body(const double *__restrict__ ata, uint32_t N, uint32_t m, uint32_t R,
                                             double *__restrict__ fac, int *__restrict__ piv, int *__restrict__ kind)
 {
 #pragma clang fp contract(off)
     __shared__ double col[RM];
-    __shared__ double V[RM * (RM + 1)];   // LU fallback, rows padded against bank conflicts
+    __shared__ double V[RM * (RM + 1)];   // L, or the LU fallback; rows padded against bank conflicts
+    __shared__ int pv[RM];
     const uint32_t RR = R * R, t = threadIdx.x;
     const bool mine = t < R;
     // row t of the Hadamard of the other modes' Grams, multiplied in mode
@@ -283,18 +358,21 @@ __device__ __forceinline__ void factor_body(const double *__restrict__ ata, uint
             }
         }
     }
+    constexpr uint32_t LD = RM + 1;
     if (spd) {
+        // L to LDS, then column t of V^{-1} = L^{-T} L^{-1}
         if (mine) {
 #pragma unroll
             for (int k = 0; k < RM; ++k)
-                if (k < (int)R) fac[t * R + k] = k <= (int)t ? row[k] : 0.0;
+                if (k < (int)R) V[t * LD + k] = k <= (int)t ? row[k] : 0.0;
         }
+        chol_wave_sync();
+        if (mine) inv_column_chol<RM>(V, R, t, fac);
         if (t == 0) kind[0] = 0;
         return;
     }
     // LU with partial pivoting of V (recomputed; in place): at step j swap row
     // j with the pivot row, then the lanes eliminate the rows below j.
-    const uint32_t LD = R + 1;
     for (uint32_t e = t; e < RR; e += 64) {
         const uint32_t i = e / R, k = e % R;
         double v = 1.0;
@@ -313,7 +391,10 @@ __device__ __forceinline__ void factor_body(const double *__restrict__ ata, uint
             if (t == 0) { kind[0] = 2; kind[1] += 1; }
             return;
         }
-        if (t == 0) piv[j] = (int)best;
+        if (t == 0) {
+            piv[j] = (int)best;
+            pv[j] = (int)best;
+        }
         if (best != j)
             for (uint32_t k = t; k < R; k += 64) {
                 const double tmp = V[j * LD + k];
@@ -328,7 +409,7 @@ __device__ __forceinline__ void factor_body(const double *__restrict__ ata, uint
         }
         chol_wave_sync();
     }
-    for (uint32_t e = t; e < RR; e += 64) fac[e] = V[(e / R) * LD + e % R];
+    if (mine) inv_column_lu<RM>(V, pv, R, t, fac);
     if (t == 0) kind[0] = 1;
 }
 
@@ -339,92 +420,45 @@ __global__ __launch_bounds__(64) void k_factor(const double *__restrict__ ata, u
     factor_body<RM>(ata, N, m, R, fac, piv, kind);
 }
 
-// A[i] = M[i] V^{-1} for every row, one thread per row, FP64 arithmetic.
+// A[i] = M[i] V^{-1} for every row: V^{-1} (FP64, from k_factor) times each
+// row of M, one thread per (row, column), a block holding 256 / RM rows;
+// every product and sum in FP64 in a fixed column order.  When V was
+// exactly singular (kind 2) the rows stay the MTTKRP's, as gesv leaves its
+// right-hand sides.
 template <int RM>
 __device__ __forceinline__ void solve_rows_body(const float *__restrict__ M, float *__restrict__ A, uint32_t I,
                                                 uint32_t R, uint32_t S, const double *__restrict__ fac,
-                                                const int *__restrict__ piv, const int *__restrict__ kind, uint32_t blk)
+                                                const int *__restrict__ kind, uint32_t blk)
 {
-    __shared__ double F[RM * RM];
-    __shared__ int pv[RM];
-    for (uint32_t e = threadIdx.x; e < R * R; e += blockDim.x) F[(e / R) * RM + e % R] = fac[e];
-    for (uint32_t e = threadIdx.x; e < R; e += blockDim.x) pv[e] = piv[e];
+#pragma clang fp contract(off)
+    constexpr uint32_t RPB = 256 / RM;
+    __shared__ double FT[RM * RM];   // FT[k * RM + c] = V^{-1}[c][k]
+    __shared__ float Ms[RPB * RM];
+    const int kd = *kind;
+    const uint32_t t = threadIdx.x, r = t / RM, c = t % RM;
+    const uint64_t i = (uint64_t)blk * RPB + r;
+    if (kd != 2)
+        for (uint32_t e = t; e < R * R; e += blockDim.x) FT[(e % R) * RM + e / R] = fac[e];
+    Ms[t] = (i < I && c < R) ? M[i * S + c] : 0.f;
     __syncthreads();
-    const int lu = *kind;
-    const uint64_t i = (uint64_t)blk * blockDim.x + threadIdx.x;
-    if (i >= I) return;
-    double x[RM];
-#pragma unroll
-    for (int a = 0; a < RM; ++a) x[a] = a < (int)R ? (double)M[i * S + a] : 0.0;
-    if (lu == 2) {
-        // singular normal equations: the row stays the MTTKRP's
-    } else if (!lu) {
-        // L y = x, then L^T z = y
-#pragma unroll
-        for (int a = 0; a < RM; ++a) {
-            if (a < (int)R) {
-                double s = x[a];
-#pragma unroll
-                for (int k = 0; k < a; ++k) s -= F[a * RM + k] * x[k];
-                x[a] = s / F[a * RM + a];
-            }
-        }
-#pragma unroll
-        for (int a = RM - 1; a >= 0; --a) {
-            if (a < (int)R) {
-                double s = x[a];
-#pragma unroll
-                for (int k = a + 1; k < RM; ++k)
-                    if (k < (int)R) s -= F[k * RM + a] * x[k];
-                x[a] = s / F[a * RM + a];
-            }
-        }
-    } else {
-        // V^T = V: solve V z = x with P V = L U
-#pragma unroll
-        for (int a = 0; a < RM; ++a) {
-            if (a < (int)R) {
-                const int p = pv[a];
-                // swap x[a], x[p] (p >= a); done with a select loop to keep x in registers
-                double xa = x[a], xp = 0.0;
-#pragma unroll
-                for (int k = 0; k < RM; ++k) if (k == p) xp = x[k];
-#pragma unroll
-                for (int k = 0; k < RM; ++k) if (k == p) x[k] = xa;
-                x[a] = xp;
-            }
-        }
-#pragma unroll
-        for (int a = 0; a < RM; ++a) {
-            if (a < (int)R) {
-                double s = x[a];
-#pragma unroll
-                for (int k = 0; k < a; ++k) s -= F[a * RM + k] * x[k];
-                x[a] = s;
-            }
-        }
-#pragma unroll
-        for (int a = RM - 1; a >= 0; --a) {
-            if (a < (int)R) {
-                double s = x[a];
-#pragma unroll
-                for (int k = a + 1; k < RM; ++k)
-                    if (k < (int)R) s -= F[a * RM + k] * x[k];
-                x[a] = s / F[a * RM + a];
-            }
-        }
+    if (i >= I || c >= R) return;
+    if (kd == 2) {
+        A[i * S + c] = Ms[t];
+        return;
     }
+    double x = 0.0;
 #pragma unroll
-    for (int a = 0; a < RM; ++a)
-        if (a < (int)R) A[i * S + a] = (float)x[a];
+    for (int k = 0; k < RM; ++k)
+        if (k < (int)R) x += FT[k * RM + c] * (double)Ms[r * RM + k];
+    A[i * S + c] = (float)x;
 }
 
 template <int RM>
 __global__ __launch_bounds__(256) void k_solve_rows(const float *__restrict__ M, float *__restrict__ A, uint32_t I,
                                                     uint32_t R, uint32_t S, const double *__restrict__ fac,
-                                                    const int *__restrict__ piv, const int *__restrict__ kind)
+                                                    const int *__restrict__ kind)
 {
-    solve_rows_body<RM>(M, A, I, R, S, fac, piv, kind, blockIdx.x);
+    solve_rows_body<RM>(M, A, I, R, S, fac, kind, blockIdx.x);
 }
 
 // partial[blk][r] = sum over this block's rows of A[i][r] * M[i][r].
@@ -733,7 +767,13 @@ static unsigned grid_rows(uint64_t n, uint32_t per_block)
 }
 
 // Row solves: one thread per row, 256 rows per block.
-static unsigned solve_blocks(uint32_t I) { return (unsigned)(((uint64_t)I + 255) / 256); }
+// Blocks of the solve over I rows at rank R: 256 / RM rows a block, RM the
+// rank class (8, 16, 32 or 64).
+static unsigned solve_blocks(uint32_t I, uint32_t R)
+{
+    const uint64_t rm = R <= 8 ? 8 : (R <= 16 ? 16 : (R <= 32 ? 32 : 64)), rpb = 256 / rm;
+    return (unsigned)(((uint64_t)I + rpb - 1) / rpb);
+}
 
 // ---------------------------------------------------------------------------
 // Batched post-MTTKRP step: one grid per phase over many requests.  A block
@@ -783,7 +823,7 @@ template <int RM>
 __global__ __launch_bounds__(256) void k_solve_b(const AlsStepDesc *__restrict__ d, uint32_t n)
 {
     const AlsStepDesc &q = als_find(d, n, kPhSolve, blockIdx.x);
-    solve_rows_body<RM>(q.M, q.A, q.I, q.R, q.S, q.fac, q.piv, q.kind, blockIdx.x - q.blk[kPhSolve]);
+    solve_rows_body<RM>(q.M, q.A, q.I, q.R, q.S, q.fac, q.kind, blockIdx.x - q.blk[kPhSolve]);
 }
 
 __global__ __launch_bounds__(256) void k_col_partial_b(const AlsStepDesc *__restrict__ d, uint32_t n)
@@ -876,11 +916,11 @@ static int solve(const float *M, float *A, uint32_t I, uint32_t R, uint32_t S, C
     else if (R <= 16) hipLaunchKernelGGL(k_factor<16>, dim3(1), dim3(64), 0, s, w.ata, N, m, R, w.fac, w.piv, w.kind);
     else if (R <= 32) hipLaunchKernelGGL(k_factor<32>, dim3(1), dim3(64), 0, s, w.ata, N, m, R, w.fac, w.piv, w.kind);
     else hipLaunchKernelGGL(k_factor<64>, dim3(1), dim3(64), 0, s, w.ata, N, m, R, w.fac, w.piv, w.kind);
-    const unsigned g = solve_blocks(I);
-    if (R <= 8) hipLaunchKernelGGL(k_solve_rows<8>, dim3(g), dim3(256), 0, s, M, A, I, R, S, w.fac, w.piv, w.kind);
-    else if (R <= 16) hipLaunchKernelGGL(k_solve_rows<16>, dim3(g), dim3(256), 0, s, M, A, I, R, S, w.fac, w.piv, w.kind);
-    else if (R <= 32) hipLaunchKernelGGL(k_solve_rows<32>, dim3(g), dim3(256), 0, s, M, A, I, R, S, w.fac, w.piv, w.kind);
-    else hipLaunchKernelGGL(k_solve_rows<64>, dim3(g), dim3(256), 0, s, M, A, I, R, S, w.fac, w.piv, w.kind);
+    const unsigned g = solve_blocks(I, R);
+    if (R <= 8) hipLaunchKernelGGL(k_solve_rows<8>, dim3(g), dim3(256), 0, s, M, A, I, R, S, w.fac, w.kind);
+    else if (R <= 16) hipLaunchKernelGGL(k_solve_rows<16>, dim3(g), dim3(256), 0, s, M, A, I, R, S, w.fac, w.kind);
+    else if (R <= 32) hipLaunchKernelGGL(k_solve_rows<32>, dim3(g), dim3(256), 0, s, M, A, I, R, S, w.fac, w.kind);
+    else hipLaunchKernelGGL(k_solve_rows<64>, dim3(g), dim3(256), 0, s, M, A, I, R, S, w.fac, w.kind);
     prof_stop(tok, s);
     SPTOL_HIP(hipGetLastError(), "CPD solve");
     return 0;
@@ -921,7 +961,7 @@ int als_step_batch(const std::vector<CpdJob *> &jobs, AlsBatchBuffers &bb, hipSt
             q.nb_gram = red_blocks(I, 32);
             q.nb_inner = q.end_iter ? red_blocks(I, 256) : 0;
             q.blk[kPhSolve] = sb;
-            sb += solve_blocks(I);
+            sb += solve_blocks(I, j->R);
             d.push_back(q);
         }
     }
@@ -962,7 +1002,7 @@ int als_step_batch(const std::vector<CpdJob *> &jobs, AlsBatchBuffers &bb, hipSt
         const uint32_t k = c0[c], nc = c0[c + 1] - c0[c];
         if (!nc) continue;
         const AlsStepDesc &last = d[c0[c + 1] - 1];
-        const unsigned g = last.blk[kPhSolve] + solve_blocks(last.I);
+        const unsigned g = last.blk[kPhSolve] + solve_blocks(last.I, last.R);
         const AlsStepDesc *dc = bb.dev + k;
         switch (c) {
         case 0: hipLaunchKernelGGL(k_solve_b<8>, dim3(g), dim3(256), 0, s, dc, nc); break;

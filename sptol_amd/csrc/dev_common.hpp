@@ -138,7 +138,7 @@ struct SegView {
 int seg_build(SegView &v, uint32_t nmodes, uint32_t mode, uint32_t nrows, uint64_t nnz,
               uint32_t *const *inds, const float *vals, bool copy_vals, hipStream_t s, const uint32_t *ndims,
               const uint32_t *perm = nullptr, const uint32_t *sorted_rows = nullptr, const uint4 *rec0 = nullptr,
-              const uint16_t *sorted_rows16 = nullptr);
+              const uint16_t *sorted_rows16 = nullptr, bool row_ranges = true);
 // Records in the tensor's own order into rec0 and, for each mode m with
 // keys16[m] non-null (keys16 itself may be null), the mode's indices as
 // 16-bit sort keys.
@@ -185,6 +185,9 @@ struct sptolDevCoo_s {
     // mode m's indices are already non-decreasing in the tensor's own order
     // (found by the upload's check pass): its view needs no sort
     bool in_order[sptol::kMaxModes] = {};
+    // views serve batched launches only (the server): no table of ranges of
+    // whole rows, which only single-tensor launches use
+    bool batch_only = false;
 };
 
 struct sptolHicooViews;

@@ -692,7 +692,8 @@ static int coo_prepare_from(sptolDevCoo h, uint32_t mode, hipStream_t s, const u
     if ((h->nmodes == 3 || h->nmodes == 4) && h->in_order[mode]) {
         // already in mode order: the records are packed in the tensor's own
         // order, which a stable sort would leave unchanged
-        SPTOL_TRY(seg_build(sc.seg, h->nmodes, mode, h->ndims[mode], n, h->inds, h->vals, true, s, h->ndims));
+        SPTOL_TRY(seg_build(sc.seg, h->nmodes, mode, h->ndims[mode], n, h->inds, h->vals, true, s, h->ndims, nullptr,
+                            nullptr, nullptr, nullptr, !h->batch_only));
         sc.ready = true;
         return 0;
     }
@@ -724,7 +725,7 @@ static int coo_prepare_from(sptolDevCoo h, uint32_t mode, hipStream_t s, const u
         // through the permutation; no sorted SoA copy is kept (the atomic
         // fallback kernel reads the tensor in its own order)
         r = seg_build(sc.seg, h->nmodes, mode, h->ndims[mode], n, h->inds, h->vals, true, s, h->ndims, perm2,
-                      k16 ? nullptr : keys2, rec0, k16 ? keys2_16 : nullptr);
+                      k16 ? nullptr : keys2, rec0, k16 ? keys2_16 : nullptr, !h->batch_only);
         cleanup();
         if (r) return r;
         sc.ready = true;

@@ -216,7 +216,7 @@ static int seg_row_ranges(SegView &v, hipStream_t s);
 
 int seg_build(SegView &v, uint32_t nmodes, uint32_t mode, uint32_t nrows, uint64_t nnz, uint32_t *const *inds,
               const float *vals, bool copy_vals, hipStream_t s, const uint32_t *ndims, const uint32_t *perm,
-              const uint32_t *sorted_rows, const uint4 *rec0, const uint16_t *sorted_rows16)
+              const uint32_t *sorted_rows, const uint4 *rec0, const uint16_t *sorted_rows16, bool row_ranges)
 {
     v.release();
     if (perm && ((!sorted_rows && !sorted_rows16) || !copy_vals))
@@ -268,7 +268,7 @@ int seg_build(SegView &v, uint32_t nmodes, uint32_t mode, uint32_t nrows, uint64
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     if (e != hipSuccess) { v.release(); SPTOL_HIP(e, "Seg view"); }
     v.ready = true;
-    return seg_row_ranges(v, s);
+    return row_ranges ? seg_row_ranges(v, s) : 0;
 }
 
 // Nonzeros per lane group for a tensor of nnz nonzeros: about 32 ranges per

@@ -122,15 +122,26 @@ static size_t estimate_bytes(const Request &r)
     return b;
 }
 
-// Seeded U[0,1) factor initialisation local to a request (splitmix64).
-static void seeded_uniform(uint64_t &state, float *dst, size_t n)
+// Seeded U[0,1) factor initialisation local to a request (splitmix64): the
+// request's stream of draws fills the rows of every mode in mode order, R
+// draws a row, so entry (i, c < R) of mode m is draw n = off_m + i * R + c
+// with state seed + (n + 1) * golden; padding columns are zero.  Each entry
+// is computed from its own index, on the device.
+__global__ void k_seeded_uniform(float *__restrict__ A, uint32_t I, uint32_t R, uint32_t S, uint64_t seed,
+                                 uint64_t off)
 {
-    for (size_t i = 0; i < n; ++i) {
-        uint64_t z = (state += 0x9E3779B97F4A7C15ull);
-        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
-        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
-        z ^= z >> 31;
-        dst[i] = (float)(z >> 40) * (1.0f / 16777216.0f);
+    const uint64_t n = (uint64_t)I * S;
+    for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t i = e / S, c = e % S;
+        float v = 0.0f;
+        if (c < R) {
+            uint64_t z = seed + (off + i * R + c + 1) * 0x9E3779B97F4A7C15ull;
+            z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+            z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+            z ^= z >> 31;
+            v = (float)(z >> 40) * (1.0f / 16777216.0f);
+        }
+        A[e] = v;
     }
 }
 
@@ -212,15 +223,18 @@ static int prepare_loaded(sptolServer srv, Request *r, hipStream_t s)
 {
     // mode views need device sorts (at most 2^31 - 1 items); a larger tensor
     // runs the unsorted (atomic) batched MTTKRP
+    r->X->batch_only = true;
     if (r->nnz <= kMaxSortItems) SPTOL_TRY(coo_prepare_all(r->X, s));
-    uint64_t st = r->seed;
+    uint64_t off = 0;
     for (uint32_t m = 0; m < r->nmodes; ++m) {
         SPTOL_TRY(sptolDevMatrixCreate(&r->mats[m], r->ndims[m], r->R, SPTOL_F32));
         const uint32_t S = r->mats[m].stride;
-        std::vector<float> h((size_t)r->ndims[m] * S, 0.0f);
-        for (uint32_t i = 0; i < r->ndims[m]; ++i) seeded_uniform(st, h.data() + (size_t)i * S, r->R);
-        SPTOL_HIP(hipMemcpyAsync(r->mats[m].values, h.data(), h.size() * 4, hipMemcpyHostToDevice, s), "Serve upload");
-        SPTOL_HIP(hipStreamSynchronize(s), "Serve upload");
+        const uint64_t n = (uint64_t)r->ndims[m] * S;
+        const unsigned grid = (unsigned)std::min<uint64_t>((n + 255) / 256, 4096);
+        hipLaunchKernelGGL(k_seeded_uniform, dim3(grid ? grid : 1), dim3(256), 0, s, (float *)r->mats[m].values,
+                           r->ndims[m], r->R, S, r->seed, off);
+        SPTOL_HIP(hipGetLastError(), "Serve init");
+        off += (uint64_t)r->ndims[m] * r->R;
     }
     r->job = new CpdJob();
     r->job->exact_fit = srv->cfg.exact_fit != 0;

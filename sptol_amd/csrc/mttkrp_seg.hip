@@ -62,22 +62,41 @@ constexpr int seg_waves() { return seg_u<LPN, NM, FT>() == 8 ? 3 : (NM == 3 ? SP
 // the 8-record classes then load records and store outputs non-temporally.
 constexpr uint64_t kSegNtFactorBytes = (uint64_t)32 << 20;
 
-template <int LPN, int NM, typename FT, bool NT = false>
-__global__ __launch_bounds__(kSegWPB * 64) __attribute__((amdgpu_waves_per_eu(seg_waves<LPN, NM, FT>(), 8)))
+// Waves per block: 4, or 16 when a factor is copied to LDS (one copy per
+// block, one block per compute unit).
+template <int LK>
+constexpr int seg_wpb() { return LK ? 16 : kSegWPB; }
+// Classes whose 16 waves' staging fits in LDS beside the factor copy (all
+// 4-mode classes but BF16 with one lane a group).
+template <int LPN, int NM, typename FT>
+constexpr bool seg_lds_class() { return NM == 4 && !(sizeof(FT) == 2 && LPN == 1); }
+
+template <int LPN, int NM, typename FT, bool NT = false, int LK = 0>
+__global__ __launch_bounds__(seg_wpb<LK>() * 64) __attribute__((amdgpu_waves_per_eu(seg_waves<LPN, NM, FT>(), 8)))
 void k_mttkrp_seg(const SegArgs a, uint32_t *__restrict__ counter)
 {
     constexpr int U = seg_u<LPN, NM, FT>();
-    __shared__ SegSharedOf<LPN, NM, FT, kSegWPB, U> sh;
+    constexpr int WPB = seg_wpb<LK>();
+    __shared__ SegSharedOf<LPN, NM, FT, WPB, U> sh;
+    __shared__ __align__(16) FT sfac[LK ? kSegLdsFactorBytes / sizeof(FT) : 1];
+    if constexpr (LK != 0) {
+        // the block's copy of factor A_oLK (rows x S values, a multiple of 16 bytes)
+        const FT *src = (const FT *)(LK == 1 ? a.A1 : (LK == 2 ? a.A2 : a.A3));
+        const uint32_t n16 = (uint32_t)((uint64_t)a.lrows * a.S * sizeof(FT) / 16);
+        for (uint32_t i = threadIdx.x; i < n16; i += WPB * 64)
+            reinterpret_cast<uint4 *>(sfac)[i] = reinterpret_cast<const uint4 *>(src)[i];
+        __syncthreads();
+    }
     // the wave index is uniform across the wave: read from lane 0, the
     // range's indices, bounds and arguments live in scalar registers
     const uint32_t lane = threadIdx.x & 63u, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     // Wave w of the W in the grid takes range w first, then pulls ranges
     // W, W+1, ... from the counter: no atomics while every wave starts, none
     // at all in a launch of at most W ranges, balancing after that.
-    const uint64_t W = (uint64_t)gridDim.x * kSegWPB;
-    uint64_t wave = (uint64_t)blockIdx.x * kSegWPB + wv;
+    const uint64_t W = (uint64_t)gridDim.x * WPB;
+    uint64_t wave = (uint64_t)blockIdx.x * WPB + wv;
     while (wave < a.nwaves) {
-        seg_range<LPN, NM, FT, kSegWPB, U, NT>(a, wave, sh, wv, lane);
+        seg_range<LPN, NM, FT, WPB, U, NT, LK>(a, wave, sh, wv, lane, sfac);
         if (W >= a.nwaves) break;   // every range was taken by index
         uint32_t wq = 0;
         if (lane == 0) wq = atomicAdd(counter, 1u);
@@ -420,23 +439,26 @@ static int seg_carry_space(SegView &v, SegArgs &a)
 }
 
 // Waves of one launch of the class's range kernel that are resident at once.
-template <int LPN, int NM, typename FT, bool NT>
+template <int LPN, int NM, typename FT, bool NT, int LK = 0>
 static uint64_t seg_resident_waves()
 {
-    const void *fn = reinterpret_cast<const void *>(&k_mttkrp_seg<LPN, NM, FT, NT>);
+    constexpr int WPB = seg_wpb<LK>();
+    const void *fn = reinterpret_cast<const void *>(&k_mttkrp_seg<LPN, NM, FT, NT, LK>);
     int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, kSegWPB * 64, 0) != hipSuccess || per_cu < 1)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, WPB * 64, 0) != hipSuccess || per_cu < 1)
         per_cu = 1;
-    return (uint64_t)num_cus() * per_cu * kSegWPB;
+    return (uint64_t)num_cus() * per_cu * WPB;
 }
 
-template <int LPN, int NM, typename FT, bool NT>
+template <int LPN, int NM, typename FT, bool NT, int LK = 0>
 static void seg_launch_kernel(const SegArgs &a, uint32_t *counter, hipStream_t s)
 {
-    uint64_t blocks = seg_resident_waves<LPN, NM, FT, NT>() / kSegWPB;
-    const uint64_t need = (a.nwaves + kSegWPB - 1) / kSegWPB;
+    constexpr int WPB = seg_wpb<LK>();
+    uint64_t blocks = seg_resident_waves<LPN, NM, FT, NT, LK>() / WPB;
+    const uint64_t need = (a.nwaves + WPB - 1) / WPB;
     if (blocks > need) blocks = need;
-    hipLaunchKernelGGL((k_mttkrp_seg<LPN, NM, FT, NT>), dim3((unsigned)blocks), dim3(kSegWPB * 64), 0, s, a, counter);
+    hipLaunchKernelGGL((k_mttkrp_seg<LPN, NM, FT, NT, LK>), dim3((unsigned)blocks), dim3(WPB * 64), 0, s, a,
+                       counter);
 }
 
 // nt: the gathered factors live in HBM (only the 8-record classes have a
@@ -450,6 +472,14 @@ static void seg_launch_main(const SegArgs &a, uint32_t *counter, hipStream_t s, 
             return;
         }
     }
+    if constexpr (seg_lds_class<LPN, NM, FT>()) {
+        switch (a.lk) {
+        case 1: seg_launch_kernel<LPN, NM, FT, false, 1>(a, counter, s); return;
+        case 2: seg_launch_kernel<LPN, NM, FT, false, 2>(a, counter, s); return;
+        case 3: seg_launch_kernel<LPN, NM, FT, false, 3>(a, counter, s); return;
+        default: break;
+        }
+    }
     seg_launch_kernel<LPN, NM, FT, false>(a, counter, s);
 }
 
@@ -458,10 +488,15 @@ template <typename FT, int I>
 constexpr int seg_lanes() { return (sizeof(FT) == 2 ? 1 : 2) << I; }
 
 template <int LPN, int NM, typename FT>
-static uint64_t seg_resident_main(bool nt)
+static uint64_t seg_resident_main(bool nt, uint32_t lk)
 {
     if constexpr (seg_u<LPN, NM, FT>() == 8) {
         if (nt) return seg_resident_waves<LPN, NM, FT, true>();
+    }
+    if constexpr (seg_lds_class<LPN, NM, FT>()) {
+        if (lk == 1) return seg_resident_waves<LPN, NM, FT, false, 1>();
+        if (lk == 2) return seg_resident_waves<LPN, NM, FT, false, 2>();
+        if (lk == 3) return seg_resident_waves<LPN, NM, FT, false, 3>();
     }
     return seg_resident_waves<LPN, NM, FT, false>();
 }
@@ -471,11 +506,12 @@ template <typename FT>
 static uint64_t seg_resident_ft(const SegArgs &a, bool nt)
 {
     const bool n3 = a.nmodes == 3;
+    const uint32_t lk = a.lk;
     switch (seg_lane_class(a.glpn, sizeof(FT) == 2 ? SPTOL_BF16 : SPTOL_F32)) {
-    case 0: return n3 ? seg_resident_main<seg_lanes<FT, 0>(), 3, FT>(nt) : seg_resident_main<seg_lanes<FT, 0>(), 4, FT>(nt);
-    case 1: return n3 ? seg_resident_main<seg_lanes<FT, 1>(), 3, FT>(nt) : seg_resident_main<seg_lanes<FT, 1>(), 4, FT>(nt);
-    case 2: return n3 ? seg_resident_main<seg_lanes<FT, 2>(), 3, FT>(nt) : seg_resident_main<seg_lanes<FT, 2>(), 4, FT>(nt);
-    default: return n3 ? seg_resident_main<seg_lanes<FT, 3>(), 3, FT>(nt) : seg_resident_main<seg_lanes<FT, 3>(), 4, FT>(nt);
+    case 0: return n3 ? seg_resident_main<seg_lanes<FT, 0>(), 3, FT>(nt, 0) : seg_resident_main<seg_lanes<FT, 0>(), 4, FT>(nt, lk);
+    case 1: return n3 ? seg_resident_main<seg_lanes<FT, 1>(), 3, FT>(nt, 0) : seg_resident_main<seg_lanes<FT, 1>(), 4, FT>(nt, lk);
+    case 2: return n3 ? seg_resident_main<seg_lanes<FT, 2>(), 3, FT>(nt, 0) : seg_resident_main<seg_lanes<FT, 2>(), 4, FT>(nt, lk);
+    default: return n3 ? seg_resident_main<seg_lanes<FT, 3>(), 3, FT>(nt, 0) : seg_resident_main<seg_lanes<FT, 3>(), 4, FT>(nt, lk);
     }
 }
 
@@ -496,6 +532,30 @@ static void seg_launch_ft(const SegArgs &a, uint32_t *counter, hipStream_t s, bo
     default:
         n3 ? seg_launch_main<seg_lanes<FT, 3>(), 3, FT>(a, counter, s, nt) : seg_launch_main<seg_lanes<FT, 3>(), 4, FT>(a, counter, s, nt);
         break;
+    }
+}
+
+// 4-mode views of at least 2M nonzeros read their smallest gathered factor
+// from a copy in LDS when it fits kSegLdsFactorBytes (SPTOL_SEG_LDS=0 turns
+// this off, for measurements).
+static void seg_pick_lds(const SegView &v, SegArgs &a, int dtype)
+{
+    static const bool on = [] {
+        const char *e = std::getenv("SPTOL_SEG_LDS");
+        return e ? std::atoi(e) != 0 : true;
+    }();
+    a.lk = 0;
+    a.lrows = 0;
+    if (!on || v.nmodes != 4 || v.nnz < ((uint64_t)2 << 20) || (dtype == SPTOL_BF16 && a.glpn == 1)) return;
+    const uint32_t o[3] = {a.o1, a.o2, a.o3};
+    uint64_t best = ~0ull;
+    for (uint32_t k = 0; k < 3; ++k) {
+        const uint64_t bytes = (uint64_t)v.ndims[o[k]] * a.S * (dtype == SPTOL_BF16 ? 2u : 4u);
+        if (bytes <= kSegLdsFactorBytes && bytes < best) {
+            best = bytes;
+            a.lk = k + 1;
+            a.lrows = v.ndims[o[k]];
+        }
     }
 }
 
@@ -565,6 +625,7 @@ int seg_mttkrp_noalloc(const SegView &v, const void *const *mats, const uint32_t
         fbytes += (uint64_t)v.ndims[order[k]] * stride * (dtype == SPTOL_BF16 ? 2u : 4u);
     const bool l2 = fbytes <= ((uint64_t)4 << 20), nt = fbytes > kSegNtFactorBytes;
     SPTOL_TRY(seg_fill_args(v, mats, order, R, stride, dtype, out, seg_chg(v.nnz, 64 / seg_glpn(stride, dtype), l2), a));
+    seg_pick_lds(v, a, dtype);
     const bool rows = seg_whole_rows() && seg_use_row_ranges(v, a, dtype, nt);
     if (rows) {   // the same ranges as seg_mttkrp: results bit for bit alike
         a.zstart = v.rr_start;
@@ -599,6 +660,7 @@ int seg_mttkrp(SegView &v, const void *const *mats, const uint32_t *order, uint3
         fbytes += (uint64_t)v.ndims[order[k]] * stride * (dtype == SPTOL_BF16 ? 2u : 4u);
     const bool l2 = fbytes <= ((uint64_t)4 << 20), nt = fbytes > kSegNtFactorBytes;
     SPTOL_TRY(seg_fill_args(v, mats, order, R, stride, dtype, out, seg_chg(v.nnz, 64 / seg_glpn(stride, dtype), l2), a));
+    seg_pick_lds(v, a, dtype);
     // ranges of whole rows where rows are short and even: no carries, and
     // every range is taken by wave index (so the work counter stays 0)
     const bool rows = seg_whole_rows() && seg_use_row_ranges(v, a, dtype, nt);

@@ -60,7 +60,12 @@ struct SegArgs {
     uint32_t nmodes;
     uint32_t glpn;           // lanes per group of the range kernel (lpn: of the carry kernel, 4 columns a lane)
     const uint64_t *zstart;  // null: range w is nonzeros [w*G*chg, (w+1)*G*chg); else [zstart[w], zstart[w+1])
+    uint32_t lk;             // 1..3: factor A1/A2/A3 is read from a copy in LDS (lrows rows); 0: none
+    uint32_t lrows;
 };
+
+// Largest factor the 4-mode range kernels copy to LDS (blocks of 16 waves).
+constexpr uint32_t kSegLdsFactorBytes = 64u << 10;
 
 // SegArgs of one MTTKRP on view v with factors of type dtype (carry space
 // left unset; mttkrp_seg.hip).
@@ -220,9 +225,12 @@ struct SegShared {
 template <int LPN, int NM, typename FT, int WPB, int U = kSegU>
 using SegSharedOf = SegShared<LPN, NM, WPB, seg_cols<FT>(), U>;
 
-template <int LPN, int NM, typename FT, int WPB, int U = kSegU, bool NT = false>
+// LK = k (1..3): factor A_ok is read from sfac, its copy in LDS, instead of
+// global memory (a range kernel's gathers are bound by the vector-memory
+// return path; an LDS read of the same shape costs less there).
+template <int LPN, int NM, typename FT, int WPB, int U = kSegU, bool NT = false, int LK = 0>
 __device__ __forceinline__ void seg_range(const SegArgs &a, uint64_t wave, SegSharedOf<LPN, NM, FT, WPB, U> &sh,
-                                          uint32_t wv, uint32_t lane)
+                                          uint32_t wv, uint32_t lane, const FT *__restrict__ sfac = nullptr)
 {
 #pragma clang fp contract(off)
     constexpr int C = seg_cols<FT>();
@@ -235,9 +243,9 @@ __device__ __forceinline__ void seg_range(const SegArgs &a, uint64_t wave, SegSh
     const bool col_ok = c0 < S;                  // S is a multiple of 8: a lane's columns are all in the row
     const uint32_t cc = col_ok ? c0 : 0u;        // idle lanes gather column 0
     const uint32_t ncol = R > c0 ? R - c0 : 0u;  // this lane's columns below R
-    const FT *__restrict__ A1 = (const FT *)a.A1;
-    const FT *__restrict__ A2 = (const FT *)a.A2;
-    const FT *__restrict__ A3 = (const FT *)a.A3;
+    const FT *__restrict__ A1 = LK == 1 ? sfac : (const FT *)a.A1;
+    const FT *__restrict__ A2 = LK == 2 ? sfac : (const FT *)a.A2;
+    const FT *__restrict__ A3 = LK == 3 ? sfac : (const FT *)a.A3;
     const uint32_t mode = a.mode, o1 = a.o1, o2 = a.o2, o3 = a.o3;
     const uint4 *__restrict__ rec = a.rec;
     float *__restrict__ out = a.out;

@@ -209,6 +209,42 @@ def test_whole_row_ranges_have_no_carry(gpu_lib, R):
         lib.sptolDevCooFree(h)
 
 
+@pytest.mark.parametrize("R", [16, 32])
+def test_lds_factor_4d(gpu_lib, R):
+    """A 4-D tensor of 2.5M nonzeros with a 50-row mode: the MTTKRPs of the
+    other modes read that mode's factor from a copy in LDS, at each of the
+    three positions of the Khatri-Rao order (mats order [m, m+1, m+2, m+3]),
+    FP32 and BF16; two runs and the allocation-free entry point agree bit
+    for bit.  The heaviest rows hold about 200k nonzeros, whose sequential
+    FP32 sums are themselves 4e-5 off, so the bar is the FP64 oracle."""
+    lib = gpu_lib
+    d, inds, vals = gen.zipf_tensor([3000, 2000, 1500, 50], 2_500_000, 1.05, seed=61)
+    A = gen.uniform_factors(d, R, seed=62)
+    h = dev_coo(lib, d, inds, vals)
+    for mode in range(4):
+        check(lib, lib.sptolDevCooPrepareMode(h, mode), "PrepareMode")
+        order = np.array([mode] + [(mode + k) % 4 for k in range(1, 4)], np.uint32)
+        for dtype in (0, SPTOL_BF16):
+            _say("lds factor", R, mode, dtype)
+            Au = [bf16_round(a) for a in A] if dtype else A
+            got = dev_mttkrp_coo(lib, d, inds, vals, A, order, mode, dtype=dtype, h=h)
+            again = dev_mttkrp_coo(lib, d, inds, vals, A, order, mode, dtype=dtype, h=h)
+            assert np.array_equal(got.view(np.uint32), again.view(np.uint32))
+            F = DevFactors(lib, A, dtype)
+            out = DevOut(lib, int(d[mode]), R)
+            check(lib, lib.sptolDevMTTKRPNoAlloc(h, F.arr, ptr(order, sptIndex), mode, C.byref(out.d), None), "NoAlloc")
+            check(lib, lib.sptolHipStreamSynchronize(None), "sync")
+            noalloc = out.fetch()
+            F.free()
+            out.free()
+            assert np.array_equal(got.view(np.uint32), noalloc.view(np.uint32))
+            assert np.all(got[:, R:] == 0), "padding columns must be zero"
+            e64 = rel_fro(got[:, :R], oracle.mttkrp_coo(inds, vals, Au, order, mode, dtype=np.float64)[:, :R])
+            _say(f"  vs FP64 oracle {e64:.2e}")
+            assert e64 <= TOL, e64
+    lib.sptolDevCooFree(h)
+
+
 def test_long_ranges_small_factors(gpu_lib):
     """20M nonzeros, R=32, factors small enough for L2 (at most 384 KB
     gathered per mode): the view's lane groups take about 616 (FP32) or 312

@@ -214,59 +214,21 @@ __global__ void k_scale(float *__restrict__ A, uint16_t *__restrict__ Abf, uint3
     scale_body(A, Abf, I, R, S, norms, blockIdx.x, gridDim.x);
 }
 
-// V = Hadamard of ata[p] for p = (m+k) % N, k = 1..N-1; then a Cholesky
-// factorisation V = L L^T.  If V is not positive definite, an LU
-// factorisation with partial pivoting of V replaces it (the gesv fallback
-// of matrix_solver.c:66-79).  Output: fac (R*R), piv (R), kind[0] = 0 (L) /
-// 1 (LU) / 2 (exactly singular: no factor, the solve leaves its right-hand
-// sides), and kind[1] incremented for each singular case.
+// V = Hadamard of ata[p] for p = (m+k) % N, k = 1..N-1, and its inverse in
+// FP64: by Gauss-Jordan elimination when V is positive definite (the
+// Cholesky case of matrix_solver.c:50-62), else from an LU factorisation
+// with partial pivoting (the gesv fallback of matrix_solver.c:66-79).
+// Output: fac = V^{-1} (R*R, row-major), piv (R), kind[0] = 0 (positive
+// definite) / 1 (LU) / 2 (exactly singular: no inverse, the solve leaves its
+// right-hand sides), and kind[1] incremented for each singular case.
 //
-// One wave (64 threads, R <= RM <= 64).  The Cholesky is right-looking with
-// lane i holding row i in registers: at step j lane j publishes its updated
-// diagonal, every lane takes its square root, lanes i > j scale their
-// column-j entry and publish it, then each lane subtracts l_ij * l_kj from
-// its row for k in (j, i].  Every entry sees the same operations in the same
-// order as the textbook left-looking loops, and the wave needs no barriers
-// (LDS operations of a wave complete in order).
+// One wave (64 threads, R <= RM <= 64), lane i holding row i in registers;
+// the wave needs no barriers (LDS operations of a wave complete in order).
 __device__ __forceinline__ void chol_wave_sync()
 {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-// Column c of V^{-1} from V = L L^T (L row-major in LDS, leading dimension
-// RM + 1): L y = e_c, then L^T x = y, in FP64; fac[a * R + c] = x[a].
-template <int RM>
-__device__ __forceinline__ void inv_column_chol(const double *__restrict__ L, uint32_t R, uint32_t c,
-                                                double *__restrict__ fac)
-{
-#pragma clang fp contract(off)
-    constexpr uint32_t LD = RM + 1;
-    double y[RM];
-#pragma unroll
-    for (int a = 0; a < RM; ++a) {
-        y[a] = 0.0;
-        if (a < (int)R) {
-            double s = a == (int)c ? 1.0 : 0.0;
-#pragma unroll
-            for (int k = 0; k < a; ++k) s -= L[a * LD + k] * y[k];
-            y[a] = s / L[a * LD + a];
-        }
-    }
-#pragma unroll
-    for (int a = RM - 1; a >= 0; --a) {
-        if (a < (int)R) {
-            double s = y[a];
-#pragma unroll
-            for (int k = a + 1; k < RM; ++k)
-                if (k < (int)R) s -= L[k * LD + a] * y[k];
-            y[a] = s / L[a * LD + a];
-        }
-    }
-#pragma unroll
-    for (int a = 0; a < RM; ++a)
-        if (a < (int)R) fac[a * R + c] = y[a];
 }
 
 // Column c of V^{-1} from P V = L U (unit L and U in LDS, leading dimension
@@ -314,8 +276,8 @@ __device__ __forceinline__ void factor_body(const double *__restrict__ ata, uint
                                             double *__restrict__ fac, int *__restrict__ piv, int *__restrict__ kind)
 {
 #pragma clang fp contract(off)
-    __shared__ double col[RM];
-    __shared__ double V[RM * (RM + 1)];   // L, or the LU fallback; rows padded against bank conflicts
+    __shared__ double col[RM + 1];        // the pivot row, then its diagonal
+    __shared__ double V[RM * (RM + 1)];   // the LU fallback; rows padded against bank conflicts
     __shared__ int pv[RM];
     const uint32_t RR = R * R, t = threadIdx.x;
     const bool mine = t < R;
@@ -332,27 +294,34 @@ __device__ __forceinline__ void factor_body(const double *__restrict__ ata, uint
 #pragma unroll
         for (int k = 0; k < RM; ++k) row[k] *= gv[k];
     }
+    // V^{-1} by Gauss-Jordan elimination in place, without pivoting: at step
+    // j lane j scales its row by 1 / v_jj (its diagonal becoming 1 / v_jj)
+    // and publishes it, and every other lane i subtracts v_ij times it (its
+    // column-j entry starting from 0).  The pivots are the diagonals of the
+    // successive Schur complements, all positive exactly when V is SPD (the
+    // condition of the Cholesky factorisation the reference tries first); a
+    // pivot <= 0 sends V to the LU fallback below.
     bool spd = true;                                    // wave-uniform
-#pragma unroll
+#pragma clang loop unroll(full)
     for (int j = 0; j < RM; ++j) {
         if (j < (int)R && spd) {
-            if (t == (uint32_t)j) col[j] = row[j];      // a_jj - sum_{k<j} l_jk^2
+            if (t == (uint32_t)j) col[RM] = row[j];
             chol_wave_sync();
-            const double d = col[j];
-            spd = d > 0.0;
+            const double p = col[RM];
+            spd = p > 0.0;
             if (spd) {
-                const double dj = sqrt(d);
-                if (t == (uint32_t)j) row[j] = dj;
-                if (mine && t > (uint32_t)j) {
-                    row[j] = row[j] / dj;
-                    col[t] = row[j];                    // l_tj
+                if (t == (uint32_t)j) {
+                    const double rp = 1.0 / p;
+#pragma unroll
+                    for (int k = 0; k < RM; ++k)
+                        if (k < (int)R) col[k] = row[k] = k == j ? rp : row[k] * rp;
                 }
                 chol_wave_sync();
-                if (mine && t > (uint32_t)j) {
-                    const double ltj = row[j];
+                if (mine && t != (uint32_t)j) {
+                    const double f = row[j];
 #pragma unroll
-                    for (int k = j + 1; k < RM; ++k)
-                        if (k <= (int)t) row[k] = row[k] - ltj * col[k];
+                    for (int k = 0; k < RM; ++k)
+                        if (k < (int)R) row[k] = (k == j ? 0.0 : row[k]) - f * col[k];
                 }
                 chol_wave_sync();                        // col is rewritten next step
             }
@@ -360,14 +329,11 @@ __device__ __forceinline__ void factor_body(const double *__restrict__ ata, uint
     }
     constexpr uint32_t LD = RM + 1;
     if (spd) {
-        // L to LDS, then column t of V^{-1} = L^{-T} L^{-1}
         if (mine) {
 #pragma unroll
             for (int k = 0; k < RM; ++k)
-                if (k < (int)R) V[t * LD + k] = k <= (int)t ? row[k] : 0.0;
+                if (k < (int)R) fac[t * R + k] = row[k];
         }
-        chol_wave_sync();
-        if (mine) inv_column_chol<RM>(V, R, t, fac);
         if (t == 0) kind[0] = 0;
         return;
     }

@@ -276,7 +276,7 @@ __device__ __forceinline__ void factor_body(const double *__restrict__ ata, uint
                                             double *__restrict__ fac, int *__restrict__ piv, int *__restrict__ kind)
 {
 #pragma clang fp contract(off)
-    __shared__ double col[RM + 1];        // the pivot row, then its diagonal
+    __shared__ double col[2][RM];         // the pivot rows of successive steps, alternately
     __shared__ double V[RM * (RM + 1)];   // the LU fallback; rows padded against bank conflicts
     __shared__ int pv[RM];
     const uint32_t RR = R * R, t = threadIdx.x;
@@ -295,35 +295,39 @@ __device__ __forceinline__ void factor_body(const double *__restrict__ ata, uint
         for (int k = 0; k < RM; ++k) row[k] *= gv[k];
     }
     // V^{-1} by Gauss-Jordan elimination in place, without pivoting: at step
-    // j lane j scales its row by 1 / v_jj (its diagonal becoming 1 / v_jj)
-    // and publishes it, and every other lane i subtracts v_ij times it (its
-    // column-j entry starting from 0).  The pivots are the diagonals of the
+    // j lane j publishes its row and every lane updates its own: lane j
+    // scales its row by 1 / v_jj (its diagonal becoming 1 / v_jj), every
+    // other lane i subtracts (v_ij / v_jj) times the pivot row (its column-j
+    // entry becoming -v_ij / v_jj).  The pivots are the diagonals of the
     // successive Schur complements, all positive exactly when V is SPD (the
     // condition of the Cholesky factorisation the reference tries first); a
-    // pivot <= 0 sends V to the LU fallback below.
+    // pivot <= 0 sends V to the LU fallback below.  Entries k >= R are 0 in
+    // every row and stay 0, so the updates run over all RM columns without a
+    // test per entry; the pivot rows alternate between two LDS rows, so one
+    // wave-level sync per step orders the write before the reads and the
+    // reads before the rewrite two steps later.
     bool spd = true;                                    // wave-uniform
 #pragma clang loop unroll(full)
     for (int j = 0; j < RM; ++j) {
         if (j < (int)R && spd) {
-            if (t == (uint32_t)j) col[RM] = row[j];
+            double *__restrict__ pr = col[j & 1];
+            if (t == (uint32_t)j) {
+#pragma unroll
+                for (int k = 0; k < RM; ++k) pr[k] = row[k];
+            }
             chol_wave_sync();
-            const double p = col[RM];
-            spd = p > 0.0;
+            const double p = pr[j];
+            spd = __builtin_amdgcn_readfirstlane((int)(p > 0.0)) != 0;
             if (spd) {
+                const double rp = 1.0 / p;
                 if (t == (uint32_t)j) {
-                    const double rp = 1.0 / p;
 #pragma unroll
-                    for (int k = 0; k < RM; ++k)
-                        if (k < (int)R) col[k] = row[k] = k == j ? rp : row[k] * rp;
-                }
-                chol_wave_sync();
-                if (mine && t != (uint32_t)j) {
-                    const double f = row[j];
+                    for (int k = 0; k < RM; ++k) row[k] = k == j ? rp : row[k] * rp;
+                } else {
+                    const double f = row[j] * rp;
 #pragma unroll
-                    for (int k = 0; k < RM; ++k)
-                        if (k < (int)R) row[k] = (k == j ? 0.0 : row[k]) - f * col[k];
+                    for (int k = 0; k < RM; ++k) row[k] = k == j ? -f : row[k] - f * pr[k];
                 }
-                chol_wave_sync();                        // col is rewritten next step
             }
         }
     }

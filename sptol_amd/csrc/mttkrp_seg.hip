@@ -62,21 +62,26 @@ constexpr int seg_waves() { return seg_u<LPN, NM, FT>() == 8 ? 3 : (NM == 3 ? SP
 // the 8-record classes then load records and store outputs non-temporally.
 constexpr uint64_t kSegNtFactorBytes = (uint64_t)32 << 20;
 
-// Waves per block: 4, or 16 when a factor is copied to LDS (one copy per
-// block, one block per compute unit).
-template <int LK>
-constexpr int seg_wpb() { return LK ? 16 : kSegWPB; }
-// Classes whose 16 waves' staging fits in LDS beside the factor copy (all
-// 4-mode classes but BF16 with one lane a group).
+// Waves per block: 4, or with a factor copied to LDS (one copy per block,
+// one block per compute unit) 16 for 4-mode tensors and 8 for 3-mode ones
+// (whose copies serve launches of at most 8 waves per compute unit).
+template <int NM, int LK>
+constexpr int seg_wpb() { return LK ? (NM == 3 ? 8 : 16) : kSegWPB; }
+// Classes whose waves' staging fits in LDS beside the factor copy (all but
+// 4-mode BF16 with one lane a group).
 template <int LPN, int NM, typename FT>
-constexpr bool seg_lds_class() { return NM == 4 && !(sizeof(FT) == 2 && LPN == 1); }
+constexpr bool seg_lds_class() { return NM == 3 || !(sizeof(FT) == 2 && LPN == 1); }
+// Register budget in waves per SIMD: a 3-mode block with a factor copy holds
+// a compute unit's LDS alone, so two waves per SIMD are all it can have.
+template <int LPN, int NM, typename FT, int LK>
+constexpr int seg_waves_lk() { return (NM == 3 && LK) ? 2 : seg_waves<LPN, NM, FT>(); }
 
 template <int LPN, int NM, typename FT, bool NT = false, int LK = 0>
-__global__ __launch_bounds__(seg_wpb<LK>() * 64) __attribute__((amdgpu_waves_per_eu(seg_waves<LPN, NM, FT>(), 8)))
+__global__ __launch_bounds__((seg_wpb<NM, LK>() * 64)) __attribute__((amdgpu_waves_per_eu(seg_waves_lk<LPN, NM, FT, LK>(), 8)))
 void k_mttkrp_seg(const SegArgs a, uint32_t *__restrict__ counter)
 {
     constexpr int U = seg_u<LPN, NM, FT>();
-    constexpr int WPB = seg_wpb<LK>();
+    constexpr int WPB = seg_wpb<NM, LK>();
     __shared__ SegSharedOf<LPN, NM, FT, WPB, U> sh;
     __shared__ __align__(16) FT sfac[LK ? kSegLdsFactorBytes / sizeof(FT) : 1];
     if constexpr (LK != 0) {
@@ -442,7 +447,7 @@ static int seg_carry_space(SegView &v, SegArgs &a)
 template <int LPN, int NM, typename FT, bool NT, int LK = 0>
 static uint64_t seg_resident_waves()
 {
-    constexpr int WPB = seg_wpb<LK>();
+    constexpr int WPB = seg_wpb<NM, LK>();
     const void *fn = reinterpret_cast<const void *>(&k_mttkrp_seg<LPN, NM, FT, NT, LK>);
     int per_cu = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, WPB * 64, 0) != hipSuccess || per_cu < 1)
@@ -453,7 +458,7 @@ static uint64_t seg_resident_waves()
 template <int LPN, int NM, typename FT, bool NT, int LK = 0>
 static void seg_launch_kernel(const SegArgs &a, uint32_t *counter, hipStream_t s)
 {
-    constexpr int WPB = seg_wpb<LK>();
+    constexpr int WPB = seg_wpb<NM, LK>();
     uint64_t blocks = seg_resident_waves<LPN, NM, FT, NT, LK>() / WPB;
     const uint64_t need = (a.nwaves + WPB - 1) / WPB;
     if (blocks > need) blocks = need;
@@ -476,7 +481,12 @@ static void seg_launch_main(const SegArgs &a, uint32_t *counter, hipStream_t s, 
         switch (a.lk) {
         case 1: seg_launch_kernel<LPN, NM, FT, false, 1>(a, counter, s); return;
         case 2: seg_launch_kernel<LPN, NM, FT, false, 2>(a, counter, s); return;
-        case 3: seg_launch_kernel<LPN, NM, FT, false, 3>(a, counter, s); return;
+        case 3:
+            if constexpr (NM == 4) {
+                seg_launch_kernel<LPN, NM, FT, false, 3>(a, counter, s);
+                return;
+            }
+            break;
         default: break;
         }
     }
@@ -496,7 +506,8 @@ static uint64_t seg_resident_main(bool nt, uint32_t lk)
     if constexpr (seg_lds_class<LPN, NM, FT>()) {
         if (lk == 1) return seg_resident_waves<LPN, NM, FT, false, 1>();
         if (lk == 2) return seg_resident_waves<LPN, NM, FT, false, 2>();
-        if (lk == 3) return seg_resident_waves<LPN, NM, FT, false, 3>();
+        if constexpr (NM == 4)
+            if (lk == 3) return seg_resident_waves<LPN, NM, FT, false, 3>();
     }
     return seg_resident_waves<LPN, NM, FT, false>();
 }
@@ -508,10 +519,10 @@ static uint64_t seg_resident_ft(const SegArgs &a, bool nt)
     const bool n3 = a.nmodes == 3;
     const uint32_t lk = a.lk;
     switch (seg_lane_class(a.glpn, sizeof(FT) == 2 ? SPTOL_BF16 : SPTOL_F32)) {
-    case 0: return n3 ? seg_resident_main<seg_lanes<FT, 0>(), 3, FT>(nt, 0) : seg_resident_main<seg_lanes<FT, 0>(), 4, FT>(nt, lk);
-    case 1: return n3 ? seg_resident_main<seg_lanes<FT, 1>(), 3, FT>(nt, 0) : seg_resident_main<seg_lanes<FT, 1>(), 4, FT>(nt, lk);
-    case 2: return n3 ? seg_resident_main<seg_lanes<FT, 2>(), 3, FT>(nt, 0) : seg_resident_main<seg_lanes<FT, 2>(), 4, FT>(nt, lk);
-    default: return n3 ? seg_resident_main<seg_lanes<FT, 3>(), 3, FT>(nt, 0) : seg_resident_main<seg_lanes<FT, 3>(), 4, FT>(nt, lk);
+    case 0: return n3 ? seg_resident_main<seg_lanes<FT, 0>(), 3, FT>(nt, lk) : seg_resident_main<seg_lanes<FT, 0>(), 4, FT>(nt, lk);
+    case 1: return n3 ? seg_resident_main<seg_lanes<FT, 1>(), 3, FT>(nt, lk) : seg_resident_main<seg_lanes<FT, 1>(), 4, FT>(nt, lk);
+    case 2: return n3 ? seg_resident_main<seg_lanes<FT, 2>(), 3, FT>(nt, lk) : seg_resident_main<seg_lanes<FT, 2>(), 4, FT>(nt, lk);
+    default: return n3 ? seg_resident_main<seg_lanes<FT, 3>(), 3, FT>(nt, lk) : seg_resident_main<seg_lanes<FT, 3>(), 4, FT>(nt, lk);
     }
 }
 
@@ -535,30 +546,6 @@ static void seg_launch_ft(const SegArgs &a, uint32_t *counter, hipStream_t s, bo
     }
 }
 
-// 4-mode views of at least 2M nonzeros read their smallest gathered factor
-// from a copy in LDS when it fits kSegLdsFactorBytes (SPTOL_SEG_LDS=0 turns
-// this off, for measurements).
-static void seg_pick_lds(const SegView &v, SegArgs &a, int dtype)
-{
-    static const bool on = [] {
-        const char *e = std::getenv("SPTOL_SEG_LDS");
-        return e ? std::atoi(e) != 0 : true;
-    }();
-    a.lk = 0;
-    a.lrows = 0;
-    if (!on || v.nmodes != 4 || v.nnz < ((uint64_t)2 << 20) || (dtype == SPTOL_BF16 && a.glpn == 1)) return;
-    const uint32_t o[3] = {a.o1, a.o2, a.o3};
-    uint64_t best = ~0ull;
-    for (uint32_t k = 0; k < 3; ++k) {
-        const uint64_t bytes = (uint64_t)v.ndims[o[k]] * a.S * (dtype == SPTOL_BF16 ? 2u : 4u);
-        if (bytes <= kSegLdsFactorBytes && bytes < best) {
-            best = bytes;
-            a.lk = k + 1;
-            a.lrows = v.ndims[o[k]];
-        }
-    }
-}
-
 // SPTOL_SEG_ROWS=0 turns ranges of whole rows off (for measurements).
 static bool seg_whole_rows()
 {
@@ -567,6 +554,45 @@ static bool seg_whole_rows()
         return e ? std::atoi(e) != 0 : true;
     }();
     return on;
+}
+
+// The smallest gathered factor is read from a copy in LDS when it fits
+// kSegLdsFactorBytes (SPTOL_SEG_LDS=0 turns this off, for measurements):
+//  * 4-mode views of at least 2M nonzeros (persistent blocks of 16 waves);
+//  * 3-mode views whose ranges of whole rows fill at most 8 waves per
+//    compute unit (blocks of 8 waves, one a compute unit: the launch is
+//    bound by each unit's gather issue, half of which the copy takes over),
+//    with at least twice as many nonzeros per block as the copy has rows.
+static void seg_pick_lds(const SegView &v, SegArgs &a, int dtype)
+{
+    static const bool on = [] {
+        const char *e = std::getenv("SPTOL_SEG_LDS");
+        return e ? std::atoi(e) != 0 : true;
+    }();
+    a.lk = 0;
+    a.lrows = 0;
+    if (!on) return;
+    if (v.nmodes == 4) {
+        if (v.nnz < ((uint64_t)2 << 20) || (dtype == SPTOL_BF16 && a.glpn == 1)) return;
+    } else if (v.nmodes == 3) {
+        if (!seg_whole_rows() || !v.rr_ok || !v.rr_start || v.rr_nwaves > (uint64_t)num_cus() * 8) return;
+    } else {
+        return;
+    }
+    const uint32_t o[3] = {a.o1, a.o2, a.o3};
+    uint64_t best = ~0ull;
+    for (uint32_t k = 0; k + 1 < v.nmodes; ++k) {
+        const uint64_t bytes = (uint64_t)v.ndims[o[k]] * a.S * (dtype == SPTOL_BF16 ? 2u : 4u);
+        if (bytes <= kSegLdsFactorBytes && bytes < best) {
+            best = bytes;
+            a.lk = k + 1;
+            a.lrows = v.ndims[o[k]];
+        }
+    }
+    if (v.nmodes == 3 && a.lk && v.nnz < 2 * (uint64_t)a.lrows * ((v.rr_nwaves + 7) / 8)) {
+        a.lk = 0;
+        a.lrows = 0;
+    }
 }
 
 // Whether a launch on v takes its ranges of whole rows (built with the view,

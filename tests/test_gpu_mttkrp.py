@@ -209,6 +209,37 @@ def test_whole_row_ranges_have_no_carry(gpu_lib, R):
         lib.sptolDevCooFree(h)
 
 
+@pytest.mark.parametrize("R,dtype", [(16, 0), (32, SPTOL_BF16)])
+def test_lds_factor_3d_matches_global(gpu_lib, R, dtype):
+    """2000 x 1500 x 1000 with 1M uniform nonzeros, mode 0: ranges of whole
+    rows, and the 1000-row factor (64 KB) fits LDS, so each block of 8 waves
+    gathers it from a copy in LDS.  Declaring the last mode 1100 rows long
+    (the same nonzeros, 100 unused factor rows) puts that factor above
+    64 KB, so every gather goes to global memory.  Both give the same bits,
+    twice in a row, and meet the FP64 oracle bar."""
+    lib = gpu_lib
+    d, inds, vals = gen.uniform_tensor([2000, 1500, 1000], 1_000_000, seed=2024)
+    A = gen.uniform_factors(d, R, seed=2025)
+    extra = gen.uniform_factors([100], R, seed=2026)[0]
+    dims2 = [int(d[0]), int(d[1]), int(d[2]) + 100]
+    A2 = [A[0], A[1], np.vstack([A[2], extra])]
+    order = np.array([0, 1, 2], np.uint32)
+    res = []
+    for dims, fac in ((d, A), (dims2, A2)):
+        _say("lds factor 3-D", R, dtype, list(dims))
+        h = dev_coo(lib, dims, inds, vals)
+        check(lib, lib.sptolDevCooPrepareMode(h, 0), "PrepareMode")
+        got = dev_mttkrp_coo(lib, dims, inds, vals, fac, order, 0, dtype=dtype, h=h)
+        again = dev_mttkrp_coo(lib, dims, inds, vals, fac, order, 0, dtype=dtype, h=h)
+        lib.sptolDevCooFree(h)
+        assert np.array_equal(got.view(np.uint32), again.view(np.uint32))
+        res.append(got)
+    assert np.array_equal(res[0].view(np.uint32), res[1].view(np.uint32))
+    Au = [bf16_round(a) for a in A] if dtype else A
+    _check(res[0], oracle.mttkrp_coo(inds, vals, Au, order, 0),
+           oracle.mttkrp_coo(inds, vals, Au, order, 0, dtype=np.float64), R)
+
+
 @pytest.mark.parametrize("R", [16, 32])
 def test_lds_factor_4d(gpu_lib, R):
     """A 4-D tensor of 2.5M nonzeros with a 50-row mode: the MTTKRPs of the

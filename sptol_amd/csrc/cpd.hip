@@ -533,6 +533,96 @@ __global__ __launch_bounds__(256) void k_inner_exact_partial(const InnerArgs a, 
     if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
 }
 
+// The same FP64 <X,K> from the packed records of one of the tensor's mode
+// views (each record holds every mode's index: {i0, i1, i2, bits(v)} for 3
+// modes, {i0, i1, i2, i3} with the view's value array for 4).  A lane group
+// takes kInnerU records a step: its lanes load them with one 16-byte access
+// each and pass indices and values to the group by shuffles, so a step
+// issues one record load (and one value load) where the kernel above issues
+// one index load per mode and nonzero.  Same per-lane sums and block
+// reduction as above; the nonzeros come in the view's order.
+template <int LPN, int NM>
+__global__ __launch_bounds__(256) void k_inner_exact_rec(const uint4 *__restrict__ rec, const float *__restrict__ vals4,
+                                                         const InnerArgs a, const double *__restrict__ lambda,
+                                                         double *__restrict__ partial)
+{
+#pragma clang fp contract(off)
+    constexpr uint32_t G = 256 / LPN;
+    constexpr int U = kInnerU;
+    constexpr int KL = (U + LPN - 1) / LPN;   // records a lane loads per step
+    __shared__ double red[256];
+    const uint32_t g = threadIdx.x / LPN, l = threadIdx.x % LPN, c0 = 4 * l;
+    const int gbase = (int)((threadIdx.x & 63u) - l);   // the group's first lane in the wave
+    double lam[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) lam[q] = c0 + q < a.R ? lambda[c0 + q] : 0.0;
+    const uint32_t cc = c0 < a.S ? c0 : 0;
+    const uint64_t per = (a.nnz + gridDim.x - 1) / gridDim.x;
+    const uint64_t z0 = (uint64_t)blockIdx.x * per, z1 = min<uint64_t>(z0 + per, a.nnz);
+    double s = 0.0;
+    for (uint64_t zb = z0 + (uint64_t)g * U; zb < z1; zb += (uint64_t)G * U) {
+        uint4 rk[KL];
+        float vk[KL];
+#pragma unroll
+        for (int k = 0; k < KL; ++k) {
+            // lanes past the step's U records load nothing
+            const uint32_t idx = l + (uint32_t)k * LPN;
+            const uint64_t z = min<uint64_t>(zb + idx, z1 - 1);   // zb < z1: in bounds
+            rk[k] = idx < (uint32_t)U ? rec[z] : make_uint4(0u, 0u, 0u, 0u);
+            vk[k] = idx < (uint32_t)U ? (NM == 4 ? vals4[z] : __uint_as_float(rk[k].w)) : 0.0f;
+        }
+        uint32_t j[U][4];
+        float v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int src = gbase + u % LPN, k = u / LPN;
+            j[u][0] = (uint32_t)__shfl((int)rk[k].x, src);
+            j[u][1] = (uint32_t)__shfl((int)rk[k].y, src);
+            j[u][2] = (uint32_t)__shfl((int)rk[k].z, src);
+            j[u][3] = NM == 4 ? (uint32_t)__shfl((int)rk[k].w, src) : 0u;
+            v[u] = zb + u < z1 ? __shfl(vk[k], src) : 0.0f;
+        }
+        float4 f[U][NM];
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+#pragma unroll
+            for (int m = 0; m < NM; ++m)
+                f[u][m] = *reinterpret_cast<const float4 *>(a.mats[m] + (size_t)j[u][m] * a.S + cc);
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            double p[4] = {lam[0], lam[1], lam[2], lam[3]};
+#pragma unroll
+            for (int m = 0; m < NM; ++m) {
+                p[0] *= (double)f[u][m].x; p[1] *= (double)f[u][m].y;
+                p[2] *= (double)f[u][m].z; p[3] *= (double)f[u][m].w;
+            }
+            // padding columns (>= R) may hold unspecified values: drop them
+#pragma unroll
+            for (int q = 0; q < 4; ++q) p[q] = c0 + q < a.R ? p[q] : 0.0;
+            s += (double)v[u] * (((p[0] + p[1]) + p[2]) + p[3]);
+        }
+    }
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (uint32_t w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+template <int NM>
+static void launch_inner_exact_rec(const SegView &v, const InnerArgs &a, const double *lambda, double *partial,
+                                   hipStream_t s)
+{
+    const dim3 g(kInnerBlocks), b(256);
+    const float *v4 = NM == 4 ? v.vals : nullptr;
+    if (a.S <= 8) hipLaunchKernelGGL((k_inner_exact_rec<2, NM>), g, b, 0, s, v.rec, v4, a, lambda, partial);
+    else if (a.S <= 16) hipLaunchKernelGGL((k_inner_exact_rec<4, NM>), g, b, 0, s, v.rec, v4, a, lambda, partial);
+    else if (a.S <= 32) hipLaunchKernelGGL((k_inner_exact_rec<8, NM>), g, b, 0, s, v.rec, v4, a, lambda, partial);
+    else hipLaunchKernelGGL((k_inner_exact_rec<16, NM>), g, b, 0, s, v.rec, v4, a, lambda, partial);
+}
+
 // The same FP64 <X,K> for any order N <= kMaxModes, over a COO tensor (inds)
 // or a HiCOO tensor (coordinate = (binds[m][b] << sb_bits) + einds[m][z] for
 // the block b holding nonzero z).  Each lane group takes a contiguous run of
@@ -1179,8 +1269,16 @@ int CpdJob::finish_enqueue(hipStream_t s)
             ia.vals = X->vals;
             ia.nnz = X->nnz;
             ia.N = N; ia.R = R; ia.S = S;
+            // a mode view's packed records when one exists (the ALS loop
+            // built them), else the tensor's own index arrays
+            const SegView *sv = nullptr;
+            for (uint32_t q = 0; q < N && !sv; ++q)
+                if (X->sorted[q].ready && X->sorted[q].seg.ready && X->sorted[q].seg.rec && X->sorted[q].seg.nnz == X->nnz)
+                    sv = &X->sorted[q].seg;
             const dim3 g(kInnerBlocks);
-            if (S <= 8) hipLaunchKernelGGL(k_inner_exact_partial<2>, g, dim3(256), 0, s, ia, w.lambda, w.partial);
+            if (sv && N == 3) launch_inner_exact_rec<3>(*sv, ia, w.lambda, w.partial, s);
+            else if (sv && N == 4) launch_inner_exact_rec<4>(*sv, ia, w.lambda, w.partial, s);
+            else if (S <= 8) hipLaunchKernelGGL(k_inner_exact_partial<2>, g, dim3(256), 0, s, ia, w.lambda, w.partial);
             else if (S <= 16) hipLaunchKernelGGL(k_inner_exact_partial<4>, g, dim3(256), 0, s, ia, w.lambda, w.partial);
             else if (S <= 32) hipLaunchKernelGGL(k_inner_exact_partial<8>, g, dim3(256), 0, s, ia, w.lambda, w.partial);
             else hipLaunchKernelGGL(k_inner_exact_partial<16>, g, dim3(256), 0, s, ia, w.lambda, w.partial);

@@ -297,8 +297,8 @@ __device__ __forceinline__ void factor_body(const double *__restrict__ ata, uint
     // V^{-1} by Gauss-Jordan elimination in place, without pivoting: at step
     // j lane j publishes its row and every lane updates its own: lane j
     // scales its row by 1 / v_jj (its diagonal becoming 1 / v_jj), every
-    // other lane i subtracts (v_ij / v_jj) times the pivot row (its column-j
-    // entry becoming -v_ij / v_jj).  The pivots are the diagonals of the
+    // other lane i subtracts (v_ij / v_jj) times the pivot row by fused
+    // multiply-adds (its column-j entry becoming -v_ij / v_jj).  The pivots are the diagonals of the
     // successive Schur complements, all positive exactly when V is SPD (the
     // condition of the Cholesky factorisation the reference tries first); a
     // pivot <= 0 sends V to the LU fallback below.  Entries k >= R are 0 in
@@ -316,7 +316,12 @@ __device__ __forceinline__ void factor_body(const double *__restrict__ ata, uint
                 for (int k = 0; k < RM; ++k) pr[k] = row[k];
             }
             chol_wave_sync();
-            const double p = pr[j];
+            // the whole pivot row is read at once, its latency overlapping
+            // the reciprocal of the pivot
+            double pv[RM];
+#pragma unroll
+            for (int k = 0; k < RM; ++k) pv[k] = pr[k];
+            const double p = pv[j];
             spd = __builtin_amdgcn_readfirstlane((int)(p > 0.0)) != 0;
             if (spd) {
                 const double rp = 1.0 / p;
@@ -324,9 +329,11 @@ __device__ __forceinline__ void factor_body(const double *__restrict__ ata, uint
 #pragma unroll
                     for (int k = 0; k < RM; ++k) row[k] = k == j ? rp : row[k] * rp;
                 } else {
+                    // one fused multiply-add per entry (one rounding): the
+                    // entries are independent, so they issue back to back
                     const double f = row[j] * rp;
 #pragma unroll
-                    for (int k = 0; k < RM; ++k) row[k] = k == j ? -f : row[k] - f * pr[k];
+                    for (int k = 0; k < RM; ++k) row[k] = k == j ? -f : __builtin_fma(-f, pv[k], row[k]);
                 }
             }
         }

@@ -214,6 +214,97 @@ __global__ void k_scale(float *__restrict__ A, uint16_t *__restrict__ Abf, uint3
     scale_body(A, Abf, I, R, S, norms, blockIdx.x, gridDim.x);
 }
 
+// One launch in place of k_col_final, k_scale and k_gram_partial: every
+// block first forms all R column norms from k_col_partial's ncb partials
+// (thread t sums partials t / R, t / R + 256 / R, ... of column t % R, then
+// the column's sums are added in thread order: a fixed order, all loads
+// issued at once; block 0 also stores lambda and norms), then scales its
+// rows of A (the rows gram_partial_body gives it; FP64 divide, FP32 store,
+// BF16 copy of every column, as scale_body) and sums the Gram partials of
+// the scaled rows (as gram_partial_body).
+__global__ __launch_bounds__(256) void k_norm_scale_gram(float *__restrict__ A, uint16_t *__restrict__ Abf, uint32_t I,
+                                                         uint32_t R, uint32_t S, int kind,
+                                                         const double *__restrict__ colpart, uint32_t ncb,
+                                                         double *__restrict__ lambda, double *__restrict__ norms,
+                                                         double *__restrict__ gpart)
+{
+    __shared__ double red[256];
+    __shared__ double nrm[kMaxRank];
+    __shared__ float tile[16][kMaxRank];
+    {
+        const uint32_t nsub = 256 / R, c = threadIdx.x % R, sub = threadIdx.x / R;   // R <= 64: nsub >= 4
+        double v = 0.0;
+        if (sub < nsub)
+            for (uint32_t b = sub; b < ncb; b += nsub) {
+                const double y = colpart[(size_t)b * R + c];
+                if (kind == 0) v += y;
+                else v = y > v ? y : v;
+            }
+        red[threadIdx.x] = v;
+        __syncthreads();
+        if (threadIdx.x < R) {
+            double t = red[threadIdx.x];
+            for (uint32_t k = 1; k < nsub; ++k) {
+                const double y = red[threadIdx.x + k * R];
+                if (kind == 0) t += y;
+                else t = y > t ? y : t;
+            }
+            const double n = kind == 0 ? sqrt(t) : (t < 1.0 ? 1.0 : t);
+            nrm[threadIdx.x] = n;
+            if (blockIdx.x == 0) {
+                norms[threadIdx.x] = n;
+                lambda[threadIdx.x] = n;
+            }
+        }
+        __syncthreads();
+    }
+    const uint32_t RR = R * R;
+    double acc[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) acc[q] = 0.0;
+    const uint64_t rows_per = ((uint64_t)I + gridDim.x - 1) / gridDim.x;
+    const uint64_t r0 = (uint64_t)blockIdx.x * rows_per;
+    const uint64_t r1 = min<uint64_t>(r0 + rows_per, I);
+    for (uint64_t base = r0; base < r1; base += 16) {
+        for (uint32_t t = threadIdx.x; t < 16 * S; t += 256) {
+            const uint32_t rr = t / S, c = t % S;
+            float x = 0.0f;
+            if (base + rr < r1) {
+                const uint64_t e = (base + rr) * S + c;
+                x = A[e];
+                if (c < R) {
+                    x = (float)((double)x / nrm[c]);
+                    A[e] = x;
+                }
+                if (Abf) {
+                    uint32_t u = __float_as_uint(x);
+                    u += 0x7FFFu + ((u >> 16) & 1u);
+                    Abf[e] = (uint16_t)(u >> 16);
+                }
+            }
+            if (c < R) tile[rr][c] = x;
+        }
+        __syncthreads();
+        const uint32_t nrow = (uint32_t)min<uint64_t>(16, r1 - base);
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const uint32_t e = threadIdx.x + 256u * q;
+            if (e < RR) {
+                const uint32_t a = e / R, b = e % R;
+                double sum = acc[q];
+                for (uint32_t rr = 0; rr < nrow; ++rr) sum += (double)tile[rr][a] * (double)tile[rr][b];
+                acc[q] = sum;
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+        const uint32_t e = threadIdx.x + 256u * q;
+        if (e < RR) gpart[(size_t)blockIdx.x * RR + e] = acc[q];
+    }
+}
+
 // V = Hadamard of ata[p] for p = (m+k) % N, k = 1..N-1, and its inverse in
 // FP64: by Gauss-Jordan elimination when V is positive definite (the
 // Cholesky case of matrix_solver.c:50-62), else from an LU factorisation
@@ -975,6 +1066,24 @@ static int colnorm(float *A, uint16_t *Abf, uint32_t I, uint32_t R, uint32_t S, 
     return 0;
 }
 
+// The step after a mode's solve: column norms (kind as in colnorm, lambda
+// set to them), the scaled factor and its BF16 copy, and the mode's Gram
+// into ata_m.  Three launches; the same results as colnorm + gram.
+static int norm_scale_gram(float *A, uint16_t *Abf, uint32_t I, uint32_t R, uint32_t S, int kind, CpdWork &w,
+                           double *ata_m, hipStream_t s)
+{
+    int tok = prof_start("cpd_norm", s);
+    const uint32_t ncb = red_blocks(I, 64), ngb = red_blocks(I, 32);
+    double *colpart = w.partial + (size_t)kRedBlocks * R * R;   // beside the Gram partials
+    hipLaunchKernelGGL(k_col_partial, dim3(ncb), dim3(256), 0, s, A, I, R, S, kind, colpart);
+    hipLaunchKernelGGL(k_norm_scale_gram, dim3(ngb), dim3(256), 0, s, A, Abf, I, R, S, kind, colpart, ncb, w.lambda,
+                       w.norms, w.partial);
+    hipLaunchKernelGGL(k_sum_partials, dim3(sum_grid(R * R)), dim3(256), 0, s, w.partial, ngb, R * R, ata_m);
+    prof_stop(tok, s);
+    SPTOL_HIP(hipGetLastError(), "CPD norm");
+    return 0;
+}
+
 static int solve(const float *M, float *A, uint32_t I, uint32_t R, uint32_t S, CpdWork &w, uint32_t N, uint32_t m,
                  hipStream_t s)
 {
@@ -1104,7 +1213,7 @@ size_t CpdJob::workspace_bytes(const uint32_t *ndims, uint32_t N, uint32_t R, in
     uint64_t maxI = 1, sumI = 0;
     for (uint32_t m = 0; m < N; ++m) { maxI = ndims[m] > maxI ? ndims[m] : maxI; sumI += ndims[m]; }
     size_t b = (size_t)maxI * S * 4 + (size_t)N * R * R * 8 +
-               std::max<size_t>((size_t)kRedBlocks * R * R, kInnerBlocks + 1) * 8 + (size_t)R * R * 8 + 4096;
+               std::max<size_t>((size_t)kRedBlocks * R * (R + 1), kInnerBlocks + 1) * 8 + (size_t)R * R * 8 + 4096;
     if (fdt == SPTOL_BF16) b += (size_t)sumI * S * 2;
     return b;
 }
@@ -1148,7 +1257,8 @@ int CpdJob::init(uint32_t R_, uint32_t niters_, double tol_, sptolDevMatrix *mat
     S = mats[0].stride;
     SPTOL_TRY(dmalloc((void **)&w.M, (size_t)(maxI ? maxI : 1) * S * 4, "CPD"));
     SPTOL_TRY(dmalloc((void **)&w.ata, (size_t)N * R * R * 8, "CPD"));
-    SPTOL_TRY(dmalloc((void **)&w.partial, std::max<size_t>((size_t)kRedBlocks * R * R, kInnerBlocks + 1) * 8, "CPD"));
+    SPTOL_TRY(dmalloc((void **)&w.partial,
+                      std::max<size_t>((size_t)kRedBlocks * R * (R + 1), kInnerBlocks + 1) * 8, "CPD"));
     SPTOL_TRY(dmalloc((void **)&w.fac, (size_t)R * R * 8, "CPD"));
     SPTOL_TRY(dmalloc((void **)&w.piv, (size_t)R * 4, "CPD"));
     SPTOL_TRY(dmalloc((void **)&w.kind, 8, "CPD"));   // [0] factor kind, [1] singular solves so far
@@ -1208,9 +1318,8 @@ int CpdJob::after_mttkrp(hipStream_t s)
 {
     const uint32_t I = ndims[m];
     SPTOL_TRY(solve(w.M, (float *)mats[m].values, I, R, S, w, N, m, s));
-    SPTOL_TRY(colnorm((float *)mats[m].values, fdt == SPTOL_BF16 ? w.bf[m] : nullptr, I, R, S, it == 0 ? 0 : 1, 0,
-                      w, s));
-    SPTOL_TRY(gram((const float *)mats[m].values, I, R, S, w.ata + (size_t)m * R * R, w.partial, s));
+    SPTOL_TRY(norm_scale_gram((float *)mats[m].values, fdt == SPTOL_BF16 ? w.bf[m] : nullptr, I, R, S, it == 0 ? 0 : 1,
+                              w, w.ata + (size_t)m * R * R, s));
     if (++m < N) return 0;
     // end of an iteration: fit from the last mode's MTTKRP, still in w.M
     const uint32_t last = N - 1;

@@ -14,7 +14,7 @@ struct CpdWork {
     float *M = nullptr;            // MTTKRP output, maxI x S
     uint16_t *bf[kMaxModes] = {};  // BF16 factor copies
     double *ata = nullptr;         // N x R x R Grams
-    double *partial = nullptr;     // kRedBlocks x R x R
+    double *partial = nullptr;     // kRedBlocks x R x R (Gram partials), then kRedBlocks x R (column partials)
     double *fac = nullptr;         // R x R Cholesky or LU factor
     int *piv = nullptr;            // R
     int *kind = nullptr;           // [0]: 0 Cholesky, 1 LU, 2 singular; [1]: singular solves so far

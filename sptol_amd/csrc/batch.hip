@@ -169,12 +169,22 @@ constexpr int kBatchWPB = 4;
 #define SPTOL_BATCH_WAVES 4
 #endif
 
+// Records a lane-group step: 3-mode FP32 classes of 32 or more columns keep
+// 8 records' gathers in flight at 3 waves per SIMD (their factors are the
+// largest of a batch), the others 4 at the budget above; the sums are the
+// same either way (records are added one at a time in order).
 template <int LPN, int NM, typename FT>
-__global__ __launch_bounds__(kBatchWPB * 64) __attribute__((amdgpu_waves_per_eu(SPTOL_BATCH_WAVES, 8))) void k_batch_seg(const SegArgs *__restrict__ items,
+constexpr int batch_u() { return (NM == 3 && sizeof(FT) == 4 && LPN >= 8) ? 8 : kSegU; }
+template <int LPN, int NM, typename FT>
+constexpr int batch_waves() { return batch_u<LPN, NM, FT>() == 8 ? 3 : SPTOL_BATCH_WAVES; }
+
+template <int LPN, int NM, typename FT>
+__global__ __launch_bounds__(kBatchWPB * 64) __attribute__((amdgpu_waves_per_eu(batch_waves<LPN, NM, FT>(), 8))) void k_batch_seg(const SegArgs *__restrict__ items,
                                                               const uint64_t *__restrict__ prefix, uint32_t n,
                                                               uint64_t total, uint32_t *__restrict__ counter)
 {
-    __shared__ SegSharedOf<LPN, NM, FT, kBatchWPB> sh;
+    constexpr int U = batch_u<LPN, NM, FT>();
+    __shared__ SegSharedOf<LPN, NM, FT, kBatchWPB, U> sh;
     // the wave index is uniform across the wave: read from lane 0, the
     // range's indices, bounds and arguments live in scalar registers
     const uint32_t lane = threadIdx.x & 63u, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -187,7 +197,7 @@ __global__ __launch_bounds__(kBatchWPB * 64) __attribute__((amdgpu_waves_per_eu(
             if (prefix[mid] <= t) lo = mid; else hi = mid;
         }
         const SegArgs a = items[lo];
-        seg_range<LPN, NM, FT, kBatchWPB>(a, t - prefix[lo], sh, wv, lane);
+        seg_range<LPN, NM, FT, kBatchWPB, U>(a, t - prefix[lo], sh, wv, lane);
         if (W >= total) break;   // every range was taken by index
         uint32_t wq = 0;
         if (lane == 0) wq = atomicAdd(counter, 1u);

@@ -169,15 +169,17 @@ constexpr int kBatchWPB = 4;
 #define SPTOL_BATCH_WAVES 4
 #endif
 
-// Records a lane-group step: 3-mode FP32 classes keep 8 records' gathers in
-// flight at 3 waves per SIMD (the 256-tensor batch of 390M nonzeros: 11.05
-// ms per step against 11.20 with 8 only at 32+ columns and 11.5 with 4
-// everywhere), the others 4 at the budget above; the sums are the same
-// either way (records are added one at a time, in order).
+// Records a lane-group step: FP32 classes keep 8 records' gathers in flight,
+// 3-mode ones at 3 waves per SIMD (the 256-tensor batch of 390M nonzeros:
+// 11.05 ms per step against 11.20 with 8 only at 32+ columns and 11.5 with
+// 4 everywhere), 4-mode ones at 2 (served CPD stream of 400 requests: 189
+// solves/s against 180 with 4 records at 4 waves); BF16 classes take 4 at
+// the budget above.  The sums are the same either way (records are added
+// one at a time, in order).
 template <int LPN, int NM, typename FT>
-constexpr int batch_u() { return (NM == 3 && sizeof(FT) == 4) ? 8 : kSegU; }
+constexpr int batch_u() { return sizeof(FT) == 4 ? 8 : kSegU; }
 template <int LPN, int NM, typename FT>
-constexpr int batch_waves() { return batch_u<LPN, NM, FT>() == 8 ? 3 : SPTOL_BATCH_WAVES; }
+constexpr int batch_waves() { return batch_u<LPN, NM, FT>() == 8 ? (NM == 4 ? 2 : 3) : SPTOL_BATCH_WAVES; }
 
 template <int LPN, int NM, typename FT>
 __global__ __launch_bounds__(kBatchWPB * 64) __attribute__((amdgpu_waves_per_eu(batch_waves<LPN, NM, FT>(), 8))) void k_batch_seg(const SegArgs *__restrict__ items,

@@ -48,25 +48,39 @@ constexpr int kSegWPB = 4;
 #endif
 
 // Records a lane-group step and the register budget (waves per SIMD) of a
-// class: 3-mode FP32 classes of 32 or more columns (factor rows that mostly
-// miss the caches) keep 8 records' gathers in flight, with 3 waves per SIMD
-// so that they fit in registers (the 100M-nonzero Zipf headline: 6.34 ms
-// per 3-mode step against 6.74 ms at 4 records and 4 waves); every other
-// class takes 4 records at the default budget.
+// class.  More records a step keep more gathers in flight:
+//  * 3-mode FP32 classes of 32 or more columns take 8 records at 3 waves
+//    per SIMD (the 100M-nonzero Zipf headline: 6.34 ms per 3-mode step
+//    against 6.74 ms at 4 records and 4 waves);
+//  * 4-mode classes take 8 records at 2 waves per SIMD (the 4-D CPD of 50M
+//    nonzeros, R = 32: MTTKRP 856 -> 783 us on the mode whose factors all
+//    come from global memory, 700 -> 605 us FP32 and 540 -> 480 us BF16 on
+//    the modes that read one factor from LDS);
+//  * the others take 4 records at the default budget.
+// The sums do not depend on it: records are added one at a time, in order.
+template <int LPN, int NM, typename FT, int LK = 0>
+constexpr int seg_u()
+{
+    return (NM == 4 || (sizeof(FT) == 4 && LPN >= 8)) ? 8 : kSegU;
+}
+template <int LPN, int NM, typename FT, int LK = 0>
+constexpr int seg_waves()
+{
+    return seg_u<LPN, NM, FT, LK>() == 8 ? (NM == 4 ? 2 : 3) : (NM == 3 ? SPTOL_SEG_WAVES3 : SPTOL_SEG_WAVES4);
+}
+// Classes with a non-temporal variant (records and outputs) for factors in HBM.
 template <int LPN, int NM, typename FT>
-constexpr int seg_u() { return (NM == 3 && sizeof(FT) == 4 && LPN >= 8) ? 8 : kSegU; }
-template <int LPN, int NM, typename FT>
-constexpr int seg_waves() { return seg_u<LPN, NM, FT>() == 8 ? 3 : (NM == 3 ? SPTOL_SEG_WAVES3 : SPTOL_SEG_WAVES4); }
+constexpr bool seg_has_nt() { return NM == 3 && sizeof(FT) == 4 && LPN >= 8; }
 
 // Gathered factors above this many bytes (the L2 of all XCDs) live in HBM:
 // the 8-record classes then load records and store outputs non-temporally.
 constexpr uint64_t kSegNtFactorBytes = (uint64_t)32 << 20;
 
-// Waves per block: 4, or with a factor copied to LDS (one copy per block,
-// one block per compute unit) 16 for 4-mode tensors and 8 for 3-mode ones
-// (whose copies serve launches of at most 8 waves per compute unit).
+// Waves per block: 4, or 8 with a factor copied to LDS (one copy per block,
+// one block per compute unit: 2 waves per SIMD, the budget of the 4-mode
+// classes and all that a 3-mode copy's launch fills).
 template <int NM, int LK>
-constexpr int seg_wpb() { return LK ? (NM == 3 ? 8 : 16) : kSegWPB; }
+constexpr int seg_wpb() { return LK ? 8 : kSegWPB; }
 // Classes whose waves' staging fits in LDS beside the factor copy (all but
 // 4-mode BF16 with one lane a group).
 template <int LPN, int NM, typename FT>
@@ -74,13 +88,13 @@ constexpr bool seg_lds_class() { return NM == 3 || !(sizeof(FT) == 2 && LPN == 1
 // Register budget in waves per SIMD: a 3-mode block with a factor copy holds
 // a compute unit's LDS alone, so two waves per SIMD are all it can have.
 template <int LPN, int NM, typename FT, int LK>
-constexpr int seg_waves_lk() { return (NM == 3 && LK) ? 2 : seg_waves<LPN, NM, FT>(); }
+constexpr int seg_waves_lk() { return (NM == 3 && LK) ? 2 : seg_waves<LPN, NM, FT, LK>(); }
 
 template <int LPN, int NM, typename FT, bool NT = false, int LK = 0>
 __global__ __launch_bounds__((seg_wpb<NM, LK>() * 64)) __attribute__((amdgpu_waves_per_eu(seg_waves_lk<LPN, NM, FT, LK>(), 8)))
 void k_mttkrp_seg(const SegArgs a, uint32_t *__restrict__ counter)
 {
-    constexpr int U = seg_u<LPN, NM, FT>();
+    constexpr int U = seg_u<LPN, NM, FT, LK>();
     constexpr int WPB = seg_wpb<NM, LK>();
     __shared__ SegSharedOf<LPN, NM, FT, WPB, U> sh;
     __shared__ __align__(16) FT sfac[LK ? kSegLdsFactorBytes / sizeof(FT) : 1];
@@ -471,7 +485,7 @@ static void seg_launch_kernel(const SegArgs &a, uint32_t *counter, hipStream_t s
 template <int LPN, int NM, typename FT>
 static void seg_launch_main(const SegArgs &a, uint32_t *counter, hipStream_t s, bool nt)
 {
-    if constexpr (seg_u<LPN, NM, FT>() == 8) {
+    if constexpr (seg_has_nt<LPN, NM, FT>()) {
         if (nt) {
             seg_launch_kernel<LPN, NM, FT, true>(a, counter, s);
             return;
@@ -500,7 +514,7 @@ constexpr int seg_lanes() { return (sizeof(FT) == 2 ? 1 : 2) << I; }
 template <int LPN, int NM, typename FT>
 static uint64_t seg_resident_main(bool nt, uint32_t lk)
 {
-    if constexpr (seg_u<LPN, NM, FT>() == 8) {
+    if constexpr (seg_has_nt<LPN, NM, FT>()) {
         if (nt) return seg_resident_waves<LPN, NM, FT, true>();
     }
     if constexpr (seg_lds_class<LPN, NM, FT>()) {
@@ -558,7 +572,7 @@ static bool seg_whole_rows()
 
 // The smallest gathered factor is read from a copy in LDS when it fits
 // kSegLdsFactorBytes (SPTOL_SEG_LDS=0 turns this off, for measurements):
-//  * 4-mode views of at least 2M nonzeros (persistent blocks of 16 waves);
+//  * 4-mode views of at least 2M nonzeros (persistent blocks of 8 waves);
 //  * 3-mode views whose ranges of whole rows fill at most 8 waves per
 //    compute unit (blocks of 8 waves, one a compute unit: the launch is
 //    bound by each unit's gather issue, half of which the copy takes over),

@@ -64,7 +64,7 @@ struct SegArgs {
     uint32_t lrows;
 };
 
-// Largest factor the 4-mode range kernels copy to LDS (blocks of 16 waves).
+// Largest factor the range kernels copy to LDS (blocks of 8 waves).
 constexpr uint32_t kSegLdsFactorBytes = 64u << 10;
 
 // SegArgs of one MTTKRP on view v with factors of type dtype (carry space

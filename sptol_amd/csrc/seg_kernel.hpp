@@ -480,7 +480,20 @@ __device__ __forceinline__ void seg_carry_slots(const float *__restrict__ carry,
         if (c0 < S) {
             float4 o = seg_ld(out + (size_t)r * S + c0);   // in flight with the carries
             float4 s = seg_ld(carry + (k0 + i) * kSegCarryStride + c0);
-            for (uint32_t q = i + 1; q < j; ++q) seg_add(s, seg_ld(carry + (k0 + q) * kSegCarryStride + c0));
+            // the run's other carries, 8 loads in flight at a time (indices
+            // clamped into the run), added in ascending order; slots past
+            // the run add -0.0f, which leaves every sum's bits unchanged
+            const float4 nz = make_float4(-0.f, -0.f, -0.f, -0.f);
+            for (uint32_t q0 = i + 1; q0 < j; q0 += 8) {
+                float4 v[8];
+#pragma unroll
+                for (uint32_t t = 0; t < 8; ++t) {
+                    const uint32_t q = min(q0 + t, j - 1);
+                    v[t] = seg_ld(carry + (k0 + q) * kSegCarryStride + c0);
+                }
+#pragma unroll
+                for (uint32_t t = 0; t < 8; ++t) seg_add(s, q0 + t < j ? v[t] : nz);
+            }
             seg_add(o, s);
             seg_st(out + (size_t)r * S + c0, o);
         }

@@ -214,19 +214,20 @@ __global__ void k_scale(float *__restrict__ A, uint16_t *__restrict__ Abf, uint3
     scale_body(A, Abf, I, R, S, norms, blockIdx.x, gridDim.x);
 }
 
-// One launch in place of k_col_final, k_scale and k_gram_partial: every
-// block first forms all R column norms from k_col_partial's ncb partials
-// (thread t sums partials t / R, t / R + 256 / R, ... of column t % R, then
-// the column's sums are added in thread order: a fixed order, all loads
-// issued at once; block 0 also stores lambda and norms), then scales its
-// rows of A (the rows gram_partial_body gives it; FP64 divide, FP32 store,
-// BF16 copy of every column, as scale_body) and sums the Gram partials of
-// the scaled rows (as gram_partial_body).
-__global__ __launch_bounds__(256) void k_norm_scale_gram(float *__restrict__ A, uint16_t *__restrict__ Abf, uint32_t I,
-                                                         uint32_t R, uint32_t S, int kind,
-                                                         const double *__restrict__ colpart, uint32_t ncb,
-                                                         double *__restrict__ lambda, double *__restrict__ norms,
-                                                         double *__restrict__ gpart)
+// One launch (single request: k_norm_scale_gram; batched: k_norm_scale_gram_b)
+// in place of k_col_final, k_scale and k_gram_partial.  Every block first
+// forms all R column norms from k_col_partial's ncb partials (thread t sums
+// partials t / R, t / R + 256 / R, ... of column t % R, then the column's
+// sums are added in thread order: a fixed order, all loads issued at once;
+// block 0 also stores lambda and norms), then scales its rows of A (the
+// rows gram_partial_body gives it; FP64 divide, FP32 store, BF16 copy of
+// every column, as scale_body) and sums the Gram partials of the scaled
+// rows (as gram_partial_body).
+__device__ __forceinline__ void norm_scale_gram_body(float *__restrict__ A, uint16_t *__restrict__ Abf, uint32_t I,
+                                                     uint32_t R, uint32_t S, int kind,
+                                                     const double *__restrict__ colpart, uint32_t ncb,
+                                                     double *__restrict__ lambda, double *__restrict__ norms,
+                                                     double *__restrict__ gpart, uint32_t blk, uint32_t nblk)
 {
     __shared__ double red[256];
     __shared__ double nrm[kMaxRank];
@@ -251,7 +252,7 @@ __global__ __launch_bounds__(256) void k_norm_scale_gram(float *__restrict__ A, 
             }
             const double n = kind == 0 ? sqrt(t) : (t < 1.0 ? 1.0 : t);
             nrm[threadIdx.x] = n;
-            if (blockIdx.x == 0) {
+            if (blk == 0) {
                 norms[threadIdx.x] = n;
                 lambda[threadIdx.x] = n;
             }
@@ -262,8 +263,8 @@ __global__ __launch_bounds__(256) void k_norm_scale_gram(float *__restrict__ A, 
     double acc[16];
 #pragma unroll
     for (int q = 0; q < 16; ++q) acc[q] = 0.0;
-    const uint64_t rows_per = ((uint64_t)I + gridDim.x - 1) / gridDim.x;
-    const uint64_t r0 = (uint64_t)blockIdx.x * rows_per;
+    const uint64_t rows_per = ((uint64_t)I + nblk - 1) / nblk;
+    const uint64_t r0 = (uint64_t)blk * rows_per;
     const uint64_t r1 = min<uint64_t>(r0 + rows_per, I);
     for (uint64_t base = r0; base < r1; base += 16) {
         for (uint32_t t = threadIdx.x; t < 16 * S; t += 256) {
@@ -301,8 +302,17 @@ __global__ __launch_bounds__(256) void k_norm_scale_gram(float *__restrict__ A, 
 #pragma unroll
     for (int q = 0; q < 16; ++q) {
         const uint32_t e = threadIdx.x + 256u * q;
-        if (e < RR) gpart[(size_t)blockIdx.x * RR + e] = acc[q];
+        if (e < RR) gpart[(size_t)blk * RR + e] = acc[q];
     }
+}
+
+__global__ __launch_bounds__(256) void k_norm_scale_gram(float *__restrict__ A, uint16_t *__restrict__ Abf, uint32_t I,
+                                                         uint32_t R, uint32_t S, int kind,
+                                                         const double *__restrict__ colpart, uint32_t ncb,
+                                                         double *__restrict__ lambda, double *__restrict__ norms,
+                                                         double *__restrict__ gpart)
+{
+    norm_scale_gram_body(A, Abf, I, R, S, kind, colpart, ncb, lambda, norms, gpart, blockIdx.x, gridDim.x);
 }
 
 // V = Hadamard of ata[p] for p = (m+k) % N, k = 1..N-1, and its inverse in
@@ -940,7 +950,7 @@ static unsigned solve_blocks(uint32_t I, uint32_t R)
 // phase and runs the same body as the single-request kernel with its block
 // index within the request.
 // ---------------------------------------------------------------------------
-enum { kPhSolve = 0, kPhColPartial, kPhColFinal, kPhScale, kPhGramPartial, kPhGramSum, kPhInner, kPhases };
+enum { kPhSolve = 0, kPhColPartial, kPhGramPartial, kPhGramSum, kPhInner, kPhases };
 
 struct AlsStepDesc {
     const float *M;            // MTTKRP output, I x S
@@ -955,7 +965,7 @@ struct AlsStepDesc {
     double *fit;
     double *host_fit;          // the job's pinned slot: the fit is also stored there
     uint32_t I, R, S, N, m, norm_kind, end_iter, pad;
-    uint32_t nb_col, nb_scale, nb_gram, nb_inner;
+    uint32_t nb_col, nb_gram, nb_inner, pad1;
     uint32_t blk[kPhases];     // first block in each phase's grid (solve: within the rank class)
     uint32_t pad2;
 };
@@ -985,28 +995,23 @@ __global__ __launch_bounds__(256) void k_solve_b(const AlsStepDesc *__restrict__
     solve_rows_body<RM>(q.M, q.A, q.I, q.R, q.S, q.fac, q.kind, blockIdx.x - q.blk[kPhSolve]);
 }
 
+// Column partials beside the Gram partials of the request's workspace.
+__device__ __forceinline__ double *als_colpart(const AlsStepDesc &q)
+{
+    return q.partial + (size_t)kRedBlocks * q.R * q.R;
+}
+
 __global__ __launch_bounds__(256) void k_col_partial_b(const AlsStepDesc *__restrict__ d, uint32_t n)
 {
     const AlsStepDesc &q = als_find(d, n, kPhColPartial, blockIdx.x);
-    col_partial_body(q.A, q.I, q.R, q.S, (int)q.norm_kind, q.partial, blockIdx.x - q.blk[kPhColPartial], q.nb_col);
+    col_partial_body(q.A, q.I, q.R, q.S, (int)q.norm_kind, als_colpart(q), blockIdx.x - q.blk[kPhColPartial], q.nb_col);
 }
 
-__global__ __launch_bounds__(256) void k_col_final_b(const AlsStepDesc *__restrict__ d, uint32_t n)
-{
-    const AlsStepDesc &q = als_find(d, n, kPhColFinal, blockIdx.x);
-    col_final_body(q.partial, q.nb_col, q.R, (int)q.norm_kind, 0, q.lambda, q.norms, blockIdx.x - q.blk[kPhColFinal]);
-}
-
-__global__ __launch_bounds__(256) void k_scale_b(const AlsStepDesc *__restrict__ d, uint32_t n)
-{
-    const AlsStepDesc &q = als_find(d, n, kPhScale, blockIdx.x);
-    scale_body(q.A, q.Abf, q.I, q.R, q.S, q.norms, blockIdx.x - q.blk[kPhScale], q.nb_scale);
-}
-
-__global__ __launch_bounds__(256) void k_gram_partial_b(const AlsStepDesc *__restrict__ d, uint32_t n)
+__global__ __launch_bounds__(256) void k_norm_scale_gram_b(const AlsStepDesc *__restrict__ d, uint32_t n)
 {
     const AlsStepDesc &q = als_find(d, n, kPhGramPartial, blockIdx.x);
-    gram_partial_body(q.A, q.I, q.R, q.S, q.partial, blockIdx.x - q.blk[kPhGramPartial], q.nb_gram);
+    norm_scale_gram_body(q.A, q.Abf, q.I, q.R, q.S, (int)q.norm_kind, als_colpart(q), q.nb_col, q.lambda, q.norms,
+                         q.partial, blockIdx.x - q.blk[kPhGramPartial], q.nb_gram);
 }
 
 __global__ __launch_bounds__(256) void k_gram_sum_b(const AlsStepDesc *__restrict__ d, uint32_t n)
@@ -1134,7 +1139,6 @@ int als_step_batch(const std::vector<CpdJob *> &jobs, AlsBatchBuffers &bb, hipSt
             q.norm_kind = j->it == 0 ? 0 : 1;
             q.end_iter = m + 1 == j->N ? 1 : 0;
             q.nb_col = red_blocks(I, 64);
-            q.nb_scale = grid_rows((uint64_t)I * j->S, 256);
             q.nb_gram = red_blocks(I, 32);
             q.nb_inner = q.end_iter ? red_blocks(I, 256) : 0;
             q.blk[kPhSolve] = sb;
@@ -1146,8 +1150,7 @@ int als_step_batch(const std::vector<CpdJob *> &jobs, AlsBatchBuffers &bb, hipSt
     const uint32_t n = (uint32_t)d.size();
     uint64_t tot[kPhases] = {};
     for (AlsStepDesc &q : d) {
-        const uint32_t nb[kPhases] = {0, q.nb_col, sum_grid(q.R), q.nb_scale, q.nb_gram, sum_grid(q.R * q.R),
-                                      q.nb_inner};
+        const uint32_t nb[kPhases] = {0, q.nb_col, q.nb_gram, sum_grid(q.R * q.R), q.nb_inner};
         for (int p = kPhColPartial; p < kPhases; ++p) {
             q.blk[p] = (uint32_t)tot[p];
             tot[p] += nb[p];
@@ -1189,9 +1192,7 @@ int als_step_batch(const std::vector<CpdJob *> &jobs, AlsBatchBuffers &bb, hipSt
         }
     }
     hipLaunchKernelGGL(k_col_partial_b, dim3((unsigned)tot[kPhColPartial]), dim3(256), 0, s, bb.dev, n);
-    hipLaunchKernelGGL(k_col_final_b, dim3((unsigned)tot[kPhColFinal]), dim3(256), 0, s, bb.dev, n);
-    hipLaunchKernelGGL(k_scale_b, dim3((unsigned)tot[kPhScale]), dim3(256), 0, s, bb.dev, n);
-    hipLaunchKernelGGL(k_gram_partial_b, dim3((unsigned)tot[kPhGramPartial]), dim3(256), 0, s, bb.dev, n);
+    hipLaunchKernelGGL(k_norm_scale_gram_b, dim3((unsigned)tot[kPhGramPartial]), dim3(256), 0, s, bb.dev, n);
     hipLaunchKernelGGL(k_gram_sum_b, dim3((unsigned)tot[kPhGramSum]), dim3(256), 0, s, bb.dev, n);
     if (tot[kPhInner]) {
         hipLaunchKernelGGL(k_inner_b, dim3((unsigned)tot[kPhInner]), dim3(256), 0, s, bb.dev, n);

@@ -55,6 +55,7 @@ struct CpdJob {
     int singular_solves = 0;              // mode updates whose normal equations were exactly singular
     bool done = false, fit_pending = false;
     bool exact_fit = false;               // final fit with <X,K> by an FP64 pass over the nonzeros
+    bool factored = false;                // the current mode's MTTKRP launch also ran its factor step
     double *h_fit = nullptr;              // pinned slot: [0] fit, [1 .. R] final lambda,
                                           // [1 + kMaxRank] singular-solve count (an int)
     std::chrono::steady_clock::time_point t_iter;

@@ -128,6 +128,17 @@ struct SegView {
     void release();
 };
 
+// The ALS factor step of one mode (k_factor's arguments: the Grams, the
+// inverse, pivots and kind), which a 4-mode MTTKRP launch can run in one
+// wave of its first block, beside its ranges.
+struct AlsFactorReq {
+    const double *ata;
+    double *fac;
+    int *piv;
+    int *kind;
+    uint32_t N, m, R, pad;
+};
+
 // Pack a mode-sorted SoA copy (inds[0..N-1], vals) into v (allocates).
 // With perm, inds and vals are in any order and nonzero perm[z] becomes
 // record z; sorted_rows[z] is then its mode index, and copy_vals must be set.
@@ -158,7 +169,8 @@ int seg_lane_class(uint32_t glpn, int dtype);
 uint32_t seg_chg(uint64_t nnz, uint32_t G, bool l2_factors);
 // out[0 .. nrows*stride) = MTTKRP of the view (no fill needed beforehand).
 int seg_mttkrp(SegView &v, const void *const *mats, const uint32_t *order, uint32_t R, uint32_t stride,
-               int dtype, float *out, hipStream_t s);
+               int dtype, float *out, hipStream_t s,
+               const AlsFactorReq *fr = nullptr, bool *factored = nullptr);
 // The same kernels with the carry space and work counter in a fixed device
 // array (no allocation; at most 65536 ranges; overlapping calls on one GPU race).
 int seg_mttkrp_noalloc(const SegView &v, const void *const *mats, const uint32_t *order, uint32_t R,
@@ -250,8 +262,10 @@ int hicoo_prepare(sptolDevHicoo h, uint32_t mode, hipStream_t s);
 // with check_vals, values) are checked on the device.  Synchronises s.
 int coo_load_binary(sptolDevCoo *h, const char *path, hipStream_t s, bool check_vals = false);
 // Device MTTKRP on stream s (sptolDevMTTKRP without the argument checks).
+// With fr, a 4-mode launch of the segmented kernel also runs that factor
+// step (*factored says whether it did).
 int coo_mttkrp(sptolDevCoo X, const sptolDevMatrix *mats, const uint32_t *order, uint32_t mode,
-               sptolDevMatrix *out, hipStream_t s);
+               sptolDevMatrix *out, hipStream_t s, const AlsFactorReq *fr = nullptr, bool *factored = nullptr);
 
 // One MTTKRP of a batch (device pointers; shared by the batch API and the server).
 struct BatchDesc {

@@ -804,13 +804,15 @@ static int check_dev_mats(uint32_t nmodes, const uint32_t *ndims, const sptolDev
 }  // extern "C"
 
 int sptol::coo_mttkrp(sptolDevCoo X, const sptolDevMatrix *mats, const uint32_t *mats_order, uint32_t mode,
-                      sptolDevMatrix *out, hipStream_t s)
+                      sptolDevMatrix *out, hipStream_t s, const AlsFactorReq *fr, bool *factored)
 {
+    if (factored) *factored = false;
     SortedCopy &sc = X->sorted[mode];
     if (sc.ready && sc.seg.ready && seg_ok(X->nmodes, out->stride)) {
         const void *mp[kMaxModes];
         for (uint32_t m = 0; m < X->nmodes; ++m) mp[m] = mats[m].values;
-        return seg_mttkrp(sc.seg, mp, mats_order, out->ncols, out->stride, mats[0].dtype, (float *)out->values, s);
+        return seg_mttkrp(sc.seg, mp, mats_order, out->ncols, out->stride, mats[0].dtype, (float *)out->values, s,
+                          fr, factored);
     }
     MttkrpArgs a;
     std::memset(&a, 0, sizeof(a));

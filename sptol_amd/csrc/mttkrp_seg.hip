@@ -11,6 +11,7 @@
 // Calls on one view must be ordered (one stream): they share the view's
 // counter and carry buffers.
 #include "seg_kernel.hpp"
+#include "als_factor.hpp"
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -90,6 +91,12 @@ constexpr bool seg_lds_class() { return NM == 3 || !(sizeof(FT) == 2 && LPN == 1
 template <int LPN, int NM, typename FT, int LK>
 constexpr int seg_waves_lk() { return (NM == 3 && LK) ? 2 : seg_waves<LPN, NM, FT, LK>(); }
 
+// 4-mode classes of at most 32 columns can run the ALS factor step of their
+// mode (wider ones leave it to k_factor: its 64-column form would weigh on
+// their registers and build time).
+template <int LPN, int NM, typename FT>
+constexpr bool seg_fused_factor() { return NM == 4 && seg_cols<FT>() * LPN <= 32; }
+
 template <int LPN, int NM, typename FT, bool NT = false, int LK = 0>
 __global__ __launch_bounds__((seg_wpb<NM, LK>() * 64)) __attribute__((amdgpu_waves_per_eu(seg_waves_lk<LPN, NM, FT, LK>(), 8)))
 void k_mttkrp_seg(const SegArgs a, uint32_t *__restrict__ counter)
@@ -105,6 +112,13 @@ void k_mttkrp_seg(const SegArgs a, uint32_t *__restrict__ counter)
         for (uint32_t i = threadIdx.x; i < n16; i += WPB * 64)
             reinterpret_cast<uint4 *>(sfac)[i] = reinterpret_cast<const uint4 *>(src)[i];
         __syncthreads();
+    }
+    if constexpr (seg_fused_factor<LPN, NM, FT>()) {
+        // the ALS factor step of this mode (it needs only the other modes'
+        // Grams), in the first wave of block 0 before its ranges: the
+        // persistent grid's other waves take the ranges meanwhile
+        if (a.fj_on && blockIdx.x == 0 && threadIdx.x < 64)
+            factor_body<seg_cols<FT>() * LPN>(a.fj.ata, a.fj.N, a.fj.m, a.fj.R, a.fj.fac, a.fj.piv, a.fj.kind);
     }
     // the wave index is uniform across the wave: read from lane 0, the
     // range's indices, bounds and arguments live in scalar registers
@@ -691,8 +705,9 @@ int seg_mttkrp_noalloc(const SegView &v, const void *const *mats, const uint32_t
 }
 
 int seg_mttkrp(SegView &v, const void *const *mats, const uint32_t *order, uint32_t R, uint32_t stride, int dtype,
-               float *out, hipStream_t s)
+               float *out, hipStream_t s, const AlsFactorReq *fr, bool *factored)
 {
+    if (factored) *factored = false;
     SegArgs a;
     // bytes of the factor rows the launch gathers from
     uint64_t fbytes = 0;
@@ -710,6 +725,12 @@ int seg_mttkrp(SegView &v, const void *const *mats, const uint32_t *order, uint3
         a.rows_per_wave = v.rr_rows;
     }
     SPTOL_TRY(seg_carry_space(v, a));
+    if (fr && v.nmodes == 4 && fr->R <= (dtype == SPTOL_BF16 ? 8u : 4u) * a.glpn &&
+        (dtype == SPTOL_BF16 ? 8u : 4u) * a.glpn <= 32u) {   // the classes of seg_fused_factor
+        a.fj = *fr;
+        a.fj_on = 1;
+        if (factored) *factored = true;
+    }
     int tok = prof_start("mttkrp_seg", s);
     if (dtype == SPTOL_BF16) seg_launch_ft<uint16_t>(a, v.counter, s, nt);
     else seg_launch_ft<float>(a, v.counter, s, nt);

@@ -62,6 +62,8 @@ struct SegArgs {
     const uint64_t *zstart;  // null: range w is nonzeros [w*G*chg, (w+1)*G*chg); else [zstart[w], zstart[w+1])
     uint32_t lk;             // 1..3: factor A1/A2/A3 is read from a copy in LDS (lrows rows); 0: none
     uint32_t lrows;
+    AlsFactorReq fj;         // with fj_on (4-mode launches): the first wave of block 0 also runs this
+    uint32_t fj_on;          //   ALS factor step before taking ranges
 };
 
 // Largest factor the range kernels copy to LDS (blocks of 8 waves).

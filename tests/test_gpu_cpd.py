@@ -383,28 +383,26 @@ def test_solve_normals_cholesky_lu_and_singular(gpu_lib):
         assert np.min(np.linalg.eigvalsh(want_V)) < 0
 
 
-def test_cpd_singular_normal_equations(gpu_lib):
+@pytest.mark.parametrize("dims", [[16, 12, 10], [16, 12, 10, 8]])
+def test_cpd_singular_normal_equations(gpu_lib, dims):
     """Initial factors with two equal columns of small dyadic values make the
     first mode update's V exactly singular (Cholesky meets an exact zero,
     LU an exact zero pivot): the solve reports it in singular_solves and
-    the run stays finite."""
-    d, inds, vals = gen.uniform_tensor([16, 12, 10], 800, seed=61)
+    the run stays finite.  The 4-D case forms V^-1 inside the MTTKRP launch
+    (the 3-D one in its own kernel)."""
+    d, inds, vals = gen.uniform_tensor(dims, 800, seed=61)
     R = 4
     rng = np.random.default_rng(62)
     init = []
     for n in d:
         a = (rng.integers(1, 5, size=(int(n), R)) * 0.5).astype(np.float32)
+        # column 0: n - 1 ones and (n - 2) / 2, whose squares sum to (n / 2)^2,
+        # a perfect square, so its norm is exact; column 1 repeats it
+        a[:, 0] = 1.0
+        a[0, 0] = (int(n) - 2) / 2
         a[:, 1] = a[:, 0]
+        assert np.sqrt((a[:, 0].astype(np.float64) ** 2).sum()) == int(n) / 2
         init.append(a)
-    # column 0 of each Gram: a perfect square, so sqrt is exact
-    for a in init:
-        s = float((a[:, 0].astype(np.float64) ** 2).sum())
-        k = 0
-        while np.sqrt(s) != np.floor(np.sqrt(s)) or np.sqrt(s) == 0:
-            a[k % len(a), 0] += 0.5
-            a[k % len(a), 1] = a[k % len(a), 0]
-            s = float((a[:, 0].astype(np.float64) ** 2).sum())
-            k += 1
     A, lam, st = _device_cpd(gpu_lib, d, inds, vals, init, 2, -1.0, SPTOL_F32)
     _say(f"  singular solves {st.singular_solves}, fit {st.fit:.6f}")
     assert st.singular_solves >= 1

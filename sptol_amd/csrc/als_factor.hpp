@@ -111,10 +111,10 @@ __device__ __forceinline__ void factor_body(const double *__restrict__ ata, uint
             chol_wave_sync();
             // the whole pivot row is read at once, its latency overlapping
             // the reciprocal of the pivot
-            double pv[RM];
+            double prow[RM];
 #pragma unroll
-            for (int k = 0; k < RM; ++k) pv[k] = pr[k];
-            const double p = pv[j];
+            for (int k = 0; k < RM; ++k) prow[k] = pr[k];
+            const double p = prow[j];
             spd = __builtin_amdgcn_readfirstlane((int)(p > 0.0)) != 0;
             if (spd) {
                 const double rp = 1.0 / p;
@@ -126,7 +126,7 @@ __device__ __forceinline__ void factor_body(const double *__restrict__ ata, uint
                     // entries are independent, so they issue back to back
                     const double f = row[j] * rp;
 #pragma unroll
-                    for (int k = 0; k < RM; ++k) row[k] = k == j ? -f : __builtin_fma(-f, pv[k], row[k]);
+                    for (int k = 0; k < RM; ++k) row[k] = k == j ? -f : __builtin_fma(-f, prow[k], row[k]);
                 }
             }
         }

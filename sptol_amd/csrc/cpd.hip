@@ -664,7 +664,16 @@ __global__ __launch_bounds__(256) void k_sqnorm_partial(const float *__restrict_
     const uint64_t per = (n + gridDim.x - 1) / gridDim.x;
     const uint64_t a = (uint64_t)blockIdx.x * per, b = min<uint64_t>(a + per, n);
     double s = 0.0;
-    for (uint64_t i = a + threadIdx.x; i < b; i += 256) s += (double)v[i] * (double)v[i];
+    uint64_t i = a + threadIdx.x;
+    // 8 loads in flight, added in the same order as one at a time
+    for (; i + 7 * 256 < b; i += 8 * 256) {
+        float x[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) x[k] = v[i + (uint64_t)k * 256];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) s += (double)x[k] * (double)x[k];
+    }
+    for (; i < b; i += 256) s += (double)v[i] * (double)v[i];
     red[threadIdx.x] = s;
     __syncthreads();
     for (uint32_t w = 128; w > 0; w >>= 1) {

@@ -60,6 +60,15 @@ constexpr size_t kPinnedChunkBytes = (size_t)16 << 20;
 int pinned_chunk(void **p);
 void pinned_chunk_free(void *p);
 
+// dst (pageable host memory) = src (device) [0, n), through two pinned chunks
+// in turn: the copy out of each chunk runs on the host's OpenMP threads
+// while the next chunk is in flight.  Synchronises s.
+int download_parallel(void *dst, const void *src, size_t n, hipStream_t s);
+// dst (device) = src (pageable host memory) [0, n), the other way round:
+// each chunk is filled on the host's threads while the previous one is in
+// flight.  Returns once every copy has completed.
+int upload_parallel(void *dst, const void *src, size_t n, hipStream_t s);
+
 // Page-locked host slot of kPinnedSlotBytes (process-lifetime slabs).
 constexpr size_t kPinnedSlotBytes = 1024;
 int pinned_slot(void **p);

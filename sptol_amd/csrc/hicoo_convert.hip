@@ -270,10 +270,10 @@ int sptolSparseTensorToHiCOODevice(sptSparseTensorHiCOO *hitsr, sptNnzIndex *max
     float *vin = nullptr, *vsc = nullptr;
     for (uint32_t m = 0; m < N; ++m) {
         HC_TRY(bufs.get(&in[m], nnz, mod));
-        HC_HIP(hipMemcpyAsync(in[m], tsr->inds[m].data, nnz * 4, hipMemcpyHostToDevice, s));
+        HC_TRY(upload_parallel(in[m], tsr->inds[m].data, nnz * 4, s));
     }
     HC_TRY(bufs.get(&vin, nnz, mod));
-    HC_HIP(hipMemcpyAsync(vin, tsr->values.data, nnz * 4, hipMemcpyHostToDevice, s));
+    HC_TRY(upload_parallel(vin, tsr->values.data, nnz * 4, s));
     HC_TRY(bufs.get(&ndims_d, N, mod));
     HC_HIP(hipMemcpyAsync(ndims_d, tsr->ndims, N * 4, hipMemcpyHostToDevice, s));
     HC_TRY(bufs.get(&bad, 1, mod));
@@ -439,19 +439,20 @@ int sptolSparseTensorToHiCOODevice(sptSparseTensorHiCOO *hitsr, sptNnzIndex *max
         if ((r = replace_vec(&hitsr->binds[m].data, &hitsr->binds[m].len, &hitsr->binds[m].cap, nb))) return fail(r);
         if ((r = replace_vec(&hitsr->einds[m].data, &hitsr->einds[m].len, &hitsr->einds[m].cap, nnz))) return fail(r);
     }
-    hipError_t e = hipMemcpyAsync(hitsr->kptr.data, kptr, (nk + 1) * 8, hipMemcpyDeviceToHost, s);
-    if (e == hipSuccess) e = hipMemcpyAsync(hitsr->bptr.data, bptr, (nb + 1) * 8, hipMemcpyDeviceToHost, s);
-    if (e == hipSuccess) e = hipMemcpyAsync(hitsr->cptr.data, cptr, (ncp + 1) * 8, hipMemcpyDeviceToHost, s);
-    if (e == hipSuccess) e = hipMemcpyAsync(hitsr->values.data, vsc, nnz * 4, hipMemcpyDeviceToHost, s);
-    for (uint32_t m = 0; m < N && e == hipSuccess; ++m) {
-        e = hipMemcpyAsync(hitsr->binds[m].data, binds[m], nb * 4, hipMemcpyDeviceToHost, s);
-        if (e == hipSuccess) e = hipMemcpyAsync(hitsr->einds[m].data, einds[m], nnz, hipMemcpyDeviceToHost, s);
+    // the host arrays are fresh (their pages not yet touched): each comes
+    // down through pinned chunks, copied out on the host's threads
+    r = download_parallel(hitsr->kptr.data, kptr, (nk + 1) * 8, s);
+    if (!r) r = download_parallel(hitsr->bptr.data, bptr, (nb + 1) * 8, s);
+    if (!r) r = download_parallel(hitsr->cptr.data, cptr, (ncp + 1) * 8, s);
+    if (!r) r = download_parallel(hitsr->values.data, vsc, nnz * 4, s);
+    for (uint32_t m = 0; m < N && !r; ++m) {
+        r = download_parallel(hitsr->binds[m].data, binds[m], nb * 4, s);
+        if (!r) r = download_parallel(hitsr->einds[m].data, einds[m], nnz, s);
         // the caller's tensor, permuted into HiCOO order
-        if (e == hipSuccess) e = hipMemcpyAsync(tsr->inds[m].data, sc[m], nnz * 4, hipMemcpyDeviceToHost, s);
+        if (!r) r = download_parallel(tsr->inds[m].data, sc[m], nnz * 4, s);
     }
-    if (e == hipSuccess) e = hipMemcpyAsync(tsr->values.data, vsc, nnz * 4, hipMemcpyDeviceToHost, s);
-    if (e == hipSuccess) e = hipStreamSynchronize(s);
-    if (e != hipSuccess) { fail(0); SPTOL_HIP(e, mod); }
+    if (!r) r = download_parallel(tsr->values.data, vsc, nnz * 4, s);
+    if (r) return fail(r);
     for (uint32_t m = 0; m < N; ++m) tsr->sortorder[m] = m;
     for (uint32_t m = 0; m < N; ++m) {
         const uint64_t kd = ((uint64_t)tsr->ndims[m] + (1ull << sk_bits) - 1) >> sk_bits;

@@ -180,6 +180,84 @@ void pinned_chunk_free(void *p)
     g_chunk_free.push_back(p);
 }
 
+int download_parallel(void *dst, const void *src, size_t n, hipStream_t s)
+{
+    if (!n) return 0;
+    if (n <= 2 * kPinnedChunkBytes) {   // small: one runtime copy
+        SPTOL_HIP(hipMemcpyAsync(dst, src, n, hipMemcpyDeviceToHost, s), "Download");
+        SPTOL_HIP(hipStreamSynchronize(s), "Download");
+        return 0;
+    }
+    void *pin[2] = {nullptr, nullptr};
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    int r = pinned_chunk(&pin[0]);
+    if (!r) r = pinned_chunk(&pin[1]);
+    for (int k = 0; k < 2 && !r; ++k)
+        if (hipEventCreateWithFlags(&ev[k], hipEventDisableTiming) != hipSuccess) r = SPTERR_CUDA_ERROR;
+    const size_t C = kPinnedChunkBytes, nc = (n + C - 1) / C;
+    auto issue = [&](size_t i) -> int {
+        const size_t off = i * C, len = n - off < C ? n - off : C;
+        SPTOL_HIP(hipMemcpyAsync(pin[i & 1], (const uint8_t *)src + off, len, hipMemcpyDeviceToHost, s), "Download");
+        SPTOL_HIP(hipEventRecord(ev[i & 1], s), "Download");
+        return 0;
+    };
+    if (!r) r = issue(0);
+    for (size_t i = 0; i < nc && !r; ++i) {
+        // chunk i + 1 goes to the other buffer, whose copy-out (chunk i - 1)
+        // finished in the previous iteration
+        if (i + 1 < nc) r = issue(i + 1);
+        if (!r && hipEventSynchronize(ev[i & 1]) != hipSuccess) r = SPTERR_CUDA_ERROR;
+        if (!r) {
+            const size_t off = i * C, len = n - off < C ? n - off : C;
+            sptol_parallel_memcpy((uint8_t *)dst + off, pin[i & 1], len);
+        }
+    }
+    if (r) (void)hip_quiet([&] { return hipStreamSynchronize(s); });
+    for (int k = 0; k < 2; ++k) {
+        if (ev[k]) (void)hipEventDestroy(ev[k]);
+        pinned_chunk_free(pin[k]);
+    }
+    if (r == SPTERR_CUDA_ERROR) SPTOL_CHECK(SPTERR_CUDA_ERROR, "Download", "device-to-host copy failed");
+    return r;
+}
+
+int upload_parallel(void *dst, const void *src, size_t n, hipStream_t s)
+{
+    if (!n) return 0;
+    if (n <= 2 * kPinnedChunkBytes) {   // small: one runtime copy
+        SPTOL_HIP(hipMemcpyAsync(dst, src, n, hipMemcpyHostToDevice, s), "Upload");
+        SPTOL_HIP(hipStreamSynchronize(s), "Upload");
+        return 0;
+    }
+    void *pin[2] = {nullptr, nullptr};
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    int r = pinned_chunk(&pin[0]);
+    if (!r) r = pinned_chunk(&pin[1]);
+    for (int k = 0; k < 2 && !r; ++k)
+        if (hipEventCreateWithFlags(&ev[k], hipEventDisableTiming) != hipSuccess) r = SPTERR_CUDA_ERROR;
+    const size_t C = kPinnedChunkBytes, nc = (n + C - 1) / C;
+    bool used[2] = {false, false};
+    for (size_t i = 0; i < nc && !r; ++i) {
+        const int k = (int)(i & 1);
+        // the buffer's previous chunk (i - 2) must have left it
+        if (used[k] && hipEventSynchronize(ev[k]) != hipSuccess) { r = SPTERR_CUDA_ERROR; break; }
+        const size_t off = i * C, len = n - off < C ? n - off : C;
+        sptol_parallel_memcpy(pin[k], (const uint8_t *)src + off, len);
+        if (hipMemcpyAsync((uint8_t *)dst + off, pin[k], len, hipMemcpyHostToDevice, s) != hipSuccess ||
+            hipEventRecord(ev[k], s) != hipSuccess)
+            r = SPTERR_CUDA_ERROR;
+        used[k] = true;
+    }
+    // the chunks go back to the pool only once their copies have run
+    if (hip_quiet([&] { return hipStreamSynchronize(s); }) != hipSuccess && !r) r = SPTERR_CUDA_ERROR;
+    for (int k = 0; k < 2; ++k) {
+        if (ev[k]) (void)hipEventDestroy(ev[k]);
+        pinned_chunk_free(pin[k]);
+    }
+    if (r == SPTERR_CUDA_ERROR) SPTOL_CHECK(SPTERR_CUDA_ERROR, "Upload", "host-to-device copy failed");
+    return r;
+}
+
 int PinnedStage::upload(void *dev, const void *host, size_t n, hipStream_t s)
 {
     if (!n) return 0;

@@ -46,6 +46,10 @@ float    sptol_rng_uniform01(void);
  * Returns 0 or an OS error code. */
 int sptol_radix_sort_u64(uint64_t *keys, uint64_t *idx, uint64_t n, unsigned bits, int nthreads);
 
+/* dst[0, n) = src[0, n), the copy split in 1 MiB pieces over the OpenMP
+ * threads (so a fresh destination's pages are first touched in parallel). */
+void sptol_parallel_memcpy(void *dst, const void *src, size_t n);
+
 /* Apply perm (new position i takes old element perm[i]) to tensor rows
  * [begin, begin+n) of a COO tensor's index and value arrays. */
 int sptol_permute_coo(sptSparseTensor *tsr, uint64_t begin, const uint64_t *perm, uint64_t n);

@@ -101,6 +101,22 @@ int sptol_radix_sort_u64(uint64_t *keys, uint64_t *idx, uint64_t n, unsigned bit
     return 0;
 }
 
+void sptol_parallel_memcpy(void *dst, const void *src, size_t n)
+{
+    const size_t piece = (size_t)1 << 20;
+    const int64_t np = (int64_t)((n + piece - 1) / piece);
+    if (np <= 1) {
+        if (n) memcpy(dst, src, n);
+        return;
+    }
+    #pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < np; ++i) {
+        const size_t off = (size_t)i * piece;
+        const size_t len = n - off < piece ? n - off : piece;
+        memcpy((char *)dst + off, (const char *)src + off, len);
+    }
+}
+
 int sptol_permute_coo(sptSparseTensor *tsr, uint64_t begin, const uint64_t *perm, uint64_t n)
 {
     if (n == 0) return 0;

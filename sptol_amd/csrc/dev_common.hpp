@@ -65,9 +65,11 @@ void pinned_chunk_free(void *p);
 // while the next chunk is in flight.  Synchronises s.
 int download_parallel(void *dst, const void *src, size_t n, hipStream_t s);
 // dst (device) = src (pageable host memory) [0, n), the other way round:
-// each chunk is filled on the host's threads while the previous one is in
-// flight.  Returns once every copy has completed.
-int upload_parallel(void *dst, const void *src, size_t n, hipStream_t s);
+// each pinned chunk is filled on the host's OpenMP threads while the other
+// is in flight, and each calling thread streams on its own (the runtime's
+// pageable path stages through shared buffers).  Synchronises s; returns 0
+// or SPTERR_CUDA_ERROR + the HIP error.
+int upload_pageable(void *dst, const void *src, size_t bytes, hipStream_t s);
 
 // Page-locked host slot of kPinnedSlotBytes (process-lifetime slabs).
 constexpr size_t kPinnedSlotBytes = 1024;

@@ -270,10 +270,10 @@ int sptolSparseTensorToHiCOODevice(sptSparseTensorHiCOO *hitsr, sptNnzIndex *max
     float *vin = nullptr, *vsc = nullptr;
     for (uint32_t m = 0; m < N; ++m) {
         HC_TRY(bufs.get(&in[m], nnz, mod));
-        HC_TRY(upload_parallel(in[m], tsr->inds[m].data, nnz * 4, s));
+        if ((r = upload_pageable(in[m], tsr->inds[m].data, nnz * 4, s))) SPTOL_CHECK(r, mod, "upload failed");
     }
     HC_TRY(bufs.get(&vin, nnz, mod));
-    HC_TRY(upload_parallel(vin, tsr->values.data, nnz * 4, s));
+    if ((r = upload_pageable(vin, tsr->values.data, nnz * 4, s))) SPTOL_CHECK(r, mod, "upload failed");
     HC_TRY(bufs.get(&ndims_d, N, mod));
     HC_HIP(hipMemcpyAsync(ndims_d, tsr->ndims, N * 4, hipMemcpyHostToDevice, s));
     HC_TRY(bufs.get(&bad, 1, mod));

@@ -221,40 +221,44 @@ int download_parallel(void *dst, const void *src, size_t n, hipStream_t s)
     return r;
 }
 
-int upload_parallel(void *dst, const void *src, size_t n, hipStream_t s)
+int upload_pageable(void *dst, const void *src, size_t bytes, hipStream_t s)
 {
-    if (!n) return 0;
-    if (n <= 2 * kPinnedChunkBytes) {   // small: one runtime copy
-        SPTOL_HIP(hipMemcpyAsync(dst, src, n, hipMemcpyHostToDevice, s), "Upload");
-        SPTOL_HIP(hipStreamSynchronize(s), "Upload");
-        return 0;
+    static const int chunked = [] {
+        const char *e = std::getenv("SPTOL_UPLOAD_CHUNKED");
+        return e ? std::atoi(e) : 1;
+    }();
+    hipError_t e = hipSuccess;
+    if (!chunked || bytes < ((size_t)4 << 20)) {
+        e = hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        return e == hipSuccess ? 0 : SPTERR_CUDA_ERROR + (int)e;
     }
-    void *pin[2] = {nullptr, nullptr};
-    hipEvent_t ev[2] = {nullptr, nullptr};
-    int r = pinned_chunk(&pin[0]);
-    if (!r) r = pinned_chunk(&pin[1]);
-    for (int k = 0; k < 2 && !r; ++k)
-        if (hipEventCreateWithFlags(&ev[k], hipEventDisableTiming) != hipSuccess) r = SPTERR_CUDA_ERROR;
-    const size_t C = kPinnedChunkBytes, nc = (n + C - 1) / C;
+    void *buf[2] = {};
+    hipEvent_t ev[2] = {};
     bool used[2] = {false, false};
-    for (size_t i = 0; i < nc && !r; ++i) {
-        const int k = (int)(i & 1);
-        // the buffer's previous chunk (i - 2) must have left it
-        if (used[k] && hipEventSynchronize(ev[k]) != hipSuccess) { r = SPTERR_CUDA_ERROR; break; }
-        const size_t off = i * C, len = n - off < C ? n - off : C;
-        sptol_parallel_memcpy(pin[k], (const uint8_t *)src + off, len);
-        if (hipMemcpyAsync((uint8_t *)dst + off, pin[k], len, hipMemcpyHostToDevice, s) != hipSuccess ||
-            hipEventRecord(ev[k], s) != hipSuccess)
-            r = SPTERR_CUDA_ERROR;
+    int r = 0;
+    for (int k = 0; k < 2 && !r; ++k) {
+        r = pinned_chunk(&buf[k]);
+        if (!r && (e = hipEventCreateWithFlags(&ev[k], hipEventDisableTiming)) != hipSuccess)
+            r = SPTERR_CUDA_ERROR + (int)e;
+    }
+    int k = 0;
+    for (size_t off = 0; off < bytes && !r; off += kPinnedChunkBytes) {
+        const size_t n = std::min(kPinnedChunkBytes, bytes - off);
+        if (used[k] && (e = hipEventSynchronize(ev[k])) != hipSuccess) { r = SPTERR_CUDA_ERROR + (int)e; break; }
+        sptol_parallel_memcpy(buf[k], (const char *)src + off, n);
+        e = hipMemcpyAsync((char *)dst + off, buf[k], n, hipMemcpyHostToDevice, s);
+        if (e == hipSuccess) e = hipEventRecord(ev[k], s);
+        if (e != hipSuccess) { r = SPTERR_CUDA_ERROR + (int)e; break; }
         used[k] = true;
+        k ^= 1;
     }
-    // the chunks go back to the pool only once their copies have run
-    if (hip_quiet([&] { return hipStreamSynchronize(s); }) != hipSuccess && !r) r = SPTERR_CUDA_ERROR;
-    for (int k = 0; k < 2; ++k) {
-        if (ev[k]) (void)hipEventDestroy(ev[k]);
-        pinned_chunk_free(pin[k]);
+    e = hipStreamSynchronize(s);
+    if (!r && e != hipSuccess) r = SPTERR_CUDA_ERROR + (int)e;
+    for (int q = 0; q < 2; ++q) {
+        pinned_chunk_free(buf[q]);
+        if (ev[q]) (void)hipEventDestroy(ev[q]);
     }
-    if (r == SPTERR_CUDA_ERROR) SPTOL_CHECK(SPTERR_CUDA_ERROR, "Upload", "host-to-device copy failed");
     return r;
 }
 

@@ -416,51 +416,6 @@ extern "C" {
 
 namespace sptol {
 
-// Pageable host memory to the device through two pinned chunks: the CPU
-// copy of one chunk overlaps the DMA of the other, and each thread streams
-// on its own (the runtime's pageable path stages through shared buffers).
-// Synchronises s.
-static int upload_pageable(void *dst, const void *src, size_t bytes, hipStream_t s)
-{
-    static const int chunked = [] {
-        const char *e = std::getenv("SPTOL_UPLOAD_CHUNKED");
-        return e ? std::atoi(e) : 1;
-    }();
-    hipError_t e = hipSuccess;
-    if (!chunked || bytes < ((size_t)4 << 20)) {
-        e = hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s);
-        if (e == hipSuccess) e = hipStreamSynchronize(s);
-        return e == hipSuccess ? 0 : SPTERR_CUDA_ERROR + (int)e;
-    }
-    void *buf[2] = {};
-    hipEvent_t ev[2] = {};
-    bool used[2] = {false, false};
-    int r = 0;
-    for (int k = 0; k < 2 && !r; ++k) {
-        r = pinned_chunk(&buf[k]);
-        if (!r && (e = hipEventCreateWithFlags(&ev[k], hipEventDisableTiming)) != hipSuccess)
-            r = SPTERR_CUDA_ERROR + (int)e;
-    }
-    int k = 0;
-    for (size_t off = 0; off < bytes && !r; off += kPinnedChunkBytes) {
-        const size_t n = std::min(kPinnedChunkBytes, bytes - off);
-        if (used[k] && (e = hipEventSynchronize(ev[k])) != hipSuccess) { r = SPTERR_CUDA_ERROR + (int)e; break; }
-        std::memcpy(buf[k], (const char *)src + off, n);
-        e = hipMemcpyAsync((char *)dst + off, buf[k], n, hipMemcpyHostToDevice, s);
-        if (e == hipSuccess) e = hipEventRecord(ev[k], s);
-        if (e != hipSuccess) { r = SPTERR_CUDA_ERROR + (int)e; break; }
-        used[k] = true;
-        k ^= 1;
-    }
-    e = hipStreamSynchronize(s);
-    if (!r && e != hipSuccess) r = SPTERR_CUDA_ERROR + (int)e;
-    for (int q = 0; q < 2; ++q) {
-        pinned_chunk_free(buf[q]);
-        if (ev[q]) (void)hipEventDestroy(ev[q]);
-    }
-    return r;
-}
-
 int dup_on(void **dst, const void *src, size_t bytes, hipStream_t s, const char *module)
 {
     SPTOL_TRY(dmalloc(dst, bytes, module));

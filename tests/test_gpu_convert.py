@@ -53,6 +53,7 @@ def test_device_converter_matches_reference_golden(gpu_lib, golden_hicoo):
 
 @pytest.mark.parametrize("dims,nnz,sb,sk,s", [
     ([12000, 9000, 28000], 200000, 7, 14, 1.1),      # the headline layout, Zipf
+    ([12000000, 9000000, 28000000], 10_000_000, 7, 14, 1.1),  # 40 MB arrays: the chunked uploads and downloads
     ([5000, 5000, 5000, 500], 150000, 4, 8, 1.05),   # 4-D
     ([3000, 2000, 2500, 400], 100000, 7, 31, 1.05),  # 4-D, 128-bit Morton key
     ([1 << 25, 20000000, 5000], 80000, 5, 28, 1.2),   # 3-D, sk above the 24-bit key

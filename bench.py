@@ -12,7 +12,7 @@ beside it.  The same line carries the served half of the metric
 (`served_cpd`): 400 requests of the served CPD-ALS stream at t=0, solves/s
 with p50/p99 latency, against the same requests through the plain
 one-at-a-time device path.  Other workloads: --workload coo | cpd | batch |
-serve | copy.
+serve | ingest | copy.
 
 Prints one JSON line: metric, value, unit, timing fields, the workload's
 config, `roofline` for the dominant kernel (algorithmic bytes per launch
@@ -596,7 +596,8 @@ def run_cpd(args, lib):
     """4-D CPD-ALS, 20 iterations, R=32: 5000x5000x5000x500, 5e7 nonzeros,
     Zipf(1.05) indices, values from a planted rank-32 model plus 1% noise;
     FP32 factors and BF16-stored factors (FP32 accumulation), FP64 solve and
-    fit.  Reports seconds per iteration and the fit of both variants."""
+    fit, then FP32 factors over the device HiCOO form of the same tensor.
+    Reports seconds per iteration and the fit of each variant."""
     dims = [5000, 5000, 5000, 500]
     nnz = int(args.nnz or 50_000_000)
     R, iters = 32, 20
@@ -627,12 +628,42 @@ def run_cpd(args, lib):
         free_factors(lib, dm)
         log(f"cpd {name}: {out[name]}")
     lib.sptolDevCooFree(h)
+    # the same ALS over the device HiCOO tensor (sb=2^7, sk=2^14), FP32
+    # factors: sptolCpdAlsHicooDevice, hicoo/cpd.c:34-135; every mode's view
+    # is prepared before the timed loop, as for the COO tensor above
+    hcoo = CooTensor(d, [a.copy() for a in inds], vals.copy())
+    t0 = time.time()
+    hh, _ = to_hicoo(lib, hcoo, 7, 14)
+    del hcoo
+    dh = C.c_void_p()
+    check(lib, lib.sptolDevHicooCreate(C.byref(dh), C.byref(hh)), "DevHicooCreate")
+    lib.sptFreeSparseTensorHiCOO(C.byref(hh))
+    for m in range(4):
+        check(lib, lib.sptolDevHicooPrepareMode(dh, m), "DevHicooPrepareMode")
+    log(f"cpd: device HiCOO tensor and views in {time.time() - t0:.1f}s")
+    dm = dev_factors(lib, init)
+    lam = np.zeros(R, np.float32)
+    st = sptolCpdStats()
+    st.factor_dtype = SPTOL_F32
+    st.exact_fit = 1
+    sync(lib)
+    t0 = time.perf_counter()
+    check(lib, lib.sptolCpdAlsHicooDevice(dh, R, iters, -1.0, dm, ptr(lam, C.c_float), C.byref(st), None),
+          "cpd hicoo")
+    sync(lib)
+    dt = time.perf_counter() - t0
+    out["hicoo_f32"] = {"seconds": dt, "s_per_iter": dt / st.iterations, "fit": st.fit,
+                        "iterations": st.iterations}
+    free_factors(lib, dm)
+    lib.sptolDevHicooFree(dh)
+    log(f"cpd hicoo_f32: {out['hicoo_f32']}")
     res = base_result(args, "CPD-ALS seconds per iteration", round(out["f32"]["s_per_iter"], 5), "s/iter",
                       round(out["f32"]["s_per_iter"] * 1e3, 4),
                       {"workload": "cpd_als_4d", "dims": dims, "nnz": nnz, "R": R, "iterations": iters,
                        "zipf_s": 1.05, "planted_rank": R, "noise": 0.01}, higher=False)
     res["variants"] = out
     res["bf16_minus_f32_fit"] = out["bf16"]["fit"] - out["f32"]["fit"]
+    res["hicoo_minus_coo_fit"] = out["hicoo_f32"]["fit"] - out["f32"]["fit"]
     res["cpu_baseline"] = None if args.no_cpu else cpu_baseline_cpd(d, inds, vals, init, R)
     if res["cpu_baseline"]:
         log(f"cpd CPU baseline: {res['cpu_baseline']}")
@@ -815,6 +846,143 @@ def run_serve(args, lib):
     return res
 
 
+def _libc():
+    libc = C.CDLL(None)
+    libc.fopen.restype = C.c_void_p
+    libc.fopen.argtypes = [C.c_char_p, C.c_char_p]
+    libc.fclose.argtypes = [C.c_void_p]
+    return libc
+
+
+def _load_tns(L, path):
+    """sptLoadSparseTensor (load.c:31) of library L on a .tns file with
+    1-based indices; returns (seconds, ndims, inds, values) as numpy copies."""
+    libc = _libc()
+    L.sptLoadSparseTensor.argtypes = [C.POINTER(sptSparseTensor), sptIndex, C.c_void_p]
+    L.sptFreeSparseTensor.argtypes = [C.POINTER(sptSparseTensor)]
+    fp = libc.fopen(path, b"r")
+    assert fp, path
+    t = sptSparseTensor()
+    t0 = time.perf_counter()
+    rc = L.sptLoadSparseTensor(C.byref(t), 1, fp)
+    dt = time.perf_counter() - t0
+    libc.fclose(fp)
+    assert rc == 0, rc
+    n, nm = int(t.nnz), int(t.nmodes)
+    d = np.ctypeslib.as_array(t.ndims, shape=(nm,)).copy()
+    inds = [np.ctypeslib.as_array(t.inds[m].data, shape=(n,)).copy() for m in range(nm)]
+    vals = np.ctypeslib.as_array(t.values.data, shape=(n,)).copy()
+    L.sptFreeSparseTensor(C.byref(t))
+    return dt, d, inds, vals
+
+
+def run_ingest(args, lib):
+    """Ingest of the 100M-nonzero Zipf(1.1) tensor 12M x 9M x 28M: one step =
+    sptolDevCooLoadBinary, which streams the binary tensor file (u32 index
+    arrays, f32 values) through pinned chunks into a device COO tensor and
+    checks every index against its dimension on the device.  The file sits
+    in the host's page cache (as after a first read).  Beside it: the host
+    binary load followed by sptolDevCooCreate; duplicate merging
+    (sptolSparseTensorMergeDuplicates) of the loaded tensor on the host
+    threads; and the .tns text reader, this library's and the compiled
+    reference's sptLoadSparseTensor (load.c:31), on a 2M-nonzero text file of
+    the same tensor, whose loaded arrays must be identical.  cpu_baseline is
+    the reference's text reader, the reference's only way in."""
+    import tempfile
+    nnz = int(args.nnz or HICOO_NNZ)
+    t0 = time.time()
+    dims, inds, vals = gen.zipf_tensor(DIMS, nnz, 1.1, seed=20260101, values="lognormal",
+                                       sort_fn=lib_sort(lib))
+    nnz = len(vals)
+    log(f"ingest: generated {nnz} nnz in {time.time() - t0:.1f}s")
+    coo = CooTensor(dims, inds, vals)
+    tmp = tempfile.mkdtemp(prefix="sptol_ingest_")
+    binpath = os.path.join(tmp, "x.bin").encode()
+    tnspath = os.path.join(tmp, "x.tns").encode()
+    res = None
+    try:
+        check(lib, lib.sptolSaveSparseTensorBinary(coo.ref(), binpath), "save binary")
+        fbytes = os.path.getsize(binpath)
+        ts = []
+        for k in range(args.warmup + args.steps):
+            h = C.c_void_p()
+            sync(lib)
+            t0 = time.perf_counter()
+            check(lib, lib.sptolDevCooLoadBinary(C.byref(h), binpath, None), "DevCooLoadBinary")
+            sync(lib)
+            if k >= args.warmup:
+                ts.append(time.perf_counter() - t0)
+            assert lib.sptolDevCooNnz(h) == nnz
+            lib.sptolDevCooFree(h)
+        dt = float(np.mean(ts))
+        log(f"ingest: file -> device {dt * 1e3:.1f} ms ({fbytes / dt / 1e9:.2f} GB/s)")
+
+        lib.sptolLoadSparseTensorBinary.argtypes = [C.POINTER(sptSparseTensor), C.c_char_p]
+        lib.sptFreeSparseTensor.argtypes = [C.POINTER(sptSparseTensor)]
+        two = []
+        for k in range(2):
+            t = sptSparseTensor()
+            t0 = time.perf_counter()
+            check(lib, lib.sptolLoadSparseTensorBinary(C.byref(t), binpath), "LoadSparseTensorBinary")
+            t1 = time.perf_counter()
+            h = C.c_void_p()
+            check(lib, lib.sptolDevCooCreate(C.byref(h), C.byref(t)), "DevCooCreate")
+            sync(lib)
+            two.append((t1 - t0, time.perf_counter() - t1))
+            lib.sptolDevCooFree(h)
+            lib.sptFreeSparseTensor(C.byref(t))
+        host_load, upload = two[-1]
+        log(f"ingest: host binary load {host_load:.3f} s, then sptolDevCooCreate {upload:.3f} s")
+
+        tk = host_threads()
+        mcoo = CooTensor(dims, [a.copy() for a in inds], vals.copy())
+        t0 = time.perf_counter()
+        check(lib, lib.sptolSparseTensorMergeDuplicates(mcoo.ref(), tk), "MergeDuplicates")
+        t_merge = time.perf_counter() - t0
+        merged = int(mcoo.st.nnz)
+        del mcoo
+        log(f"ingest: merge duplicates {t_merge:.2f} s ({nnz} -> {merged} nonzeros)")
+
+        ns = min(nnz, 2_000_000)
+        sample = CooTensor(dims, [a[:ns].copy() for a in inds], vals[:ns].copy())
+        libc = _libc()
+        lib.sptDumpSparseTensor.argtypes = [C.POINTER(sptSparseTensor), sptIndex, C.c_void_p]
+        fp = libc.fopen(tnspath, b"w")
+        check(lib, lib.sptDumpSparseTensor(sample.ref(), 1, fp), "DumpSparseTensor")
+        libc.fclose(fp)
+        t_ours, d1, i1, v1 = _load_tns(lib, tnspath)
+        text = {"nnz": ns, "file_bytes": os.path.getsize(tnspath), "this_library_s": t_ours,
+                "this_library_gnnz_s": ns / t_ours / 1e9}
+        ref = None if args.no_cpu else ref_library()
+        cpu = None
+        if ref is not None:
+            t_ref, d2, i2, v2 = _load_tns(ref, tnspath)
+            same = (np.array_equal(d1, d2) and all(np.array_equal(a, b) for a, b in zip(i1, i2))
+                    and np.array_equal(v1.view(np.uint32), v2.view(np.uint32)))
+            assert same, "the .tns readers disagree"
+            text.update({"reference_s": t_ref, "reference_gnnz_s": ns / t_ref / 1e9, "identical_arrays": same})
+            cpu = {"value": ns / t_ref / 1e9, "unit": "Gnnz/s", "cores": 1, "kind": "reference",
+                   "sample": f"the compiled reference's sptLoadSparseTensor (load.c:31, one thread) on a "
+                             f".tns text file of the first {ns} nonzeros of the tensor, {t_ref:.1f} s"}
+        log(f"ingest: .tns text {text}")
+    finally:
+        for p in (binpath, tnspath):
+            if os.path.exists(p):
+                os.remove(p)
+        os.rmdir(tmp)
+    res = base_result(args, "ingest Gnnz/s (binary file to device COO tensor)", round(nnz / dt / 1e9, 4),
+                      "Gnnz/s", round(dt * 1e3, 4),
+                      {"workload": "ingest_binary_to_device", "dims": DIMS, "nnz": nnz, "zipf_s": 1.1,
+                       "values": "lognormal(0,1)", "file_bytes": fbytes, "file": "in the page cache"},
+                      dtype="u32")
+    res["file_gb_s"] = round(fbytes / dt / 1e9, 3)
+    res["host_load_then_upload_s"] = {"host_binary_load": host_load, "sptolDevCooCreate": upload}
+    res["merge_duplicates"] = {"seconds": t_merge, "threads": tk, "nnz_in": nnz, "nnz_out": merged}
+    res["tns_text"] = text
+    res["cpu_baseline"] = cpu
+    return res
+
+
 def run_copy(args, lib):
     """Device-to-device copy of 1 GiB with hipMemcpy (the runtime's blit
     kernel): the known-size reference for scaling rocprofv3 FETCH_SIZE /
@@ -846,10 +1014,11 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="hicoo", choices=["hicoo", "coo", "cpd", "batch", "serve", "copy"],
+    ap.add_argument("--workload", default="hicoo", choices=["hicoo", "coo", "cpd", "batch", "serve", "ingest", "copy"],
                     help="hicoo: the headline HiCOO MTTKRP (default); coo: 3-D COO mode 0 (2000x1500x1000, 1e6 nnz); "
                          "cpd: 4-D CPD-ALS FP32 and BF16; batch: 256 tensors in one persistent launch; "
-                         "serve: CPD request stream through the server; copy: 1 GiB device copy (counter scale)")
+                         "serve: CPD request stream through the server; ingest: binary tensor file to the device; "
+                         "copy: 1 GiB device copy (counter scale)")
     ap.add_argument("--nnz", type=float, default=None, help="override the workload's nonzero count")
     ap.add_argument("--requests", type=int, default=2000, help="serve: number of requests")
     ap.add_argument("--rate", type=float, default=0.0, help="serve: Poisson arrival rate (req/s); 0 = all at once")
@@ -871,7 +1040,7 @@ def main():
     if args.workload == "hicoo" and args.nnz is None:
         args.nnz = 1e8
     result = {"hicoo": run_hicoo, "coo": run_coo, "cpd": run_cpd, "batch": run_batch,
-              "serve": run_serve, "copy": run_copy}[args.workload](args, lib)
+              "serve": run_serve, "ingest": run_ingest, "copy": run_copy}[args.workload](args, lib)
     print(json.dumps(result), flush=True)
 
 

@@ -126,18 +126,20 @@ int  sptolDevHicooCreate(sptolDevHicoo *h, const sptSparseTensorHiCOO *X);
  * (built once; the HiCOO arrays themselves are not changed).  Two kinds:
  *  - tiled: blocks stably ordered by their mode-th block coordinate
  *    (entries move with their blocks) plus the work list of the LDS-tiled
- *    kernel, which accumulates each 2^sb_bits x R output tile in LDS.  Pays
- *    off when blocks hold many nonzeros.  Needs sb_bits <= 8, stride <= 64.
+ *    kernel, which accumulates each 2^sb_bits x R output tile in LDS.
+ *    Needs sb_bits <= 8, stride <= 64.  Built only on request (kind 1) or
+ *    for tensors of an order the segmented kernel does not take.
  *  - segmented: coordinates rebuilt from block coordinates and element
  *    offsets, sorted by the mode, packed 16 bytes per nonzero, for the
  *    segmented kernel (one plain store per output row, no atomics,
- *    deterministic).  For 3- and 4-mode tensors with stride <= 64; chosen
- *    when blocks are sparse (fewer than 8 nonzeros per block on average),
- *    where blocking saves no index bytes.
+ *    deterministic).  For 3- and 4-mode tensors with stride <= 64; the
+ *    default for them at any block density (7-14x faster than tiled from
+ *    15 to 651 nonzeros per block).
  * Shapes that neither kind handles keep the block-order kernel. */
 int  sptolDevHicooPrepareMode(sptolDevHicoo h, sptIndex mode);
 /* View kind for later sptolDevHicooPrepareMode calls (process-wide):
- * 0 = by block density (default), 1 = tiled, 2 = segmented. */
+ * 0 = automatic (default: segmented for 3 and 4 modes, tiled otherwise),
+ * 1 = tiled, 2 = segmented. */
 #define SPTOL_HICOO_VIEW_AUTO 0
 #define SPTOL_HICOO_VIEW_TILED 1
 #define SPTOL_HICOO_VIEW_SEG 2

@@ -1150,10 +1150,10 @@ int CpdJob::mttkrp(hipStream_t s)
     order(o);
     Mout.nrows = ndims[m];
     factored = false;
-    if (H) return hicoo_mttkrp(H, mt, o, m, &Mout, s);
     // a 4-mode launch forms this mode's V^{-1} beside its ranges (the
     // factor step needs only the other modes' Grams)
     const AlsFactorReq fr = {w.ata, w.fac, w.piv, w.kind, N, m, R, 0u};
+    if (H) return hicoo_mttkrp(H, mt, o, m, &Mout, s, &fr, &factored);
     return coo_mttkrp(X, mt, o, m, &Mout, s, &fr, &factored);
 }
 

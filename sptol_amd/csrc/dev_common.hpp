@@ -266,7 +266,7 @@ int coo_prepare_all(sptolDevCoo h, hipStream_t s);
 // Device HiCOO MTTKRP on stream s (sptolDevMTTKRPHiCOO without the checks)
 // and the stream-aware form of sptolDevHicooPrepareMode.
 int hicoo_mttkrp(sptolDevHicoo X, const sptolDevMatrix *mats, const uint32_t *order, uint32_t mode,
-                 sptolDevMatrix *out, hipStream_t s);
+                 sptolDevMatrix *out, hipStream_t s, const AlsFactorReq *fr = nullptr, bool *factored = nullptr);
 int hicoo_prepare(sptolDevHicoo h, uint32_t mode, hipStream_t s);
 // Device COO tensor straight from a binary tensor file (the layout of
 // sptolSaveSparseTensorBinary), streamed through pinned chunks; indices (and
@@ -331,7 +331,8 @@ int batch_launch(std::vector<BatchDesc> &descs, int dtype, double tile_scale, in
 
 // HiCOO per-mode views (hicoo_view.hip).
 void free_hicoo_views(sptolDevHicoo h);
-int launch_hicoo_view(sptolDevHicoo X, const MttkrpArgs &a, int dtype, hipStream_t s, int *used);
+int launch_hicoo_view(sptolDevHicoo X, const MttkrpArgs &a, int dtype, hipStream_t s, int *used,
+                      const AlsFactorReq *fr = nullptr, bool *factored = nullptr);
 
 int launch_mttkrp_coo(const MttkrpArgs &a, int dtype, hipStream_t s);
 int launch_mttkrp_hicoo(const MttkrpArgs &a, int dtype, hipStream_t s);

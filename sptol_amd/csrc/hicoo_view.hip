@@ -528,7 +528,8 @@ void free_hicoo_views(sptolDevHicoo h)
 
 // Run the mode's view (segmented, or zero fill + tiled kernel); *used = 0
 // when no view applies (the caller runs the block-order kernel).
-int launch_hicoo_view(sptolDevHicoo X, const MttkrpArgs &in, int dtype, hipStream_t s, int *used)
+int launch_hicoo_view(sptolDevHicoo X, const MttkrpArgs &in, int dtype, hipStream_t s, int *used,
+                      const AlsFactorReq *fr, bool *factored)
 {
     *used = 0;
     if (!X->views || !X->views->v[in.mode].ready) return 0;
@@ -536,7 +537,7 @@ int launch_hicoo_view(sptolDevHicoo X, const MttkrpArgs &in, int dtype, hipStrea
     if (v.use_seg) {
         if (!seg_ok(in.nmodes, in.stride)) return 0;
         *used = 1;
-        return seg_mttkrp(v.seg, in.mats, in.order, in.R, in.stride, dtype, in.out, s);
+        return seg_mttkrp(v.seg, in.mats, in.order, in.R, in.stride, dtype, in.out, s, fr, factored);
     }
     if (!hicoo_tiled_ok(in.sb_bits, in.stride)) return 0;
     *used = 1;

@@ -881,8 +881,9 @@ int sptolDevMTTKRPHiCOO(sptolDevHicoo X, const sptolDevMatrix *mats, const sptIn
 }  // extern "C"
 
 int sptol::hicoo_mttkrp(sptolDevHicoo X, const sptolDevMatrix *mats, const uint32_t *mats_order, uint32_t mode,
-                        sptolDevMatrix *out, hipStream_t s)
+                        sptolDevMatrix *out, hipStream_t s, const AlsFactorReq *fr, bool *factored)
 {
+    if (factored) *factored = false;
     MttkrpArgs a;
     std::memset(&a, 0, sizeof(a));
     a.nnz = X->nnz;
@@ -902,7 +903,7 @@ int sptol::hicoo_mttkrp(sptolDevHicoo X, const sptolDevMatrix *mats, const uint3
     a.sb_bits = X->sb_bits;
     a.out = (float *)out->values;
     int used = 0;
-    SPTOL_TRY(launch_hicoo_view(X, a, mats[0].dtype, s, &used));
+    SPTOL_TRY(launch_hicoo_view(X, a, mats[0].dtype, s, &used, fr, factored));
     if (used) return 0;
     SPTOL_TRY(launch_fill_zero(a.out, (uint64_t)X->ndims[mode] * out->stride, s));
     return launch_mttkrp_hicoo(a, mats[0].dtype, s);

@@ -1220,8 +1220,18 @@ void CpdJob::fit_ready()
 // pinned slot.
 int CpdJob::finish_enqueue(hipStream_t s)
 {
+    // a HiCOO tensor's segmented view packs the same records as a COO view
+    const SegView *hv = (H && (N == 3 || N == 4)) ? hicoo_seg_records(H) : nullptr;
     if (exact_fit) {
-        if (X && N <= 4) {
+        if (hv) {
+            InnerArgs ia;
+            std::memset(&ia, 0, sizeof(ia));
+            for (uint32_t q = 0; q < N; ++q) ia.mats[q] = (const float *)mats[q].values;
+            ia.nnz = hv->nnz;
+            ia.N = N; ia.R = R; ia.S = S;
+            if (N == 3) launch_inner_exact_rec<3>(*hv, ia, w.lambda, w.partial, s);
+            else launch_inner_exact_rec<4>(*hv, ia, w.lambda, w.partial, s);
+        } else if (X && N <= 4) {
             InnerArgs ia;
             std::memset(&ia, 0, sizeof(ia));
             for (uint32_t q = 0; q < N; ++q) { ia.inds[q] = X->inds[q]; ia.mats[q] = (const float *)mats[q].values; }

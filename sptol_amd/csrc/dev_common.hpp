@@ -333,6 +333,9 @@ int batch_launch(std::vector<BatchDesc> &descs, int dtype, double tile_scale, in
 void free_hicoo_views(sptolDevHicoo h);
 int launch_hicoo_view(sptolDevHicoo X, const MttkrpArgs &a, int dtype, hipStream_t s, int *used,
                       const AlsFactorReq *fr = nullptr, bool *factored = nullptr);
+// A built segmented view of X holding all its nonzeros as packed records
+// (every mode's index in mode order), or nullptr.
+const SegView *hicoo_seg_records(sptolDevHicoo X);
 
 int launch_mttkrp_coo(const MttkrpArgs &a, int dtype, hipStream_t s);
 int launch_mttkrp_hicoo(const MttkrpArgs &a, int dtype, hipStream_t s);

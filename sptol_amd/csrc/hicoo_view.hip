@@ -518,6 +518,16 @@ int hicoo_prepare(sptolDevHicoo h, uint32_t mode, hipStream_t s)
     return build_hicoo_view(h, mode, v, s);
 }
 
+const SegView *hicoo_seg_records(sptolDevHicoo X)
+{
+    if (!X->views) return nullptr;
+    for (uint32_t q = 0; q < X->nmodes; ++q) {
+        const HicooView &v = X->views->v[q];
+        if (v.ready && v.use_seg && v.seg.ready && v.seg.rec && v.seg.nnz == X->nnz) return &v.seg;
+    }
+    return nullptr;
+}
+
 void free_hicoo_views(sptolDevHicoo h)
 {
     if (!h->views) return;

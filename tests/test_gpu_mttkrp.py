@@ -413,6 +413,34 @@ def test_hicoo_tiled_block_sizes_4d(gpu_lib, sb, sk, kind):
         gpu_lib.sptFreeSparseTensorHiCOO(C.byref(h))
 
 
+def test_hicoo_default_view_dense_blocks(gpu_lib):
+    """The default view kind is the segmented one for 3- and 4-mode tensors
+    at any block density: on tensors of dense blocks (hundreds of nonzeros
+    per 2^7 block) the default view's MTTKRP equals the segmented view's bit
+    for bit and the FP64 oracle within the kernel's tolerance; the tiled view
+    of the same tensor agrees with the oracle as well."""
+    for dims, nnz in (([600, 500, 400], 400000), ([300, 250, 200, 40], 300000)):
+        d, inds, vals = gen.uniform_tensor(dims, nnz, seed=17)
+        coo = CooTensor(d, [a.copy() for a in inds], vals.copy())
+        h, _ = to_hicoo(gpu_lib, coo, 7, 14)
+        nb = int(h.bptr.len) - 1
+        assert len(vals) >= 100 * nb, (len(vals), nb)
+        A = gen.uniform_factors(d, 16, seed=4)
+        N = len(d)
+        for mode in range(N):
+            _say("default view, dense blocks", N, mode)
+            order = [mode] + [(mode + i) % N for i in range(1, N)]
+            auto = dev_mttkrp_hicoo(gpu_lib, h, d, A, order, mode, prepare="auto")
+            seg = dev_mttkrp_hicoo(gpu_lib, h, d, A, order, mode, prepare="seg")
+            tiled = dev_mttkrp_hicoo(gpu_lib, h, d, A, order, mode, prepare="tiled")
+            assert np.array_equal(auto.view(np.uint32), seg.view(np.uint32)), mode
+            ref32 = oracle.mttkrp_coo(coo.inds, coo.vals, A, order, mode)
+            ref64 = oracle.mttkrp_coo(coo.inds, coo.vals, A, order, mode, dtype=np.float64)
+            _check(auto, ref32, ref64, 16)
+            _check(tiled, ref32, ref64, 16)
+        gpu_lib.sptFreeSparseTensorHiCOO(C.byref(h))
+
+
 def test_segmented_kernel_deterministic_and_rowwise_exact(gpu_lib):
     """The segmented kernel (sorted copy, 3-D and 4-D) is deterministic: two
     runs agree bit for bit.  Rows whose nonzeros all fall inside one lane

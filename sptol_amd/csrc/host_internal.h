@@ -50,6 +50,11 @@ int sptol_radix_sort_u64(uint64_t *keys, uint64_t *idx, uint64_t n, unsigned bit
  * threads (so a fresh destination's pages are first touched in parallel). */
 void sptol_parallel_memcpy(void *dst, const void *src, size_t n);
 
+/* dst[0, n) = bytes [off, off+n) of the file open as fd, read with pread in
+ * 1 MiB pieces over the OpenMP threads.  Returns 0, or an errno value (EIO
+ * when the file ends early). */
+int sptol_parallel_pread(int fd, void *dst, size_t n, uint64_t off);
+
 /* Apply perm (new position i takes old element perm[i]) to tensor rows
  * [begin, begin+n) of a COO tensor's index and value arrays. */
 int sptol_permute_coo(sptSparseTensor *tsr, uint64_t begin, const uint64_t *perm, uint64_t n);

@@ -14,6 +14,7 @@
 #include <stdio.h>
 #include <ctype.h>
 #include <math.h>
+#include <unistd.h>
 #ifdef _OPENMP
 #include <omp.h>
 #endif
@@ -115,6 +116,27 @@ void sptol_parallel_memcpy(void *dst, const void *src, size_t n)
         const size_t len = n - off < piece ? n - off : piece;
         memcpy((char *)dst + off, (const char *)src + off, len);
     }
+}
+
+int sptol_parallel_pread(int fd, void *dst, size_t n, uint64_t off)
+{
+    const size_t piece = (size_t)1 << 20;
+    const int64_t np = (int64_t)((n + piece - 1) / piece);
+    int err = 0;
+    #pragma omp parallel for schedule(static) reduction(max : err) if (np > 1)
+    for (int64_t i = 0; i < np; ++i) {
+        size_t done = (size_t)i * piece;
+        const size_t end = n - done < piece ? n : done + piece;
+        while (done < end) {
+            const ssize_t got = pread(fd, (char *)dst + done, end - done, (off_t)(off + done));
+            if (got <= 0) {
+                err = got < 0 && errno ? errno : EIO;
+                break;
+            }
+            done += (size_t)got;
+        }
+    }
+    return err;
 }
 
 int sptol_permute_coo(sptSparseTensor *tsr, uint64_t begin, const uint64_t *perm, uint64_t n)

@@ -519,8 +519,15 @@ int coo_load_binary(sptolDevCoo *h, const char *path, hipStream_t s, bool check_
     for (uint32_t m = 0; m < nm; ++m) t->ndims[m] = nd[m];
     for (uint32_t m = 0; m <= nm && !r; ++m)
         r = dmalloc(m < nm ? (void **)&t->inds[m] : (void **)&t->vals, nnz * 4, "DevCoo Load");
-    // every array in file order, chunk by chunk: the read of a chunk
-    // overlaps the copy of the previous one
+    // every array in file order, chunk by chunk: the read of a chunk (its
+    // pieces read in parallel) overlaps the copy of the previous one
+    const long data0 = r ? -1 : ftell(fp);
+    if (!r && data0 < 0) {
+        const int en = errno ? errno : EIO;
+        sptol_complain("DevCoo Load", SPTERR_OS_ERROR + en, __FILE__, __LINE__, strerror(en));
+        r = SPTERR_OS_ERROR + en;
+    }
+    const int fd = fileno(fp);
     void *buf[2] = {};
     hipEvent_t ev[2] = {};
     bool used[2] = {false, false};
@@ -536,9 +543,11 @@ int coo_load_binary(sptolDevCoo *h, const char *path, hipStream_t s, bool check_
         for (uint64_t off = 0; off < nnz * 4 && !r; off += kPinnedChunkBytes) {
             const size_t n = (size_t)std::min<uint64_t>(kPinnedChunkBytes, nnz * 4 - off);
             if (used[k] && (e = hipEventSynchronize(ev[k])) != hipSuccess) { r = SPTERR_CUDA_ERROR + (int)e; break; }
-            if (fread(buf[k], 1, n, fp) != n) {
-                sptol_complain("DevCoo Load", SPTERR_OS_ERROR + EIO, __FILE__, __LINE__, "short read");
-                r = SPTERR_OS_ERROR + EIO;
+            const int en = sptol_parallel_pread(fd, buf[k], n, (uint64_t)data0 + (uint64_t)m * nnz * 4 + off);
+            if (en) {
+                sptol_complain("DevCoo Load", SPTERR_OS_ERROR + en, __FILE__, __LINE__,
+                               en == EIO ? "short read" : strerror(en));
+                r = SPTERR_OS_ERROR + en;
                 break;
             }
             e = hipMemcpyAsync(dst + off, buf[k], n, hipMemcpyHostToDevice, s);
